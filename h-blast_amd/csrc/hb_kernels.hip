@@ -1,0 +1,430 @@
+// hb_kernels.hip -- gfx950 kernels of the seed-extension stage.
+//
+// Scan kernel (hb_scan_kernel): the BLASTP/BLASTX word scan, two-hit
+// diagonal tracking and X-drop ungapped extension of
+// h_blast_v1.1/H-BLAST/h_blast_kernel.cu:21-1540, restructured for CDNA4:
+//   * one wavefront owns one subject at a time; subjects come from a
+//     length-descending tile queue (one atomic per tile);
+//   * lane i owns subject position s0+i of a 64-position window; the 3-mer
+//     index is tested against the presence vector in LDS and the cell
+//     {start, count} fetched from the lookup table (L2 resident);
+//   * the window's hits are flattened in (s, q) order into an LDS list by a
+//     wavefront prefix sum, then processed 64 at a time;
+//   * the per-diagonal state {last_hit + offset | flag} lives in LDS (int32
+//     per diagonal, no 15-bit limit).  Hits of one round that fall on the
+//     same diagonal are ordered by an LDS atomic-max marker so that every
+//     diagonal sees its hits in subject order -- the exact sequential
+//     semantics of the CPU two-hit word finder (the reference kernel's
+//     deferral, h_blast_kernel.cu:873-942, is not reproduced);
+//   * X-drop extensions read the query and the score matrix from LDS.
+#include <hip/hip_runtime.h>
+
+#include "hb_internal.h"
+
+namespace {
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < HB_WAVE; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, HB_WAVE);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, HB_WAVE);
+    return v;
+}
+
+// ---------------------------------------------------------------- residues
+__global__ void check_residues_kernel(const uint8_t *res, uint64_t n, uint32_t *flag)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t bad = 0;
+    const uint64_t n4 = n / 4;
+    const uint32_t *r4 = reinterpret_cast<const uint32_t *>(res);
+    for (uint64_t k = i; k < n4; k += stride) {
+        const uint32_t w = r4[k];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bad |= (((w >> (8 * b)) & 0xffu) >= HB_ALPHA);
+    }
+    for (uint64_t k = n4 * 4 + i; k < n; k += stride) bad |= (res[k] >= HB_ALPHA);
+    if (bad) atomicOr(flag, 1u);
+}
+
+// ---------------------------------------------------------------- lookup table
+// qflag[q] = residue | 0x80 when the 3-mer at q lies inside one context.
+// Cell w = (a<<10)|(b<<5)|c holds q when M[q0][a]+M[q1][b]+M[q2][c] >= T or
+// w is the query word itself (an exact word always seeds).
+template <bool FILL>
+__global__ __launch_bounds__(256) void lut_kernel(const uint8_t *qflag, int32_t qlen,
+                                                  const int8_t *mat32, int32_t threshold,
+                                                  uint32_t *count, const uint32_t *start,
+                                                  uint32_t *entries, uint2 *cells,
+                                                  uint32_t *pv)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    int8_t *sm = reinterpret_cast<int8_t *>(lds);        // 32x32
+    uint8_t *sq = lds + 1024;                              // qlen bytes
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) sm[i] = mat32[i];
+    for (int i = threadIdx.x; i < qlen; i += blockDim.x) sq[i] = qflag[i];
+    __syncthreads();
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= HB_NCELLS) return;
+    const uint32_t a = w >> 10, b = (w >> 5) & 31, c = w & 31;
+    uint32_t n = 0;
+    if (a < HB_ALPHA && b < HB_ALPHA && c < HB_ALPHA) {
+        uint32_t k = FILL ? start[w] : 0;
+        for (int32_t q = 0; q + 3 <= qlen; ++q) {
+            const uint8_t f0 = sq[q];
+            if (!(f0 & 0x80)) continue;
+            const uint32_t q0 = f0 & 31, q1 = sq[q + 1] & 31, q2 = sq[q + 2] & 31;
+            const int32_t s = sm[(q0 << 5) | a] + sm[(q1 << 5) | b] + sm[(q2 << 5) | c];
+            const bool self = (q0 == a) & (q1 == b) & (q2 == c);
+            if (s >= threshold || self) {
+                if (FILL) entries[k++] = (uint32_t)q;
+                ++n;
+            }
+        }
+    }
+    if (!FILL) {
+        count[w] = n;
+    } else {
+        cells[w] = make_uint2(start[w], n);
+        const uint64_t m = __ballot(n != 0);
+        const int lane = threadIdx.x & 63;
+        if (lane == 0) pv[w >> 5] = (uint32_t)m;
+        if (lane == 32) pv[w >> 5] = (uint32_t)(m >> 32);
+    }
+}
+
+// Exclusive scan of HB_NCELLS counts in place; start[HB_NCELLS] = total.
+__global__ __launch_bounds__(1024) void lut_scan_kernel(uint32_t *cnt, uint32_t *total)
+{
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = HB_NCELLS / 1024;
+    uint32_t s = 0;
+    for (int i = 0; i < per; ++i) s += cnt[t * per + i];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        uint32_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (int i = 0; i < per; ++i) {
+        uint32_t v = cnt[t * per + i];
+        cnt[t * per + i] = run;
+        run += v;
+    }
+    if (t == 1023) {
+        cnt[HB_NCELLS] = run;
+        *total = run;
+    }
+}
+
+// ---------------------------------------------------------------- extensions
+// Left X-drop walk from (s_off, q_off) down to offset 0
+// (h_blast_kernel.cu:1329-1540).  Returns the best score; *len = number of
+// positions kept (0 if no prefix improved on maxscore).
+__device__ __forceinline__ int32_t ext_left(const int8_t *m, const uint8_t *S,
+                                            const uint8_t *Q, int32_t s_off, int32_t q_off,
+                                            int32_t X, int32_t maxscore, int32_t *len)
+{
+    const int32_t n = min(s_off, q_off);
+    int32_t best = -1, sc = maxscore;
+    for (int32_t k = 0; k <= n; ++k) {
+        sc += m[((uint32_t)Q[q_off - k] << 5) | S[s_off - k]];
+        if (sc > maxscore) {
+            maxscore = sc;
+            best = k;
+        }
+        if (maxscore - sc >= X) break;
+    }
+    *len = best + 1;
+    return maxscore;
+}
+
+// Right X-drop walk (h_blast_kernel.cu:1133-1326): also stops at score <= 0.
+__device__ __forceinline__ int32_t ext_right(const int8_t *m, const uint8_t *S, int32_t slen,
+                                             const uint8_t *Q, int32_t qlen, int32_t s_off,
+                                             int32_t q_off, int32_t X, int32_t maxscore,
+                                             int32_t *len, int32_t *s_last)
+{
+    const int32_t n = min(slen - s_off, qlen - q_off);
+    int32_t best = -1, sc = maxscore, i;
+    for (i = 0; i < n; ++i) {
+        sc += m[((uint32_t)Q[q_off + i] << 5) | S[s_off + i]];
+        if (sc > maxscore) {
+            maxscore = sc;
+            best = i;
+        }
+        if (sc <= 0 || maxscore - sc >= X) break;
+    }
+    *len = best + 1;
+    *s_last = s_off + i;
+    return maxscore;
+}
+
+struct ext_result {
+    int32_t q_start, s_start, len, score, right, s_last;
+};
+
+// Two-hit extension (h_blast_kernel.cu:1069-1130).
+__device__ __forceinline__ ext_result ext_two_hit(const int8_t *m, const uint8_t *S, int32_t slen,
+                                                  const uint8_t *Q, int32_t qlen,
+                                                  int32_t s_left_off, int32_t s_right_off,
+                                                  int32_t q_right_off, int32_t X)
+{
+    int32_t sc = 0, left_score = 0, right_d = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        sc += m[((uint32_t)Q[q_right_off + i] << 5) | S[s_right_off + i]];
+        if (sc > left_score) {
+            left_score = sc;
+            right_d = i + 1;
+        }
+    }
+    q_right_off += right_d;
+    s_right_off += right_d;
+    ext_result r;
+    int32_t left_len = 0, right_len = 0, right_score = 0;
+    r.right = 0;
+    r.s_last = s_right_off;
+    left_score = ext_left(m, S, Q, s_right_off - 1, q_right_off - 1, X, 0, &left_len);
+    if (left_len >= s_right_off - s_left_off) {
+        r.right = 1;
+        right_score = ext_right(m, S, slen, Q, qlen, s_right_off, q_right_off, X, left_score,
+                                &right_len, &r.s_last);
+    }
+    r.q_start = q_right_off - left_len;
+    r.s_start = s_right_off - left_len;
+    r.len = left_len + right_len;
+    r.score = max(left_score, right_score);
+    return r;
+}
+
+__device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_ctx, int32_t q)
+{
+    int32_t lo = 0, hi = n_ctx;
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (ctx_off[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------- scan
+__global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x & (HB_WAVE - 1);
+    const int wid = threadIdx.x / HB_WAVE;
+    int8_t *sm_mat = reinterpret_cast<int8_t *>(lds);
+    uint32_t *sm_pv = reinterpret_cast<uint32_t *>(lds + 1024);
+    uint8_t *sm_q = lds + 1024 + 4 * HB_PV_WORDS;
+    unsigned char *wb = sm_q + a.qlen_pad + (size_t)wid * a.wave_lds;
+    // diag[2*d] = state (bit 31 flag, low bits last_hit + offset), diag[2*d+1] = marker
+    int32_t *diag = reinterpret_cast<int32_t *>(wb);
+    uint32_t *marker = reinterpret_cast<uint32_t *>(wb);
+    uint32_t *hitbuf = reinterpret_cast<uint32_t *>(wb + 8 * (size_t)a.diag_len);
+
+    for (int i = threadIdx.x; i < 256; i += blockDim.x)
+        reinterpret_cast<uint32_t *>(sm_mat)[i] = reinterpret_cast<const uint32_t *>(a.mat32)[i];
+    for (int i = threadIdx.x; i < HB_PV_WORDS; i += blockDim.x) sm_pv[i] = a.pv[i];
+    for (int i = threadIdx.x; i < a.qlen_pad / 4; i += blockDim.x)
+        reinterpret_cast<uint32_t *>(sm_q)[i] = reinterpret_cast<const uint32_t *>(a.query)[i];
+    for (int i = lane; i < 2 * a.diag_len; i += HB_WAVE) diag[i] = 0;
+    __syncthreads();
+
+    const int32_t window = a.window;
+    const int32_t mask = a.diag_len - 1;
+    const int32_t qlen = a.qlen;
+    int32_t base = window;   // diag offset: stale entries read as "no recent hit"
+    uint32_t gen = 0;        // marker generation
+
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(a.tile_counter, 1u);
+        t = __shfl(t, 0, HB_WAVE);
+        if (t >= a.n_tiles) break;
+        const uint32_t i_end = a.tiles[t + 1];
+        for (uint32_t it = a.tiles[t]; it < i_end; ++it) {
+            const uint32_t oid = a.order[it];
+            const uint64_t sb = a.offs[oid];
+            const int32_t slen = (int32_t)(a.offs[oid + 1] - sb);
+            const uint8_t *S = a.res + sb;
+            uint32_t hits = 0, n_ext = 0, n_hsp = 0;
+            for (int32_t s0 = 0; s0 + 3 <= slen; s0 += HB_WAVE) {
+                const int32_t s = s0 + lane;
+                uint32_t cnt = 0, cst = 0;
+                if (s + 3 <= slen) {
+                    const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
+                    if ((sm_pv[idx >> 5] >> (idx & 31)) & 1u) {
+                        const uint2 c = a.cells[idx];
+                        cst = c.x;
+                        cnt = c.y;
+                    }
+                }
+                const uint32_t incl = wave_incl_scan(cnt, lane);
+                const uint32_t excl = incl - cnt;
+                const uint32_t H = __shfl(incl, HB_WAVE - 1, HB_WAVE);
+                hits += H;
+                for (uint32_t c0 = 0; c0 < H; c0 += HB_HITCAP) {
+                    const uint32_t lo = max(c0, excl), hi = min(c0 + HB_HITCAP, incl);
+                    for (uint32_t f = lo; f < hi; ++f)
+                        hitbuf[f - c0] = (a.entries[cst + (f - excl)] << 6) | (uint32_t)lane;
+                    wave_lds_sync();
+                    const uint32_t Hc = min((uint32_t)HB_HITCAP, H - c0);
+                    for (uint32_t r0 = 0; r0 < Hc; r0 += HB_WAVE) {
+                        const uint32_t h = r0 + lane;
+                        bool active = h < Hc;
+                        const uint32_t e = active ? hitbuf[h] : 0u;
+                        const int32_t q = (int32_t)(e >> 6);
+                        const int32_t so = s0 + (int32_t)(e & 63);
+                        const int32_t d = (q - so) & mask;
+                        while (__ballot(active)) {
+                            if (++gen == (1u << 26)) {
+                                for (int i = lane; i < a.diag_len; i += HB_WAVE) marker[2 * i + 1] = 0;
+                                wave_lds_sync();
+                                gen = 1;
+                            }
+                            const uint32_t key = (gen << 6) | (uint32_t)(63 - lane);
+                            if (active) atomicMax(&marker[2 * d + 1], key);
+                            wave_lds_sync();
+                            const bool win = active && marker[2 * d + 1] == key;
+                            if (win) {
+                                const int32_t st = diag[2 * d];
+                                if (st < 0) {
+                                    if (so + base >= (st & 0x7fffffff)) diag[2 * d] = so + base;
+                                } else {
+                                    const int32_t last_s = st - base;
+                                    const int32_t diff = so - last_s;
+                                    if (diff >= window) {
+                                        diag[2 * d] = so + base;
+                                    } else if (diff >= 3) {
+                                        const int32_t k = context_of(a.ctx_off, a.n_ctx, q);
+                                        const ext_result r = ext_two_hit(sm_mat, S, slen, sm_q, qlen,
+                                                                         last_s + 3, so, q,
+                                                                         a.ctx_xdrop[k]);
+                                        ++n_ext;
+                                        if (r.score >= a.ctx_cutoff[k]) {
+                                            ++n_hsp;
+                                            const uint32_t slot = atomicAdd(a.out_count, 1u);
+                                            if (slot < a.out_cap) {
+                                                hb_hsp o;
+                                                o.oid = oid;
+                                                o.context = k;
+                                                o.q_start = r.q_start;
+                                                o.s_start = r.s_start;
+                                                o.q_off = q;
+                                                o.s_off = so;
+                                                o.len = r.len;
+                                                o.score = r.score;
+                                                a.out[slot] = o;
+                                            }
+                                        }
+                                        diag[2 * d] = r.right
+                                            ? (int32_t)(0x80000000u | (uint32_t)(r.s_last - 2 + base))
+                                            : so + base;
+                                    }
+                                }
+                            }
+                            active = active && !win;
+                            wave_lds_sync();
+                        }
+                    }
+                    wave_lds_sync();
+                }
+            }
+            n_ext = wave_sum(n_ext);
+            n_hsp = wave_sum(n_hsp);
+            if (lane == 0) {
+                a.stats[3 * (size_t)oid + 0] = hits;
+                a.stats[3 * (size_t)oid + 1] = n_ext;
+                a.stats[3 * (size_t)oid + 2] = n_hsp;
+            }
+            if (base + slen + window >= (1 << 30)) {
+                for (int i = lane; i < a.diag_len; i += HB_WAVE) diag[2 * i] = 0;
+                wave_lds_sync();
+                base = window;
+            } else {
+                base += slen + window;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t hb_launch_check_residues(const uint8_t *res, uint64_t n, uint32_t *flag,
+                                    hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n / 4 + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(check_residues_kernel, dim3((unsigned)blocks), dim3(256), 0, st, res, n,
+                       flag);
+    return hipGetLastError();
+}
+
+hipError_t hb_launch_lut_count(const uint8_t *qflag, int32_t qlen, const int32_t *,
+                               const int8_t *mat32, int32_t threshold, uint32_t *count,
+                               hipStream_t st)
+{
+    const size_t lds = 1024 + (size_t)((qlen + 15) & ~15);
+    hipLaunchKernelGGL(lut_kernel<false>, dim3(HB_NCELLS / 256), dim3(256), lds, st, qflag, qlen,
+                       mat32, threshold, count, nullptr, nullptr, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t hb_launch_lut_scan(uint32_t *count_to_start, uint32_t *total, hipStream_t st)
+{
+    hipLaunchKernelGGL(lut_scan_kernel, dim3(1), dim3(1024), 0, st, count_to_start, total);
+    return hipGetLastError();
+}
+
+hipError_t hb_launch_lut_fill(const uint8_t *qflag, int32_t qlen, const int32_t *,
+                              const int8_t *mat32, int32_t threshold, const uint32_t *start,
+                              uint32_t *entries, uint2 *cells, uint32_t *pv, hipStream_t st)
+{
+    const size_t lds = 1024 + (size_t)((qlen + 15) & ~15);
+    hipLaunchKernelGGL(lut_kernel<true>, dim3(HB_NCELLS / 256), dim3(256), lds, st, qflag, qlen,
+                       mat32, threshold, nullptr, start, entries, cells, pv);
+    return hipGetLastError();
+}
+
+hipError_t hb_launch_scan(const hb_scan_args &a, int n_blocks, size_t lds_bytes, hipStream_t st)
+{
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(hb_scan_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(hb_scan_kernel, dim3(n_blocks), dim3(HB_BLOCK), lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+hipError_t hb_scan_occupancy(size_t lds_bytes, int *blocks_per_cu)
+{
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(hb_scan_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hb_scan_kernel, HB_BLOCK,
+                                                        lds_bytes);
+}

@@ -1,0 +1,364 @@
+"""hbseed -- Python view of the MI355X seed-extension stage (libhbseed.so).
+
+The stage itself is C++/HIP behind the C ABI in include/hb_seed.h; this
+module only loads that library with ctypes and moves numpy arrays across
+the boundary (tests and the benchmark use it).  Nothing here computes
+results: if the native library is missing, loading raises.
+
+The calls mirror the reference's host interface
+(h_blast_v1.1/H-BLAST/ncbi_blast_files/gpu_cpu_common.h:43-126):
+``Database`` replaces Read_H_BLAST_GPU_DB, ``Query`` the lookup-table
+staging in H_BLAST, ``Scan.start`` H_BLAST_pre and ``Scan.results``
+H_BLAST_get_data.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBDIR = os.path.join(os.path.dirname(_HERE), "lib")
+
+HB_OK = 0
+ERRORS = {
+    -1: "HB_ERR_ARG", -2: "HB_ERR_NOMEM", -3: "HB_ERR_DEVICE", -4: "HB_ERR_CAPACITY",
+    -5: "HB_ERR_RESIDUE", -6: "HB_ERR_STATE", -7: "HB_ERR_UNSUPPORTED", -8: "HB_ERR_BUSY",
+}
+DEFAULT_XDROP = 16
+DEFAULT_CUTOFF = 41
+STDAA = "-ABCDEFGHIKLMNPQRSTVWXYZU*OJ"
+
+HSP_DTYPE = np.dtype([("oid", "<u4"), ("context", "<i4"), ("q_start", "<i4"),
+                      ("s_start", "<i4"), ("q_off", "<i4"), ("s_off", "<i4"),
+                      ("len", "<i4"), ("score", "<i4")])
+STATS_DTYPE = np.dtype([("lookup_hits", "<u4"), ("extensions", "<u4"), ("n_hsp", "<u4"),
+                        ("first", "<i8")], align=True)
+
+# Every symbol include/hb_seed.h declares (checked by the CPU tests).
+EXPORTS = [
+    "hb_params_default", "hb_strerror", "hb_blosum62", "hb_create", "hb_destroy",
+    "hb_mem_info", "hb_synchronize", "hb_db_upload", "hb_db_free", "hb_db_subjects",
+    "hb_db_residues", "hb_query_prepare", "hb_query_free", "hb_query_lut_info",
+    "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
+    "hb_encode_protein", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
+    "hb_scan_elapsed_ms", "hb_scan_free", "hb_search",
+]
+
+
+class HBError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = _lib().hb_strerror(code).decode() if _LIB is not None else ERRORS.get(code, code)
+        super().__init__(f"{what}: {ERRORS.get(code, code)} ({msg})")
+
+
+class _Params(C.Structure):
+    _fields_ = [("word_size", C.c_int32), ("threshold", C.c_int32), ("window", C.c_int32),
+                ("matrix", C.c_void_p)]
+
+
+_LIB = None
+_SYNTH = None
+
+
+def lib_path(name="libhbseed.so"):
+    return os.path.join(_LIBDIR, name)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"native library {path} is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        vp, i32, u32, u64 = C.c_void_p, C.c_int32, C.c_uint32, C.c_uint64
+        pp = C.POINTER(C.c_void_p)
+        L.hb_strerror.restype = C.c_char_p
+        L.hb_strerror.argtypes = [C.c_int]
+        L.hb_blosum62.restype = vp
+        L.hb_create.argtypes = [C.c_int, pp]
+        L.hb_destroy.argtypes = [vp]
+        L.hb_mem_info.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
+        L.hb_synchronize.argtypes = [vp]
+        L.hb_db_upload.argtypes = [vp, vp, vp, u32, pp]
+        L.hb_db_free.argtypes = [vp]
+        L.hb_db_subjects.restype = u32
+        L.hb_db_subjects.argtypes = [vp]
+        L.hb_db_residues.restype = u64
+        L.hb_db_residues.argtypes = [vp]
+        L.hb_query_prepare.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, vp, pp]
+        L.hb_query_free.argtypes = [vp]
+        L.hb_query_lut_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
+        L.hb_query_lut_fetch.argtypes = [vp, vp, vp, u32]
+        L.hb_blastx_concat_len.restype = i32
+        L.hb_blastx_concat_len.argtypes = [i32]
+        L.hb_blastx_translate.argtypes = [C.c_char_p, i32, vp, vp, vp]
+        L.hb_encode_protein.argtypes = [C.c_char_p, i32, vp]
+        L.hb_scan_start.argtypes = [vp, vp, vp, pp]
+        L.hb_scan_wait.argtypes = [vp]
+        L.hb_scan_results.argtypes = [vp, vp, vp, u64, C.POINTER(u64)]
+        L.hb_scan_elapsed_ms.argtypes = [vp, C.POINTER(C.c_float)]
+        L.hb_scan_free.argtypes = [vp]
+        L.hb_search.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, u64,
+                                C.POINTER(u64)]
+        _LIB = L
+    return _LIB
+
+
+def _synth():
+    global _SYNTH
+    if _SYNTH is None:
+        path = lib_path("libhbsynth.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"native library {path} is not built")
+        S = C.CDLL(path)
+        S.hbs_lognormal_lengths.restype = C.c_uint64
+        S.hbs_lognormal_lengths.argtypes = [C.c_uint64, C.c_uint32, C.c_double, C.c_double,
+                                            C.c_uint32, C.c_uint32, C.c_void_p]
+        S.hbs_random_protein.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p]
+        S.hbs_db_fill.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_int32, C.c_double, C.c_int]
+        S.hbs_homolog.restype = C.c_int32
+        S.hbs_homolog.argtypes = [C.c_uint64, C.c_void_p, C.c_int32, C.c_double, C.c_int,
+                                  C.c_void_p, C.c_int32]
+        S.hbs_blastx_query.argtypes = [C.c_uint64, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
+                                       C.c_int32, C.c_void_p]
+        _SYNTH = S
+    return _SYNTH
+
+
+def _check(rc, what):
+    if rc != HB_OK:
+        raise HBError(rc, what)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def blosum62():
+    p = _lib().hb_blosum62()
+    return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_int8)), shape=(28, 28)).copy()
+
+
+def encode_protein(text: str) -> np.ndarray:
+    out = np.zeros(len(text), np.uint8)
+    _check(_lib().hb_encode_protein(text.encode(), len(text), _ptr(out)), "encode_protein")
+    return out
+
+
+def decode_protein(codes) -> str:
+    return "".join(STDAA[int(c)] for c in codes)
+
+
+# ------------------------------------------------------------------ queries
+class QuerySpec:
+    """A concatenated query: residues plus per-context offsets/lengths and
+    per-context X-drop/cutoff (what NCBI's BLAST_SequenceBlk, BlastQueryInfo
+    and BlastInitialWordParameters carry to H_BLAST_pre)."""
+
+    def __init__(self, seq, ctx_off, ctx_len, xdrop=None, cutoff=None):
+        self.seq = np.ascontiguousarray(seq, dtype=np.uint8)
+        self.ctx_off = np.ascontiguousarray(ctx_off, dtype=np.int32)
+        self.ctx_len = np.ascontiguousarray(ctx_len, dtype=np.int32)
+        n = len(self.ctx_off)
+        self.xdrop = np.full(n, DEFAULT_XDROP, np.int32) if xdrop is None else \
+            np.ascontiguousarray(xdrop, dtype=np.int32)
+        self.cutoff = np.full(n, DEFAULT_CUTOFF, np.int32) if cutoff is None else \
+            np.ascontiguousarray(cutoff, dtype=np.int32)
+
+    @property
+    def n_ctx(self):
+        return len(self.ctx_off)
+
+
+def blastp_query(seq, xdrop=DEFAULT_XDROP, cutoff=DEFAULT_CUTOFF) -> QuerySpec:
+    seq = np.ascontiguousarray(seq, dtype=np.uint8)
+    return QuerySpec(seq, [0], [len(seq)], [xdrop], [cutoff])
+
+
+def blastx_query(nt: str, xdrop=DEFAULT_XDROP, cutoff=DEFAULT_CUTOFF) -> QuerySpec:
+    L = _lib()
+    n = L.hb_blastx_concat_len(len(nt))
+    seq = np.zeros(n, np.uint8)
+    off = np.zeros(6, np.int32)
+    ln = np.zeros(6, np.int32)
+    _check(L.hb_blastx_translate(nt.encode(), len(nt), _ptr(seq), _ptr(off), _ptr(ln)),
+           "blastx_translate")
+    return QuerySpec(seq, off, ln, [xdrop] * 6, [cutoff] * 6)
+
+
+# ------------------------------------------------------------------ runtime
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(_lib().hb_create(device, C.byref(h)), "hb_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib().hb_destroy(self.h)
+            self.h = None
+
+    def mem_info(self):
+        f, t = C.c_uint64(), C.c_uint64()
+        _check(_lib().hb_mem_info(self.h, C.byref(f), C.byref(t)), "hb_mem_info")
+        return f.value, t.value
+
+    def synchronize(self):
+        _check(_lib().hb_synchronize(self.h), "hb_synchronize")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Database:
+    def __init__(self, ctx: Context, residues: np.ndarray, offsets: np.ndarray):
+        self.ctx = ctx
+        residues = np.ascontiguousarray(residues, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self.n = len(offsets) - 1
+        h = C.c_void_p()
+        _check(_lib().hb_db_upload(ctx.h, _ptr(residues), _ptr(offsets), self.n, C.byref(h)),
+               "hb_db_upload")
+        self.h = h
+
+    @property
+    def residues(self):
+        return _lib().hb_db_residues(self.h)
+
+    def close(self):
+        if self.h:
+            _lib().hb_db_free(self.h)
+            self.h = None
+
+
+def _params_struct(threshold=11, window=40, word_size=3, matrix=None):
+    p = _Params(word_size, threshold, window, None)
+    keep = None
+    if matrix is not None:
+        keep = np.ascontiguousarray(matrix, dtype=np.int8).reshape(28 * 28)
+        p.matrix = keep.ctypes.data
+    return p, keep
+
+
+class Query:
+    def __init__(self, ctx: Context, spec: QuerySpec, threshold=11, window=40, word_size=3,
+                 matrix=None):
+        self.ctx = ctx
+        self.spec = spec
+        p, self._keep = _params_struct(threshold, window, word_size, matrix)
+        h = C.c_void_p()
+        _check(_lib().hb_query_prepare(ctx.h, _ptr(spec.seq), len(spec.seq), spec.n_ctx,
+                                       _ptr(spec.ctx_off), _ptr(spec.ctx_len), _ptr(spec.xdrop),
+                                       _ptr(spec.cutoff), C.byref(p), C.byref(h)),
+               "hb_query_prepare")
+        self.h = h
+
+    def lut(self):
+        ne, nu = C.c_uint32(), C.c_uint32()
+        _check(_lib().hb_query_lut_info(self.h, C.byref(ne), C.byref(nu)), "lut_info")
+        start = np.zeros(32769, np.uint32)
+        ent = np.zeros(max(ne.value, 1), np.uint32)
+        _check(_lib().hb_query_lut_fetch(self.h, _ptr(start), _ptr(ent), ne.value),
+               "lut_fetch")
+        return start, ent[:ne.value]
+
+    def close(self):
+        if self.h:
+            _lib().hb_query_free(self.h)
+            self.h = None
+
+
+class Scan:
+    def __init__(self, ctx: Context, db: Database, q: Query):
+        self.db = db
+        h = C.c_void_p()
+        _check(_lib().hb_scan_start(ctx.h, db.h, q.h, C.byref(h)), "hb_scan_start")
+        self.h = h
+
+    def wait(self):
+        _check(_lib().hb_scan_wait(self.h), "hb_scan_wait")
+
+    def elapsed_ms(self):
+        ms = C.c_float()
+        _check(_lib().hb_scan_elapsed_ms(self.h, C.byref(ms)), "hb_scan_elapsed_ms")
+        return ms.value
+
+    def results(self):
+        """-> (stats[n_subjects], hsps[n]) numpy structured arrays."""
+        n = C.c_uint64()
+        _check(_lib().hb_scan_results(self.h, None, None, 0, C.byref(n)), "hb_scan_results")
+        stats = np.zeros(self.db.n, STATS_DTYPE)
+        hsps = np.zeros(n.value, HSP_DTYPE)
+        _check(_lib().hb_scan_results(self.h, _ptr(stats), _ptr(hsps), n.value, C.byref(n)),
+               "hb_scan_results")
+        return stats, hsps
+
+    def close(self):
+        if self.h:
+            _lib().hb_scan_free(self.h)
+            self.h = None
+
+
+def search(ctx, db, spec: QuerySpec, **kw):
+    q = Query(ctx, spec, **kw)
+    try:
+        s = Scan(ctx, db, q)
+        try:
+            return s.results()
+        finally:
+            s.close()
+    finally:
+        q.close()
+
+
+# ------------------------------------------------------------------ synthetic data
+def lognormal_lengths(seed, n, median, sigma, lo, hi):
+    out = np.zeros(n, np.uint32)
+    _synth().hbs_lognormal_lengths(seed, n, median, sigma, lo, hi, _ptr(out))
+    return out
+
+
+def random_protein(seed, n):
+    out = np.zeros(n, np.uint8)
+    _synth().hbs_random_protein(seed, n, _ptr(out))
+    return out
+
+
+def make_db(seed, n, median=300.0, sigma=0.6, lo=30, hi=5000, plant_query=None,
+            plant_frac=0.0, threads=None):
+    """Synthetic packed database -> (residues uint8, offsets uint64[n+1])."""
+    lens = lognormal_lengths(seed, n, median, sigma, lo, hi)
+    offs = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=offs[1:])
+    res = np.zeros(int(offs[-1]), np.uint8)
+    pq = None if plant_query is None else np.ascontiguousarray(plant_query, np.uint8)
+    threads = threads or min(16, os.cpu_count() or 1)
+    _synth().hbs_db_fill(seed, n, _ptr(offs), _ptr(res), _ptr(pq),
+                         0 if pq is None else len(pq), plant_frac, threads)
+    return res, offs
+
+
+def homolog(seed, src, identity, n_indel):
+    src = np.ascontiguousarray(src, np.uint8)
+    out = np.zeros(len(src) + 16, np.uint8)
+    n = _synth().hbs_homolog(seed, _ptr(src), len(src), identity, n_indel, _ptr(out), len(out))
+    return out[:n]
+
+
+def blastx_nt(seed, nt_len, seg1, seg2):
+    seg1 = np.ascontiguousarray(seg1, np.uint8)
+    seg2 = np.ascontiguousarray(seg2, np.uint8)
+    out = C.create_string_buffer(nt_len)
+    rc = _synth().hbs_blastx_query(seed, nt_len, _ptr(seg1), len(seg1), _ptr(seg2), len(seg2),
+                                   out)
+    if rc:
+        raise ValueError("coding regions do not fit the query")
+    return out.raw.decode()

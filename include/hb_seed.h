@@ -1,0 +1,180 @@
+/*
+ * hb_seed.h -- C ABI of the MI355X seed-extension stage (BLOSUM62 word scan,
+ * two-hit diagonal tracking, X-drop ungapped extension) for BLASTP and
+ * six-frame BLASTX queries.
+ *
+ * Drop-in boundary: these entry points replace the extern "C" API H-BLAST
+ * exports to NCBI's engine (h_blast_v1.1/H-BLAST/ncbi_blast_files/
+ * gpu_cpu_common.h:43-126, implemented in H-BLAST/h_blast.cu).  The
+ * reference passes NCBI structs (LookupTableWrap, Blast_ExtendWord,
+ * BlastInitialWordParameters, BLAST_SequenceBlk, BlastQueryInfo); this ABI
+ * takes the plain arrays those structs hold, so a caller without NCBI
+ * headers can use it.  All functions return an int status (HB_OK or a
+ * negative HB_ERR_*) instead of calling exit() (H-BLAST/h_blast.cu:311-332).
+ *
+ * Conventions
+ *   - residues: ncbistdaa codes 0..27 (0 = gap/sentinel, 1 = A, ... 27 = J);
+ *   - query: contexts concatenated, separated by code-0 sentinels; offsets
+ *     in results are positions in that concatenation (as NCBI's
+ *     BlastInitHitList holds them);
+ *   - word size 3 only; scores are int32.
+ *
+ * Threading: an hb_ctx serialises its own calls; distinct contexts may be
+ * used from distinct threads.  Handles are owned by the library and released
+ * with the matching *_free; output arrays are owned by the caller.
+ */
+#ifndef HB_SEED_H
+#define HB_SEED_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HB_OK 0
+#define HB_ERR_ARG (-1)          /* invalid argument */
+#define HB_ERR_NOMEM (-2)        /* host or device allocation failed */
+#define HB_ERR_DEVICE (-3)       /* HIP runtime error */
+#define HB_ERR_CAPACITY (-4)     /* caller's output array too small */
+#define HB_ERR_RESIDUE (-5)      /* residue code >= 28 */
+#define HB_ERR_STATE (-6)        /* handle used in the wrong state */
+#define HB_ERR_UNSUPPORTED (-7)  /* e.g. word size != 3 */
+#define HB_ERR_BUSY (-8)         /* serving queue full / not admitted */
+
+typedef struct hb_ctx hb_ctx;
+typedef struct hb_db hb_db;
+typedef struct hb_query hb_query;
+typedef struct hb_scan hb_scan;
+
+/* One initial HSP.  Replaces GPUBlastInitHitList
+ * (ncbi_blast_files/gpu_cpu_common.h:31-40): full-width coordinates, the
+ * context and the triggering hit's offsets (what BlastSaveInitHsp receives,
+ * ncbi_blast_files/blast_engine.c:568-576). */
+typedef struct hb_hsp {
+    uint32_t oid;     /* subject index in the uploaded database */
+    int32_t context;  /* query context (frame for BLASTX) */
+    int32_t q_start;  /* HSP start, concatenated-query coordinates */
+    int32_t s_start;  /* HSP start in the subject */
+    int32_t q_off;    /* query offset of the second (triggering) hit */
+    int32_t s_off;    /* subject offset of that hit */
+    int32_t len;      /* HSP length */
+    int32_t score;    /* ungapped raw score */
+} hb_hsp;
+
+/* Per-subject counters.  Replaces the d_Hits[4*oid .. 4*oid+3] quadruple
+ * {lookup hits, HSPs, extensions, list base | -1}
+ * (H-BLAST/h_blast_kernel.cu:462-492, consumed at
+ * ncbi_blast_files/blast_engine.c:540-545).  There is no "-1 = recompute on
+ * the CPU" state: the device never drops HSPs. */
+typedef struct hb_subject_stats {
+    uint32_t lookup_hits;  /* word hits (sum of cell sizes over positions) */
+    uint32_t extensions;   /* two-hit extensions performed */
+    uint32_t n_hsp;        /* HSPs with score >= cutoff */
+    int64_t first;         /* index of the subject's first HSP in the list, -1 if none */
+} hb_subject_stats;
+
+/* Search parameters.  The reference reads these from NCBI structs
+ * (lookup->word_length, ewp->diag_table->window, the LUT threshold;
+ * H-BLAST/h_blast.cu:959-983). */
+typedef struct hb_params {
+    int32_t word_size;     /* must be 3 */
+    int32_t threshold;     /* neighbourhood threshold T (default 11) */
+    int32_t window;        /* two-hit window (default 40) */
+    const int8_t *matrix;  /* 28x28 row-major scores, or NULL for BLOSUM62 */
+} hb_params;
+
+/* Fills the defaults: word 3, T 11, window 40, BLOSUM62. */
+void hb_params_default(hb_params *p);
+/* Default per-context X-drop (16) and cutoff (41): BLOSUM62 ungapped
+ * lambda 0.3176, K 0.134; X = ceil(7 ln2 / lambda); cutoff =
+ * floor((22 ln2 + ln K) / lambda). */
+#define HB_DEFAULT_XDROP 16
+#define HB_DEFAULT_CUTOFF 41
+
+const char *hb_strerror(int err);
+const int8_t *hb_blosum62(void);  /* 28x28 row-major */
+
+/* --- device context ----------------------------------------------------
+ * Replaces GPU_BLAST_check_availability / SetNumOfGpuCards /
+ * set_first_gpu_id (ncbi_blast_files/gpu_cpu_common.h:99,111,79). */
+int hb_create(int device, hb_ctx **out);
+int hb_destroy(hb_ctx *ctx);
+/* Free / total device memory in bytes (admission control uses this;
+ * replaces H_BLAST_check_memory, gpu_cpu_common.h:93-102). */
+int hb_mem_info(hb_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+int hb_synchronize(hb_ctx *ctx);
+
+/* --- packed subject database -------------------------------------------
+ * Replaces Read_H_BLAST_GPU_DB + the per-call DB upload
+ * (H-BLAST/h_blast.cu:587-653, 1094-1096): the database is uploaded once
+ * and stays resident in HBM across scans.  Subject i is
+ * residues[offsets[i] .. offsets[i+1]); offsets has n_subjects+1 entries. */
+int hb_db_upload(hb_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
+                 uint32_t n_subjects, hb_db **out);
+int hb_db_free(hb_db *db);
+uint32_t hb_db_subjects(const hb_db *db);
+uint64_t hb_db_residues(const hb_db *db);
+
+/* --- query ---------------------------------------------------------------
+ * Builds the neighbourhood lookup table, presence vector and query tables
+ * on the device.  Replaces the LUT/PV/matrix/constant staging of H_BLAST
+ * (H-BLAST/h_blast.cu:940-1091); the NCBI lookup table itself
+ * (BlastAaLookupTable) is rebuilt here from the query.
+ *   seq/len:      concatenated query (contexts separated by code 0)
+ *   ctx_off/len:  n_ctx ascending context ranges inside seq
+ *   xdrop/cutoff: per-context raw X-drop and cutoff score
+ *                 (word_params->cutoffs[k], H-BLAST/h_blast.cu:968-975) */
+int hb_query_prepare(hb_ctx *ctx, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                     const int32_t *ctx_off, const int32_t *ctx_len,
+                     const int32_t *xdrop, const int32_t *cutoff,
+                     const hb_params *params, hb_query **out);
+int hb_query_free(hb_query *q);
+/* Number of (query offset, cell) entries of the lookup table. */
+int hb_query_lut_info(const hb_query *q, uint32_t *n_entries, uint32_t *n_cells_used);
+/* Copies the device lookup table back (CSR over 32768 cells:
+ * cell_start[32769], entries[n_entries]); for tests. */
+int hb_query_lut_fetch(const hb_query *q, uint32_t *cell_start, uint32_t *entries,
+                       uint32_t cap);
+
+/* Helpers that build the concatenated query of a BLASTP query (1 context)
+ * or a six-frame translated BLASTX query (frames +1,+2,+3,-1,-2,-3, one
+ * code-0 sentinel between frames, standard genetic code, stop = '*' (25),
+ * ambiguous codon = X (21)).  nt is IUPAC text (ACGTN..., case-insensitive).
+ * out_seq needs hb_blastx_concat_len(nt_len) bytes; ctx arrays need 6. */
+int32_t hb_blastx_concat_len(int32_t nt_len);
+int hb_blastx_translate(const char *nt, int32_t nt_len, uint8_t *out_seq,
+                        int32_t *ctx_off, int32_t *ctx_len);
+/* IUPAC protein letters <-> ncbistdaa codes. */
+int hb_encode_protein(const char *text, int32_t len, uint8_t *out);
+
+/* --- plain per-query scan ----------------------------------------------
+ * hb_scan_start launches the stage on the context's stream and returns
+ * (asynchronous, like H_BLAST_pre, ncbi_blast_files/gpu_cpu_common.h:48-54).
+ * hb_scan_results blocks until the scan is done and copies results out
+ * (like H_BLAST_get_data, gpu_cpu_common.h:56-60): stats gets
+ * hb_db_subjects() entries, hsps gets every HSP ordered by
+ * (oid, s_off, q_off) -- the order the CPU word finder discovers them --
+ * and *n_hsp the total.  If cap is too small, *n_hsp is still set and
+ * HB_ERR_CAPACITY is returned; call again with a larger array. */
+int hb_scan_start(hb_ctx *ctx, hb_db *db, hb_query *q, hb_scan **out);
+int hb_scan_wait(hb_scan *s);
+int hb_scan_results(hb_scan *s, hb_subject_stats *stats, hb_hsp *hsps,
+                    uint64_t cap, uint64_t *n_hsp);
+/* Device time of the scan kernel(s) in milliseconds (hipEvent pair on the
+ * stream the kernels run on); replaces get_gpu_walltime
+ * (gpu_cpu_common.h:71). */
+int hb_scan_elapsed_ms(hb_scan *s, float *ms);
+int hb_scan_free(hb_scan *s);
+
+/* Convenience: prepare + start + results for one query. */
+int hb_search(hb_ctx *ctx, hb_db *db, const uint8_t *seq, int32_t len,
+              int32_t n_ctx, const int32_t *ctx_off, const int32_t *ctx_len,
+              const int32_t *xdrop, const int32_t *cutoff, const hb_params *params,
+              hb_subject_stats *stats, hb_hsp *hsps, uint64_t cap, uint64_t *n_hsp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HB_SEED_H */

@@ -1,0 +1,106 @@
+"""GPU parity: the HIP stage against the CPU oracle, bit-exact.
+
+Every comparison is exact: HSP records (oid, context, q_start, s_start,
+q_off, s_off, len, score) in (oid, s_off, q_off) order, and the per-subject
+counters (lookup hits, extensions, HSPs).
+"""
+import numpy as np
+import pytest
+
+import hbseed
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = hbseed.Context(0)
+    yield c
+    c.close()
+
+
+def compare(ctx, spec, res, offs, threshold=11, window=40, matrix=None, threads=8):
+    db = hbseed.Database(ctx, res, offs)
+    try:
+        stats, hsps = hbseed.search(ctx, db, spec, threshold=threshold, window=window,
+                                    matrix=matrix)
+    finally:
+        db.close()
+    oq = oracle_lib.OracleQuery.from_spec(spec, threshold=threshold, window=window,
+                                          matrix=matrix)
+    cnt, ref = oq.scan(res, offs, threads=threads)
+    assert len(hsps) == len(ref), (len(hsps), len(ref))
+    got = hsps.view(np.int32).reshape(-1, 8)
+    exp = ref.view(np.int32).reshape(-1, 8)
+    bad = np.nonzero((got != exp).any(axis=1))[0]
+    assert bad.size == 0, f"first mismatch at {bad[:5]}: got {hsps[bad[:3]]} exp {ref[bad[:3]]}"
+    assert np.array_equal(stats["lookup_hits"], cnt[:, 0])
+    assert np.array_equal(stats["extensions"], cnt[:, 1])
+    assert np.array_equal(stats["n_hsp"], cnt[:, 2])
+    first = np.full(len(offs) - 1, -1, np.int64)
+    if len(ref):
+        starts = np.nonzero(np.r_[True, ref["oid"][1:] != ref["oid"][:-1]])[0]
+        first[ref["oid"][starts]] = starts
+    assert np.array_equal(stats["first"], first)
+    return stats, hsps
+
+
+@pytest.mark.parametrize("qlen,n_subj,seed", [(300, 20000, 1000), (1000, 5000, 1100),
+                                               (64, 3000, 7), (2000, 2000, 9)])
+def test_blastp_matches_oracle(ctx, qlen, n_subj, seed):
+    q = hbseed.random_protein(seed + 1, qlen)
+    res, offs = hbseed.make_db(seed, n_subj, plant_query=q, plant_frac=0.1)
+    stats, hsps = compare(ctx, hbseed.blastp_query(q), res, offs)
+    assert len(hsps) > 0
+
+
+def test_blastx_matches_oracle(ctx):
+    pq = hbseed.random_protein(1201, 1000)
+    res, offs = hbseed.make_db(1200, 4000, plant_query=pq, plant_frac=0.1)
+    # two coding regions: homologs of two 100-aa database segments
+    lens = np.diff(offs)
+    big = np.nonzero(lens >= 150)[0]
+    s1 = res[offs[big[0]]:offs[big[0]] + 100]
+    s2 = res[offs[big[1]] + 20:offs[big[1]] + 120]
+    nt = hbseed.blastx_nt(1202, 1500, hbseed.homolog(1203, s1, 0.7, 1),
+                          hbseed.homolog(1204, s2, 0.7, 1))
+    spec = hbseed.blastx_query(nt)
+    stats, hsps = compare(ctx, spec, res, offs)
+    assert set(np.unique(hsps["context"])) >= {0, 3} or len(hsps) > 0
+
+
+def test_edge_subjects(ctx):
+    q = hbseed.random_protein(5, 200)
+    parts = [np.zeros(0, np.uint8), q[:1], q[:2], q[:3], q[10:80], q[:200],
+             np.full(500, 1, np.uint8), q[::-1].copy(), hbseed.random_protein(6, 4000)]
+    res = np.concatenate(parts)
+    offs = np.zeros(len(parts) + 1, np.uint64)
+    offs[1:] = np.cumsum([len(p) for p in parts])
+    compare(ctx, hbseed.blastp_query(q), res, offs)
+
+
+def test_repetitive_query(ctx):
+    # long runs make large lookup cells and many hits on one diagonal
+    q = np.concatenate([np.full(150, 1, np.uint8), hbseed.random_protein(8, 100),
+                        np.tile(np.array([9, 11, 19], np.uint8), 50)])
+    res, offs = hbseed.make_db(11, 2000, plant_query=q, plant_frac=0.2)
+    extra = np.concatenate([np.full(300, 1, np.uint8), np.tile(np.array([9, 11, 19], np.uint8), 100)])
+    res = np.concatenate([res, extra])
+    offs = np.append(offs, offs[-1] + len(extra)).astype(np.uint64)
+    compare(ctx, hbseed.blastp_query(q), res, offs)
+
+
+def test_device_lut_matches_oracle(ctx):
+    q = hbseed.random_protein(21, 700)
+    q[5:8] = [21, 25, 21]   # X * X: self-score below T, must still seed itself
+    spec = hbseed.blastp_query(q)
+    dq = hbseed.Query(ctx, spec)
+    try:
+        start, ent = dq.lut()
+    finally:
+        dq.close()
+    oq = oracle_lib.OracleQuery.from_spec(spec)
+    ostart, oent = oq.lut_arrays()
+    assert np.array_equal(start, ostart)
+    assert np.array_equal(ent, oent)
