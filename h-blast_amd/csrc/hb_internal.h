@@ -57,3 +57,4 @@ hipError_t hb_launch_lut_fill(const uint8_t *query, int32_t qlen, const int32_t 
 hipError_t hb_launch_scan(const hb_scan_args &a, int n_blocks, size_t lds_bytes,
                           hipStream_t st);
 hipError_t hb_scan_occupancy(size_t lds_bytes, int *blocks_per_cu);
+hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);

@@ -371,7 +371,27 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
     }
 }
 
+// Plain 16-byte-per-lane copy: the known-size reference kernel for
+// calibrating HBM byte counters and the achievable copy bandwidth.
+__global__ __launch_bounds__(256) void copy_kernel(uint4 *dst, const uint4 *src, uint64_t n16)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st)
+{
+    const uint64_t n16 = bytes / 16;
+    uint64_t blocks = (n16 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
+                       reinterpret_cast<uint4 *>(dst), reinterpret_cast<const uint4 *>(src), n16);
+    return hipGetLastError();
+}
 
 hipError_t hb_launch_check_residues(const uint8_t *res, uint64_t n, uint32_t *flag,
                                     hipStream_t st)

@@ -48,7 +48,42 @@ struct hb_ctx {
     int n_cu = 0;
     size_t max_lds = 0;
     std::mutex mu;
+    // Device buffers of finished scans, reused by later scans so that a
+    // per-query loop does not pay hipMalloc/hipFree per call.
+    std::mutex pool_mu;
+    std::vector<std::pair<void *, size_t>> pool;
 };
+
+static void *pool_get(hb_ctx *c, size_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> lk(c->pool_mu);
+        size_t best = (size_t)-1;
+        for (size_t i = 0; i < c->pool.size(); ++i)
+            if (c->pool[i].second >= bytes && c->pool[i].second <= 2 * bytes + 4096 &&
+                (best == (size_t)-1 || c->pool[i].second < c->pool[best].second))
+                best = i;
+        if (best != (size_t)-1) {
+            void *p = c->pool[best].first;
+            c->pool.erase(c->pool.begin() + (long)best);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    return p;
+}
+
+static void pool_put(hb_ctx *c, void *p, size_t bytes)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(c->pool_mu);
+    c->pool.emplace_back(p, bytes);
+    if (c->pool.size() > 32) {
+        hipFree(c->pool.front().first);
+        c->pool.erase(c->pool.begin());
+    }
+}
 
 struct hb_db {
     hb_ctx *ctx = nullptr;
@@ -149,6 +184,8 @@ int hb_destroy(hb_ctx *c)
 {
     if (!c) return HB_ERR_ARG;
     hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &b : c->pool) hipFree(b.first);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return HB_OK;
@@ -480,9 +517,9 @@ int hb_scan_free(hb_scan *s)
     if (!s) return HB_ERR_ARG;
     hipSetDevice(s->ctx->device);
     if (!s->waited) hipStreamSynchronize(s->ctx->stream);
-    hipFree(s->d_out);
-    hipFree(s->d_misc);
-    hipFree(s->d_stats);
+    pool_put(s->ctx, s->d_out, (size_t)s->cap * sizeof(hb_hsp));
+    pool_put(s->ctx, s->d_misc, 2 * sizeof(uint32_t));
+    pool_put(s->ctx, s->d_stats, (size_t)std::max<uint32_t>(s->db->n, 1) * 3 * sizeof(uint32_t));
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     delete s;
@@ -501,10 +538,10 @@ int hb_scan_start(hb_ctx *c, hb_db *db, hb_query *q, hb_scan **out)
     s->db = db;
     s->q = q;
     s->cap = std::max<uint32_t>(1u << 18, db->n / 4);
-    if (hipMalloc(&s->d_out, (size_t)s->cap * sizeof(hb_hsp)) != hipSuccess ||
-        hipMalloc(&s->d_misc, 2 * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&s->d_stats, (size_t)std::max<uint32_t>(db->n, 1) * 3 * sizeof(uint32_t)) !=
-            hipSuccess) {
+    s->d_out = (hb_hsp *)pool_get(c, (size_t)s->cap * sizeof(hb_hsp));
+    s->d_misc = (uint32_t *)pool_get(c, 2 * sizeof(uint32_t));
+    s->d_stats = (uint32_t *)pool_get(c, (size_t)std::max<uint32_t>(db->n, 1) * 3 * sizeof(uint32_t));
+    if (!s->d_out || !s->d_misc || !s->d_stats) {
         hb_scan_free(s);
         return HB_ERR_NOMEM;
     }
@@ -538,11 +575,13 @@ int hb_scan_wait(hb_scan *s)
         }
         // The kernel counted every HSP but stored only cap: rerun with room
         // for all of them (the stage is deterministic).
-        hipFree(s->d_out);
-        s->d_out = nullptr;
+        pool_put(c, s->d_out, (size_t)s->cap * sizeof(hb_hsp));
         s->cap = n + n / 8 + 1024;
-        if (hipMalloc(&s->d_out, (size_t)s->cap * sizeof(hb_hsp)) != hipSuccess)
+        s->d_out = (hb_hsp *)pool_get(c, (size_t)s->cap * sizeof(hb_hsp));
+        if (!s->d_out) {
+            s->cap = 0;
             return HB_ERR_NOMEM;
+        }
         int rc = launch_scan(s);
         if (rc) return rc;
     }
@@ -609,6 +648,41 @@ int hb_search(hb_ctx *c, hb_db *db, const uint8_t *seq, int32_t len, int32_t n_c
     if (!rc) rc = hb_scan_results(s, stats, hsps, cap, n_hsp);
     if (s) hb_scan_free(s);
     hb_query_free(q);
+    return rc;
+}
+
+// ------------------------------------------------------------------ diagnostics
+int hb_diag_copy(hb_ctx *c, uint64_t bytes, int reps, float *ms_per_copy)
+{
+    if (!c || bytes < 16 || reps < 1 || !ms_per_copy) return HB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HB_CHECK(hipSetDevice(c->device));
+    bytes &= ~(uint64_t)15;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = HB_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        rc = HB_ERR_NOMEM;
+    } else if (hipMemsetAsync(a, 1, bytes, c->stream) != hipSuccess ||
+               hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+               hb_launch_copy(b, a, bytes, c->stream) != hipSuccess ||
+               hipEventRecord(e0, c->stream) != hipSuccess) {
+        rc = HB_ERR_DEVICE;
+    } else {
+        for (int r = 0; r < reps && rc == HB_OK; ++r)
+            if (hb_launch_copy(b, a, bytes, c->stream) != hipSuccess) rc = HB_ERR_DEVICE;
+        float ms = 0;
+        if (rc == HB_OK &&
+            (hipEventRecord(e1, c->stream) != hipSuccess ||
+             hipEventSynchronize(e1) != hipSuccess ||
+             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = HB_ERR_DEVICE;
+        *ms_per_copy = ms / (float)reps;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
     return rc;
 }
 

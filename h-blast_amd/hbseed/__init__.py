@@ -43,7 +43,7 @@ EXPORTS = [
     "hb_db_residues", "hb_query_prepare", "hb_query_free", "hb_query_lut_info",
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
-    "hb_scan_elapsed_ms", "hb_scan_free", "hb_search",
+    "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
 ]
 
 
@@ -102,6 +102,7 @@ def _lib():
         L.hb_scan_results.argtypes = [vp, vp, vp, u64, C.POINTER(u64)]
         L.hb_scan_elapsed_ms.argtypes = [vp, C.POINTER(C.c_float)]
         L.hb_scan_free.argtypes = [vp]
+        L.hb_diag_copy.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
         L.hb_search.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, u64,
                                 C.POINTER(u64)]
         _LIB = L

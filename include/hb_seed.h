@@ -174,6 +174,14 @@ int hb_search(hb_ctx *ctx, hb_db *db, const uint8_t *seq, int32_t len,
               const int32_t *xdrop, const int32_t *cutoff, const hb_params *params,
               hb_subject_stats *stats, hb_hsp *hsps, uint64_t cap, uint64_t *n_hsp);
 
+/* --- diagnostics ---------------------------------------------------------
+ * Copies `bytes` between two fresh device buffers `reps` times with a plain
+ * 16-byte-per-lane copy kernel and returns the mean time per copy: the
+ * known-size kernel used to calibrate rocprof byte counters and to measure
+ * the achievable HBM bandwidth.  No counterpart in the reference (its
+ * timing is get_gpu_walltime, gpu_cpu_common.h:71). */
+int hb_diag_copy(hb_ctx *ctx, uint64_t bytes, int reps, float *ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
