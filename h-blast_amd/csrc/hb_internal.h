@@ -10,7 +10,9 @@
 #define HB_NCELLS 32768     // 3 letters x 5 bits
 #define HB_PV_WORDS 1024    // HB_NCELLS / 32
 #define HB_WAVE 64
-#define HB_HITCAP 512       // flattened hits staged per wave in LDS
+#define HB_JOBCAP 128       // queued two-hit extensions per wave
+#define HB_BLKCAP 128       // hits waiting on a pending diagonal per wave
+#define HB_SCAP 4096        // subjects up to this length are staged in LDS
 #define HB_BLOCK 256        // 4 wavefronts per workgroup
 
 // Kernel arguments of the scan (one query group against one database).
@@ -34,7 +36,7 @@ struct hb_scan_args {
     int32_t n_ctx;
     const int8_t *mat32;         // 32x32 scores, -128 padded
     int32_t window;
-    int32_t diag_len;            // power of two >= qlen + window
+    int32_t diag_len;            // >= qlen + window, multiple of 64 (diagonal = (q - s) mod diag_len)
     uint32_t wave_lds;           // LDS bytes per wave
     // work queue and output
     uint32_t *tile_counter;
@@ -54,7 +56,7 @@ hipError_t hb_launch_lut_scan(uint32_t *count_to_start, uint32_t *total, hipStre
 hipError_t hb_launch_lut_fill(const uint8_t *query, int32_t qlen, const int32_t *word_ok,
                               const int8_t *mat32, int32_t threshold, const uint32_t *start,
                               uint32_t *entries, uint2 *cells, uint32_t *pv, hipStream_t st);
-hipError_t hb_launch_scan(const hb_scan_args &a, int n_blocks, size_t lds_bytes,
-                          hipStream_t st);
-hipError_t hb_scan_occupancy(size_t lds_bytes, int *blocks_per_cu);
+hipError_t hb_launch_scan(const hb_scan_args &a, int n_blocks, int block_threads,
+                          size_t lds_bytes, hipStream_t st);
+hipError_t hb_scan_occupancy(size_t lds_bytes, int block_threads, int *blocks_per_cu);
 hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);

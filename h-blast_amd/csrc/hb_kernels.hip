@@ -231,6 +231,243 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 }
 
 // ---------------------------------------------------------------- scan
+// Per-wavefront LDS of the scan (byte offsets from the wave's base):
+//   state[D]   int32  diagonal state: bit 31 = extended-right flag,
+//                     bit 30 = extension pending, bits 0..29 = last hit + offset
+//   marker[D]  uint32 round-ordering marker (generation << 6 | 63 - lane)
+//   seg[64]    uint32 segment starts of the current hit round
+//   jobs[kJobCap]  uint4 pending two-hit extensions (q, s, last hit, diagonal)
+//   blk[kBlkCap]   uint4 hits waiting on a pending diagonal (q, s, diagonal)
+//   sbuf[kSCap]    subject residues (subjects up to kSCap residues)
+constexpr int kJobCap = HB_JOBCAP;
+constexpr int kBlkCap = HB_BLKCAP;
+constexpr int kSCap = HB_SCAP;
+constexpr uint32_t kFlag = 0x80000000u;
+constexpr uint32_t kPend = 0x40000000u;
+constexpr uint32_t kVal = 0x3fffffffu;
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_max_scan(uint32_t v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < HB_WAVE; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, HB_WAVE);
+        if (lane >= o) v = max(v, t);
+    }
+    return v;
+}
+
+struct WaveScan {
+    const hb_scan_args *a;
+    const int8_t *mat;
+    const uint8_t *qry;
+    int32_t *state;
+    uint32_t *marker;
+    uint32_t *seg;
+    uint4 *jobs;
+    uint4 *blk;
+    uint8_t *sbuf;
+    int lane;
+    int32_t D;
+    int32_t window;
+    int32_t base;      // diagonal offset: stale entries read as "no recent hit"
+    uint32_t gen;      // marker generation
+    uint32_t n_jobs;   // wave-uniform
+    uint32_t n_blk;    // wave-uniform
+    uint32_t n_ext;    // per lane
+    uint32_t n_hsp;    // per lane
+
+    // Two-hit decisions for one set of hits (one per lane, active lanes in
+    // subject order by lane).  Hits of the set that share a diagonal are
+    // taken one per pass, lowest lane first (LDS atomic max on the marker),
+    // so every diagonal sees its hits in subject order.  A hit whose
+    // diagonal waits on a pending extension is appended to the blocked list.
+    __device__ __forceinline__ void decide(bool active, uint32_t q, int32_t so, uint32_t d)
+    {
+        while (__ballot(active)) {
+            if (++gen == (1u << 26)) {
+                for (int i = lane; i < D; i += HB_WAVE) marker[i] = 0;
+                wave_lds_sync();
+                gen = 1;
+            }
+            const uint32_t key = (gen << 6) | (uint32_t)(63 - lane);
+            if (active) atomicMax(&marker[d], key);
+            wave_lds_sync();
+            const bool win = active && marker[d] == key;
+            bool job = false, wait = false;
+            int32_t last_s = 0;
+            if (win) {
+                const int32_t st = state[d];
+                if ((uint32_t)st & kPend) {
+                    wait = true;
+                } else if (st < 0) {
+                    if (so + base >= (int32_t)((uint32_t)st & kVal)) state[d] = so + base;
+                } else {
+                    last_s = st - base;
+                    const int32_t diff = so - last_s;
+                    if (diff >= window) {
+                        state[d] = so + base;
+                    } else if (diff >= 3) {
+                        job = true;
+                        state[d] = (int32_t)kPend;
+                    }
+                }
+            }
+            const uint64_t mj = __ballot(job), mb = __ballot(wait);
+            if (job) jobs[n_jobs + lane_rank(mj)] = make_uint4(q, (uint32_t)so, (uint32_t)last_s, d);
+            if (wait) blk[n_blk + lane_rank(mb)] = make_uint4(q, (uint32_t)so, d, 0u);
+            n_jobs += (uint32_t)__popcll(mj);
+            n_blk += (uint32_t)__popcll(mb);
+            n_ext += job ? 1u : 0u;
+            active = active && !win;
+            wave_lds_sync();
+        }
+    }
+
+    // Runs every queued extension (one per lane), writes HSPs with score >=
+    // cutoff and the diagonals' new states.
+    template <typename SubjPtr>
+    __device__ __forceinline__ void run_jobs(SubjPtr S, int32_t slen, uint32_t oid)
+    {
+        for (uint32_t j0 = 0; j0 < n_jobs; j0 += HB_WAVE) {
+            const uint32_t j = j0 + (uint32_t)lane;
+            const bool v = j < n_jobs;
+            ext_result r = {0, 0, 0, 0, 0, 0};
+            int32_t k = 0;
+            uint4 jb = make_uint4(0, 0, 0, 0);
+            if (v) {
+                jb = jobs[j];
+                k = context_of(a->ctx_off, a->n_ctx, (int32_t)jb.x);
+                r = ext_two_hit(mat, S, slen, qry, a->qlen, (int32_t)jb.z + 3, (int32_t)jb.y,
+                                (int32_t)jb.x, a->ctx_xdrop[k]);
+                state[jb.w] = r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + base))
+                                      : (int32_t)jb.y + base;
+            }
+            const bool emit = v && r.score >= a->ctx_cutoff[k];
+            const uint64_t m = __ballot(emit);
+            if (m) {
+                uint32_t slot0 = 0;
+                if (lane == 0) slot0 = atomicAdd(a->out_count, (uint32_t)__popcll(m));
+                slot0 = __shfl(slot0, 0, HB_WAVE);
+                if (emit) {
+                    const uint32_t slot = slot0 + lane_rank(m);
+                    if (slot < a->out_cap) {
+                        hb_hsp o;
+                        o.oid = oid;
+                        o.context = k;
+                        o.q_start = r.q_start;
+                        o.s_start = r.s_start;
+                        o.q_off = (int32_t)jb.x;
+                        o.s_off = (int32_t)jb.y;
+                        o.len = r.len;
+                        o.score = r.score;
+                        a->out[slot] = o;
+                    }
+                }
+            }
+            n_hsp += emit ? 1u : 0u;
+        }
+        n_jobs = 0;
+        wave_lds_sync();
+    }
+
+    // Re-decides the blocked hits in their original order (in place: each
+    // round reads its 64 entries before any new entry is written, and new
+    // entries never outnumber the ones read).
+    __device__ __forceinline__ void replay()
+    {
+        const uint32_t nb = n_blk;
+        n_blk = 0;
+        for (uint32_t b0 = 0; b0 < nb; b0 += HB_WAVE) {
+            const uint32_t b = b0 + (uint32_t)lane;
+            const bool v = b < nb;
+            const uint4 e = v ? blk[b] : make_uint4(0, 0, 0, 0);
+            wave_lds_sync();
+            decide(v, e.x, (int32_t)e.y, e.z);
+        }
+    }
+
+    template <typename SubjPtr>
+    __device__ __forceinline__ void drain(SubjPtr S, int32_t slen, uint32_t oid)
+    {
+        while (n_jobs) {
+            run_jobs(S, slen, oid);
+            replay();
+        }
+    }
+
+    template <typename SubjPtr>
+    __device__ __forceinline__ void subject(SubjPtr S, int32_t slen, uint32_t oid)
+    {
+        uint32_t hits = 0;
+        n_ext = 0;
+        n_hsp = 0;
+        int32_t s0m = 0;  // s0 mod D
+        for (int32_t s0 = 0; s0 + 3 <= slen; s0 += HB_WAVE) {
+            const int32_t s = s0 + lane;
+            uint32_t cnt = 0, cst = 0;
+            if (s + 3 <= slen) {
+                const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
+                if ((sm_pv_of()[idx >> 5] >> (idx & 31)) & 1u) {
+                    const uint2 c = a->cells[idx];
+                    cst = c.x;
+                    cnt = c.y;
+                }
+            }
+            const uint32_t incl = wave_incl_scan(cnt, lane);
+            const uint32_t excl = incl - cnt;
+            const uint32_t H = __shfl(incl, HB_WAVE - 1, HB_WAVE);
+            hits += H;
+            int32_t sm = s0m + lane;
+            if (sm >= D) sm -= D;
+            uint32_t carry = 0;
+            for (uint32_t r0 = 0; r0 < H; r0 += HB_WAVE) {
+                if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain(S, slen, oid);
+                seg[lane] = 0;
+                wave_lds_sync();
+                if (cnt && excl >= r0 && excl < r0 + HB_WAVE) seg[excl - r0] = (uint32_t)lane + 1;
+                wave_lds_sync();
+                const uint32_t v = wave_max_scan(seg[lane], lane);
+                const uint32_t p = v ? v - 1 : carry;
+                carry = __shfl(p, HB_WAVE - 1, HB_WAVE);
+                const uint32_t h = r0 + (uint32_t)lane;
+                const bool active = h < H;
+                const uint32_t ex_p = __shfl(excl, (int)p, HB_WAVE);
+                const uint32_t cs_p = __shfl(cst, (int)p, HB_WAVE);
+                const int32_t sm_p = __shfl(sm, (int)p, HB_WAVE);
+                const uint32_t q = active ? a->entries[cs_p + (h - ex_p)] : 0u;
+                int32_t d = (int32_t)q - sm_p;
+                if (d < 0) d += D;
+                wave_lds_sync();
+                decide(active, q, s0 + (int32_t)p, (uint32_t)d);
+            }
+            s0m += HB_WAVE;
+            if (s0m >= D) s0m -= D;
+        }
+        drain(S, slen, oid);
+        const uint32_t e = wave_sum(n_ext), hs = wave_sum(n_hsp);
+        if (lane == 0) {
+            a->stats[3 * (size_t)oid + 0] = hits;
+            a->stats[3 * (size_t)oid + 1] = e;
+            a->stats[3 * (size_t)oid + 2] = hs;
+        }
+        if (base + slen + window >= (1 << 29)) {
+            for (int i = lane; i < D; i += HB_WAVE) state[i] = 0;
+            wave_lds_sync();
+            base = window;
+        } else {
+            base += slen + window;
+        }
+    }
+
+    const uint32_t *pv;
+    __device__ __forceinline__ const uint32_t *sm_pv_of() const { return pv; }
+};
+
 __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -240,24 +477,34 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
     uint32_t *sm_pv = reinterpret_cast<uint32_t *>(lds + 1024);
     uint8_t *sm_q = lds + 1024 + 4 * HB_PV_WORDS;
     unsigned char *wb = sm_q + a.qlen_pad + (size_t)wid * a.wave_lds;
-    // diag[2*d] = state (bit 31 flag, low bits last_hit + offset), diag[2*d+1] = marker
-    int32_t *diag = reinterpret_cast<int32_t *>(wb);
-    uint32_t *marker = reinterpret_cast<uint32_t *>(wb);
-    uint32_t *hitbuf = reinterpret_cast<uint32_t *>(wb + 8 * (size_t)a.diag_len);
+    const int32_t D = a.diag_len;
 
     for (int i = threadIdx.x; i < 256; i += blockDim.x)
         reinterpret_cast<uint32_t *>(sm_mat)[i] = reinterpret_cast<const uint32_t *>(a.mat32)[i];
     for (int i = threadIdx.x; i < HB_PV_WORDS; i += blockDim.x) sm_pv[i] = a.pv[i];
     for (int i = threadIdx.x; i < a.qlen_pad / 4; i += blockDim.x)
         reinterpret_cast<uint32_t *>(sm_q)[i] = reinterpret_cast<const uint32_t *>(a.query)[i];
-    for (int i = lane; i < 2 * a.diag_len; i += HB_WAVE) diag[i] = 0;
-    __syncthreads();
 
-    const int32_t window = a.window;
-    const int32_t mask = a.diag_len - 1;
-    const int32_t qlen = a.qlen;
-    int32_t base = window;   // diag offset: stale entries read as "no recent hit"
-    uint32_t gen = 0;        // marker generation
+    WaveScan w;
+    w.a = &a;
+    w.mat = sm_mat;
+    w.qry = sm_q;
+    w.pv = sm_pv;
+    w.state = reinterpret_cast<int32_t *>(wb);
+    w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)D);
+    w.seg = reinterpret_cast<uint32_t *>(wb + 8 * (size_t)D);
+    w.jobs = reinterpret_cast<uint4 *>(wb + 8 * (size_t)D + 256);
+    w.blk = w.jobs + kJobCap;
+    w.sbuf = reinterpret_cast<uint8_t *>(w.blk + kBlkCap);
+    w.lane = lane;
+    w.D = D;
+    w.window = a.window;
+    w.base = a.window;
+    w.gen = 0;
+    w.n_jobs = 0;
+    w.n_blk = 0;
+    for (int i = lane; i < 2 * D; i += HB_WAVE) w.state[i] = 0;   // state and marker
+    __syncthreads();
 
     for (;;) {
         uint32_t t = 0;
@@ -270,102 +517,12 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
             const uint64_t sb = a.offs[oid];
             const int32_t slen = (int32_t)(a.offs[oid + 1] - sb);
             const uint8_t *S = a.res + sb;
-            uint32_t hits = 0, n_ext = 0, n_hsp = 0;
-            for (int32_t s0 = 0; s0 + 3 <= slen; s0 += HB_WAVE) {
-                const int32_t s = s0 + lane;
-                uint32_t cnt = 0, cst = 0;
-                if (s + 3 <= slen) {
-                    const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
-                    if ((sm_pv[idx >> 5] >> (idx & 31)) & 1u) {
-                        const uint2 c = a.cells[idx];
-                        cst = c.x;
-                        cnt = c.y;
-                    }
-                }
-                const uint32_t incl = wave_incl_scan(cnt, lane);
-                const uint32_t excl = incl - cnt;
-                const uint32_t H = __shfl(incl, HB_WAVE - 1, HB_WAVE);
-                hits += H;
-                for (uint32_t c0 = 0; c0 < H; c0 += HB_HITCAP) {
-                    const uint32_t lo = max(c0, excl), hi = min(c0 + HB_HITCAP, incl);
-                    for (uint32_t f = lo; f < hi; ++f)
-                        hitbuf[f - c0] = (a.entries[cst + (f - excl)] << 6) | (uint32_t)lane;
-                    wave_lds_sync();
-                    const uint32_t Hc = min((uint32_t)HB_HITCAP, H - c0);
-                    for (uint32_t r0 = 0; r0 < Hc; r0 += HB_WAVE) {
-                        const uint32_t h = r0 + lane;
-                        bool active = h < Hc;
-                        const uint32_t e = active ? hitbuf[h] : 0u;
-                        const int32_t q = (int32_t)(e >> 6);
-                        const int32_t so = s0 + (int32_t)(e & 63);
-                        const int32_t d = (q - so) & mask;
-                        while (__ballot(active)) {
-                            if (++gen == (1u << 26)) {
-                                for (int i = lane; i < a.diag_len; i += HB_WAVE) marker[2 * i + 1] = 0;
-                                wave_lds_sync();
-                                gen = 1;
-                            }
-                            const uint32_t key = (gen << 6) | (uint32_t)(63 - lane);
-                            if (active) atomicMax(&marker[2 * d + 1], key);
-                            wave_lds_sync();
-                            const bool win = active && marker[2 * d + 1] == key;
-                            if (win) {
-                                const int32_t st = diag[2 * d];
-                                if (st < 0) {
-                                    if (so + base >= (st & 0x7fffffff)) diag[2 * d] = so + base;
-                                } else {
-                                    const int32_t last_s = st - base;
-                                    const int32_t diff = so - last_s;
-                                    if (diff >= window) {
-                                        diag[2 * d] = so + base;
-                                    } else if (diff >= 3) {
-                                        const int32_t k = context_of(a.ctx_off, a.n_ctx, q);
-                                        const ext_result r = ext_two_hit(sm_mat, S, slen, sm_q, qlen,
-                                                                         last_s + 3, so, q,
-                                                                         a.ctx_xdrop[k]);
-                                        ++n_ext;
-                                        if (r.score >= a.ctx_cutoff[k]) {
-                                            ++n_hsp;
-                                            const uint32_t slot = atomicAdd(a.out_count, 1u);
-                                            if (slot < a.out_cap) {
-                                                hb_hsp o;
-                                                o.oid = oid;
-                                                o.context = k;
-                                                o.q_start = r.q_start;
-                                                o.s_start = r.s_start;
-                                                o.q_off = q;
-                                                o.s_off = so;
-                                                o.len = r.len;
-                                                o.score = r.score;
-                                                a.out[slot] = o;
-                                            }
-                                        }
-                                        diag[2 * d] = r.right
-                                            ? (int32_t)(0x80000000u | (uint32_t)(r.s_last - 2 + base))
-                                            : so + base;
-                                    }
-                                }
-                            }
-                            active = active && !win;
-                            wave_lds_sync();
-                        }
-                    }
-                    wave_lds_sync();
-                }
-            }
-            n_ext = wave_sum(n_ext);
-            n_hsp = wave_sum(n_hsp);
-            if (lane == 0) {
-                a.stats[3 * (size_t)oid + 0] = hits;
-                a.stats[3 * (size_t)oid + 1] = n_ext;
-                a.stats[3 * (size_t)oid + 2] = n_hsp;
-            }
-            if (base + slen + window >= (1 << 30)) {
-                for (int i = lane; i < a.diag_len; i += HB_WAVE) diag[2 * i] = 0;
+            if (slen <= kSCap) {
+                for (int i = lane; i < slen; i += HB_WAVE) w.sbuf[i] = S[i];
                 wave_lds_sync();
-                base = window;
+                w.subject(w.sbuf, slen, oid);
             } else {
-                base += slen + window;
+                w.subject(S, slen, oid);
             }
         }
     }
@@ -431,20 +588,23 @@ hipError_t hb_launch_lut_fill(const uint8_t *qflag, int32_t qlen, const int32_t 
     return hipGetLastError();
 }
 
-hipError_t hb_launch_scan(const hb_scan_args &a, int n_blocks, size_t lds_bytes, hipStream_t st)
+hipError_t hb_launch_scan(const hb_scan_args &a, int n_blocks, int block_threads, size_t lds_bytes,
+                          hipStream_t st)
 {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(hb_scan_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(hb_scan_kernel, dim3(n_blocks), dim3(HB_BLOCK), lds_bytes, st, a);
+    if (block_threads < HB_WAVE || block_threads > HB_BLOCK || block_threads % HB_WAVE)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(hb_scan_kernel, dim3(n_blocks), dim3(block_threads), lds_bytes, st, a);
     return hipGetLastError();
 }
 
-hipError_t hb_scan_occupancy(size_t lds_bytes, int *blocks_per_cu)
+hipError_t hb_scan_occupancy(size_t lds_bytes, int block_threads, int *blocks_per_cu)
 {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(hb_scan_kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hb_scan_kernel, HB_BLOCK,
-                                                        lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hb_scan_kernel,
+                                                        block_threads, lds_bytes);
 }

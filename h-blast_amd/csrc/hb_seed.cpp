@@ -323,7 +323,12 @@ int hb_query_free(hb_query *q)
     return HB_OK;
 }
 
-static size_t scan_wave_lds(int32_t diag_len) { return 8 * (size_t)diag_len + 4 * HB_HITCAP; }
+// per wave: diagonal state + marker, 64 segment starts, job and blocked
+// queues (16 B entries), staged subject
+static size_t scan_wave_lds(int32_t diag_len)
+{
+    return 8 * (size_t)diag_len + 256 + 16 * (size_t)(HB_JOBCAP + HB_BLKCAP) + HB_SCAP;
+}
 static size_t scan_fixed_lds(int32_t qlen_pad) { return 1024 + 4 * HB_PV_WORDS + qlen_pad; }
 
 int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
@@ -359,8 +364,10 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
     q->n_ctx = n_ctx;
     q->window = p.window;
     q->threshold = p.threshold;
-    q->diag_len = 64;
-    while (q->diag_len < len + p.window) q->diag_len <<= 1;
+    // Diagonals are (q - s) mod diag_len.  Any diag_len >= qlen + window
+    // gives the same two-hit decisions as the CPU word finder's power-of-two
+    // table (two hits that alias are always more than a window apart).
+    q->diag_len = (len + p.window + 63) & ~63;
     if (scan_fixed_lds(q->qlen_pad) + scan_wave_lds(q->diag_len) > c->max_lds) {
         delete q;
         return HB_ERR_UNSUPPORTED;
@@ -468,12 +475,14 @@ static int launch_scan(hb_scan *s)
     hb_query *q = s->q;
     const size_t wave_lds = scan_wave_lds(q->diag_len);
     const size_t fixed = scan_fixed_lds(q->qlen_pad);
+    // 4 wavefronts per workgroup share the query tables; fewer when the
+    // diagonal arrays of long (e.g. six-frame) queries do not fit.
     int waves = 4;
-    while (waves > 1 && fixed + waves * wave_lds > c->max_lds) --waves;
+    while (waves > 1 && fixed + waves * wave_lds > c->max_lds) waves /= 2;
     const size_t lds = fixed + waves * wave_lds;
-    if (waves != 4) return HB_ERR_UNSUPPORTED;  // kernel assumes 4-wave blocks
+    if (lds > c->max_lds) return HB_ERR_UNSUPPORTED;
     int per_cu = 0;
-    HB_CHECK(hb_scan_occupancy(lds, &per_cu));
+    HB_CHECK(hb_scan_occupancy(lds, waves * HB_WAVE, &per_cu));
     if (per_cu < 1) per_cu = 1;
     int blocks = c->n_cu * per_cu;
     const int need = (int)((db->n_tiles + waves - 1) / waves);
@@ -507,7 +516,7 @@ static int launch_scan(hb_scan *s)
     a.stats = s->d_stats;
     HB_CHECK(hipMemsetAsync(s->d_misc, 0, 2 * sizeof(uint32_t), c->stream));
     HB_CHECK(hipEventRecord(s->ev0, c->stream));
-    if (db->n_tiles) HB_CHECK(hb_launch_scan(a, blocks, lds, c->stream));
+    if (db->n_tiles) HB_CHECK(hb_launch_scan(a, blocks, waves * HB_WAVE, lds, c->stream));
     HB_CHECK(hipEventRecord(s->ev1, c->stream));
     return HB_OK;
 }

@@ -104,3 +104,35 @@ def test_device_lut_matches_oracle(ctx):
     ostart, oent = oq.lut_arrays()
     assert np.array_equal(start, ostart)
     assert np.array_equal(ent, oent)
+
+
+def test_long_subjects_global_path(ctx):
+    # subjects longer than the LDS staging buffer take the global-memory path
+    q = hbseed.random_protein(31, 400)
+    parts = [hbseed.random_protein(32, 6000), hbseed.random_protein(33, 10000)]
+    for i, seed in enumerate((34, 35)):
+        s = hbseed.random_protein(seed, 9000)
+        h = hbseed.homolog(seed + 100, q[50:170], 0.6, 2)
+        s[4500 + i:4500 + i + len(h)] = h
+        parts.append(s)
+    res = np.concatenate(parts)
+    offs = np.zeros(len(parts) + 1, np.uint64)
+    offs[1:] = np.cumsum([len(p) for p in parts])
+    stats, hsps = compare(ctx, hbseed.blastp_query(q), res, offs)
+    assert len(hsps) > 0
+
+
+@pytest.mark.parametrize("nt_len", [600, 3000])
+def test_blastx_lengths(ctx, nt_len):
+    pq = hbseed.random_protein(41, 800)
+    res, offs = hbseed.make_db(40, 3000, plant_query=pq, plant_frac=0.1)
+    seg = hbseed.homolog(42, res[offs[5]:offs[5] + 60], 0.7, 0)
+    nt = hbseed.blastx_nt(43 + nt_len, nt_len, seg, hbseed.homolog(44, pq[100:160], 0.6, 0))
+    compare(ctx, hbseed.blastx_query(nt), res, offs)
+
+
+def test_long_concatenated_query(ctx):
+    # 12,000-residue query: diagonal arrays force one wavefront per workgroup
+    q = hbseed.random_protein(51, 12000)
+    res, offs = hbseed.make_db(50, 800, plant_query=q, plant_frac=0.2)
+    compare(ctx, hbseed.blastp_query(q), res, offs)
