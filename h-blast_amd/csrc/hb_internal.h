@@ -12,7 +12,7 @@
 #define HB_WAVE 64
 #define HB_JOBCAP 128       // queued two-hit extensions per wave
 #define HB_BLKCAP 128       // hits waiting on a pending diagonal per wave
-#define HB_SCAP 4096        // subjects up to this length are staged in LDS
+#define HB_SCAP 1024        // subjects up to this length are staged in LDS
 #define HB_BLOCK 256        // 4 wavefronts per workgroup
 
 // Kernel arguments of the scan (one query group against one database).
@@ -26,6 +26,7 @@ struct hb_scan_args {
     // query group
     const uint2 *cells;          // HB_NCELLS cells {start, count}
     const uint32_t *entries;     // query offsets
+    uint32_t n_entries;
     const uint32_t *pv;          // HB_PV_WORDS presence bits
     const uint8_t *query;        // qlen_pad bytes (zero padded)
     int32_t qlen;

@@ -30,22 +30,44 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
+// Wavefront-wide inclusive scans on DPP lane moves (VALU only): row_shr
+// 1/2/4/8 scans each row of 16 lanes, row_bcast:15 and row_bcast:31 carry the
+// row totals into the following rows.  Lanes outside a row read 0, the
+// identity of both operations on unsigned values.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp(uint32_t v)
 {
-#pragma unroll
-    for (int o = 1; o < HB_WAVE; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, HB_WAVE);
-        if (lane >= o) v += t;
-    }
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_add(uint32_t v)
+{
+    v += dpp<0x111, 0xf>(v);  // row_shr:1
+    v += dpp<0x112, 0xf>(v);  // row_shr:2
+    v += dpp<0x114, 0xf>(v);  // row_shr:4
+    v += dpp<0x118, 0xf>(v);  // row_shr:8
+    v += dpp<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, HB_WAVE);
+    v = max(v, dpp<0x111, 0xf>(v));
+    v = max(v, dpp<0x112, 0xf>(v));
+    v = max(v, dpp<0x114, 0xf>(v));
+    v = max(v, dpp<0x118, 0xf>(v));
+    v = max(v, dpp<0x142, 0xa>(v));
+    v = max(v, dpp<0x143, 0xc>(v));
     return v;
 }
+
+__device__ __forceinline__ uint32_t lane63(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return lane63(wave_incl_add(v)); }
 
 // ---------------------------------------------------------------- residues
 __global__ void check_residues_kernel(const uint8_t *res, uint64_t n, uint32_t *flag)
@@ -140,41 +162,83 @@ __global__ __launch_bounds__(1024) void lut_scan_kernel(uint32_t *cnt, uint32_t 
 
 // ---------------------------------------------------------------- extensions
 // Left X-drop walk from (s_off, q_off) down to offset 0
-// (h_blast_kernel.cu:1329-1540).  Returns the best score; *len = number of
-// positions kept (0 if no prefix improved on maxscore).
-__device__ __forceinline__ int32_t ext_left(const int8_t *m, const uint8_t *S,
-                                            const uint8_t *Q, int32_t s_off, int32_t q_off,
-                                            int32_t X, int32_t maxscore, int32_t *len)
+// (h_blast_kernel.cu:1329-1540): positions k = 0..n, n = min(s_off, q_off);
+// strict '>' keeps the first best; stops once max - score >= X.  Returns the
+// best score; *len = positions kept (0 if no prefix improved on maxscore).
+// Residues are fetched four steps at a time so that the LDS/L1 latency is
+// paid once per four steps.
+template <typename SP>
+__device__ __forceinline__ int32_t ext_left(const int8_t *m, SP S, const uint8_t *Q,
+                                            int32_t s_off, int32_t q_off, int32_t X,
+                                            int32_t maxscore, int32_t *len)
 {
     const int32_t n = min(s_off, q_off);
-    int32_t best = -1, sc = maxscore;
-    for (int32_t k = 0; k <= n; ++k) {
-        sc += m[((uint32_t)Q[q_off - k] << 5) | S[s_off - k]];
-        if (sc > maxscore) {
-            maxscore = sc;
-            best = k;
+    int32_t best = -1, sc = maxscore, k = 0;
+    bool done = false;
+    for (; k + 4 <= n + 1; k += 4) {
+        const int32_t m0 = m[((uint32_t)Q[q_off - k] << 5) | S[s_off - k]];
+        const int32_t m1 = m[((uint32_t)Q[q_off - k - 1] << 5) | S[s_off - k - 1]];
+        const int32_t m2 = m[((uint32_t)Q[q_off - k - 2] << 5) | S[s_off - k - 2]];
+        const int32_t m3 = m[((uint32_t)Q[q_off - k - 3] << 5) | S[s_off - k - 3]];
+        sc += m0;
+        if (sc > maxscore) { maxscore = sc; best = k; }
+        if (maxscore - sc >= X) { done = true; break; }
+        sc += m1;
+        if (sc > maxscore) { maxscore = sc; best = k + 1; }
+        if (maxscore - sc >= X) { done = true; break; }
+        sc += m2;
+        if (sc > maxscore) { maxscore = sc; best = k + 2; }
+        if (maxscore - sc >= X) { done = true; break; }
+        sc += m3;
+        if (sc > maxscore) { maxscore = sc; best = k + 3; }
+        if (maxscore - sc >= X) { done = true; break; }
+    }
+    if (!done) {
+        for (; k <= n; ++k) {
+            sc += m[((uint32_t)Q[q_off - k] << 5) | S[s_off - k]];
+            if (sc > maxscore) { maxscore = sc; best = k; }
+            if (maxscore - sc >= X) break;
         }
-        if (maxscore - sc >= X) break;
     }
     *len = best + 1;
     return maxscore;
 }
 
-// Right X-drop walk (h_blast_kernel.cu:1133-1326): also stops at score <= 0.
-__device__ __forceinline__ int32_t ext_right(const int8_t *m, const uint8_t *S, int32_t slen,
-                                             const uint8_t *Q, int32_t qlen, int32_t s_off,
-                                             int32_t q_off, int32_t X, int32_t maxscore,
-                                             int32_t *len, int32_t *s_last)
+// Right X-drop walk (h_blast_kernel.cu:1133-1326): positions i = 0..n-1,
+// n = min(slen - s_off, qlen - q_off); also stops at score <= 0;
+// *s_last = s_off + stopping index (n if the walk ran out).
+template <typename SP>
+__device__ __forceinline__ int32_t ext_right(const int8_t *m, SP S, int32_t slen, const uint8_t *Q,
+                                             int32_t qlen, int32_t s_off, int32_t q_off, int32_t X,
+                                             int32_t maxscore, int32_t *len, int32_t *s_last)
 {
     const int32_t n = min(slen - s_off, qlen - q_off);
-    int32_t best = -1, sc = maxscore, i;
-    for (i = 0; i < n; ++i) {
-        sc += m[((uint32_t)Q[q_off + i] << 5) | S[s_off + i]];
-        if (sc > maxscore) {
-            maxscore = sc;
-            best = i;
+    int32_t best = -1, sc = maxscore, i = 0;
+    bool done = false;
+    for (; i + 4 <= n; i += 4) {
+        const int32_t m0 = m[((uint32_t)Q[q_off + i] << 5) | S[s_off + i]];
+        const int32_t m1 = m[((uint32_t)Q[q_off + i + 1] << 5) | S[s_off + i + 1]];
+        const int32_t m2 = m[((uint32_t)Q[q_off + i + 2] << 5) | S[s_off + i + 2]];
+        const int32_t m3 = m[((uint32_t)Q[q_off + i + 3] << 5) | S[s_off + i + 3]];
+        sc += m0;
+        if (sc > maxscore) { maxscore = sc; best = i; }
+        if (sc <= 0 || maxscore - sc >= X) { done = true; break; }
+        sc += m1;
+        if (sc > maxscore) { maxscore = sc; best = i + 1; }
+        if (sc <= 0 || maxscore - sc >= X) { i += 1; done = true; break; }
+        sc += m2;
+        if (sc > maxscore) { maxscore = sc; best = i + 2; }
+        if (sc <= 0 || maxscore - sc >= X) { i += 2; done = true; break; }
+        sc += m3;
+        if (sc > maxscore) { maxscore = sc; best = i + 3; }
+        if (sc <= 0 || maxscore - sc >= X) { i += 3; done = true; break; }
+    }
+    if (!done) {
+        for (; i < n; ++i) {
+            sc += m[((uint32_t)Q[q_off + i] << 5) | S[s_off + i]];
+            if (sc > maxscore) { maxscore = sc; best = i; }
+            if (sc <= 0 || maxscore - sc >= X) break;
         }
-        if (sc <= 0 || maxscore - sc >= X) break;
     }
     *len = best + 1;
     *s_last = s_off + i;
@@ -185,8 +249,12 @@ struct ext_result {
     int32_t q_start, s_start, len, score, right, s_last;
 };
 
-// Two-hit extension (h_blast_kernel.cu:1069-1130).
-__device__ __forceinline__ ext_result ext_two_hit(const int8_t *m, const uint8_t *S, int32_t slen,
+// Two-hit extension (h_blast_kernel.cu:1069-1130): start just after the
+// best-scoring prefix of the second word, extend left from maxscore 0, and
+// extend right from the left score only if the left walk reached
+// s_left_off (the end of the first hit's word).
+template <typename SP>
+__device__ __forceinline__ ext_result ext_two_hit(const int8_t *m, SP S, int32_t slen,
                                                   const uint8_t *Q, int32_t qlen,
                                                   int32_t s_left_off, int32_t s_right_off,
                                                   int32_t q_right_off, int32_t X)
@@ -231,14 +299,14 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 }
 
 // ---------------------------------------------------------------- scan
-// Per-wavefront LDS of the scan (byte offsets from the wave's base):
+// Per-wavefront LDS of the scan:
 //   state[D]   int32  diagonal state: bit 31 = extended-right flag,
 //                     bit 30 = extension pending, bits 0..29 = last hit + offset
-//   marker[D]  uint32 round-ordering marker (generation << 6 | 63 - lane)
-//   seg[64]    uint32 segment starts of the current hit round
-//   jobs[kJobCap]  uint4 pending two-hit extensions (q, s, last hit, diagonal)
-//   blk[kBlkCap]   uint4 hits waiting on a pending diagonal (q, s, diagonal)
-//   sbuf[kSCap]    subject residues (subjects up to kSCap residues)
+//   marker[D]  uint32 same-round ordering marker (generation << 6 | 63 - lane)
+//   seg[64]    uint32 tagged segment starts of the current hit round
+//   jobs[kJobCap] uint2 pending two-hit extensions (q | (s - last) << 24, s)
+//   blk[kBlkCap]  uint2 hits waiting on a pending diagonal (q, s)
+//   sbuf[kSCap]   residues of the current subject (if it fits)
 constexpr int kJobCap = HB_JOBCAP;
 constexpr int kBlkCap = HB_BLKCAP;
 constexpr int kSCap = HB_SCAP;
@@ -251,44 +319,49 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-__device__ __forceinline__ uint32_t wave_max_scan(uint32_t v, int lane)
-{
-#pragma unroll
-    for (int o = 1; o < HB_WAVE; o <<= 1) {
-        const uint32_t t = __shfl_up(v, o, HB_WAVE);
-        if (lane >= o) v = max(v, t);
-    }
-    return v;
-}
-
 struct WaveScan {
     const hb_scan_args *a;
     const int8_t *mat;
     const uint8_t *qry;
+    const uint32_t *pv;
     int32_t *state;
     uint32_t *marker;
     uint32_t *seg;
-    uint4 *jobs;
-    uint4 *blk;
+    uint2 *jobs;
+    uint2 *blk;
     uint8_t *sbuf;
     int lane;
     int32_t D;
     int32_t window;
     int32_t base;      // diagonal offset: stale entries read as "no recent hit"
     uint32_t gen;      // marker generation
+    uint32_t rgen;     // segment-tag generation
     uint32_t n_jobs;   // wave-uniform
     uint32_t n_blk;    // wave-uniform
     uint32_t n_ext;    // per lane
     uint32_t n_hsp;    // per lane
 
-    // Two-hit decisions for one set of hits (one per lane, active lanes in
-    // subject order by lane).  Hits of the set that share a diagonal are
-    // taken one per pass, lowest lane first (LDS atomic max on the marker),
-    // so every diagonal sees its hits in subject order.  A hit whose
-    // diagonal waits on a pending extension is appended to the blocked list.
+    __device__ __forceinline__ uint32_t diag_of(uint32_t q, int32_t so) const
+    {
+        int32_t d = (int32_t)q - so % D;
+        return (uint32_t)(d < 0 ? d + D : d);
+    }
+
+    // Two-hit decisions for one set of hits (at most one per lane; active
+    // lanes are in subject order by lane).  Hits of the set that share a
+    // diagonal are taken one per pass, lowest lane first (LDS atomic max on
+    // the marker), so every diagonal sees its hits in subject order.  A hit
+    // whose diagonal waits on a pending extension joins the blocked list.
+    // Per hit (state per diagonal, flagged = extended to the right):
+    //   pending            -> wait
+    //   flagged            -> s < last: skip; else last = s, unflag
+    //   diff >= window     -> last = s
+    //   diff < 3           -> skip
+    //   otherwise          -> extension (diagonal pending until it ran)
     __device__ __forceinline__ void decide(bool active, uint32_t q, int32_t so, uint32_t d)
     {
-        while (__ballot(active)) {
+        for (;;) {
+            if (!__ballot(active)) break;
             if (++gen == (1u << 26)) {
                 for (int i = lane; i < D; i += HB_WAVE) marker[i] = 0;
                 wave_lds_sync();
@@ -297,29 +370,26 @@ struct WaveScan {
             const uint32_t key = (gen << 6) | (uint32_t)(63 - lane);
             if (active) atomicMax(&marker[d], key);
             wave_lds_sync();
-            const bool win = active && marker[d] == key;
-            bool job = false, wait = false;
-            int32_t last_s = 0;
-            if (win) {
-                const int32_t st = state[d];
-                if ((uint32_t)st & kPend) {
-                    wait = true;
-                } else if (st < 0) {
-                    if (so + base >= (int32_t)((uint32_t)st & kVal)) state[d] = so + base;
-                } else {
-                    last_s = st - base;
-                    const int32_t diff = so - last_s;
-                    if (diff >= window) {
-                        state[d] = so + base;
-                    } else if (diff >= 3) {
-                        job = true;
-                        state[d] = (int32_t)kPend;
-                    }
-                }
-            }
+            const uint32_t mk = marker[d];
+            const int32_t st = state[d];
+            const bool win = active && mk == key;
+            const uint32_t ust = (uint32_t)st;
+            const bool pend = (ust & kPend) != 0;
+            const bool flag = st < 0;
+            const int32_t sb = so + base;
+            const int32_t diff = (int32_t)((uint32_t)sb - ust);  // s - last if unflagged
+            const bool job = win && !pend && !flag && diff >= 3 && diff < window;
+            const bool wait = win && pend;
+            int32_t nst;
+            if (flag) nst = sb >= (int32_t)(ust & kVal) ? sb : st;
+            else nst = diff >= window ? sb : (job ? (int32_t)kPend : st);
+            if (win && !pend && nst != st) state[d] = nst;
             const uint64_t mj = __ballot(job), mb = __ballot(wait);
-            if (job) jobs[n_jobs + lane_rank(mj)] = make_uint4(q, (uint32_t)so, (uint32_t)last_s, d);
-            if (wait) blk[n_blk + lane_rank(mb)] = make_uint4(q, (uint32_t)so, d, 0u);
+            if (job) {
+                const uint32_t delta = (uint32_t)diff;  // 3..window-1
+                jobs[n_jobs + lane_rank(mj)] = make_uint2(q | (delta << 24), (uint32_t)so);
+            }
+            if (wait) blk[n_blk + lane_rank(mb)] = make_uint2(q, (uint32_t)so);
             n_jobs += (uint32_t)__popcll(mj);
             n_blk += (uint32_t)__popcll(mb);
             n_ext += job ? 1u : 0u;
@@ -330,29 +400,30 @@ struct WaveScan {
 
     // Runs every queued extension (one per lane), writes HSPs with score >=
     // cutoff and the diagonals' new states.
-    template <typename SubjPtr>
-    __device__ __forceinline__ void run_jobs(SubjPtr S, int32_t slen, uint32_t oid)
+    template <typename SP>
+    __device__ __forceinline__ void run_jobs(SP S, int32_t slen, uint32_t oid)
     {
         for (uint32_t j0 = 0; j0 < n_jobs; j0 += HB_WAVE) {
             const uint32_t j = j0 + (uint32_t)lane;
             const bool v = j < n_jobs;
             ext_result r = {0, 0, 0, 0, 0, 0};
-            int32_t k = 0;
-            uint4 jb = make_uint4(0, 0, 0, 0);
+            int32_t k = 0, q = 0, so = 0;
             if (v) {
-                jb = jobs[j];
-                k = context_of(a->ctx_off, a->n_ctx, (int32_t)jb.x);
-                r = ext_two_hit(mat, S, slen, qry, a->qlen, (int32_t)jb.z + 3, (int32_t)jb.y,
-                                (int32_t)jb.x, a->ctx_xdrop[k]);
-                state[jb.w] = r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + base))
-                                      : (int32_t)jb.y + base;
+                const uint2 jb = jobs[j];
+                q = (int32_t)(jb.x & 0xffffffu);
+                so = (int32_t)jb.y;
+                const int32_t last_s = so - (int32_t)(jb.x >> 24);
+                k = context_of(a->ctx_off, a->n_ctx, q);
+                r = ext_two_hit(mat, S, slen, qry, a->qlen, last_s + 3, so, q, a->ctx_xdrop[k]);
+                state[diag_of((uint32_t)q, so)] =
+                    r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + base)) : so + base;
             }
             const bool emit = v && r.score >= a->ctx_cutoff[k];
             const uint64_t m = __ballot(emit);
             if (m) {
                 uint32_t slot0 = 0;
                 if (lane == 0) slot0 = atomicAdd(a->out_count, (uint32_t)__popcll(m));
-                slot0 = __shfl(slot0, 0, HB_WAVE);
+                slot0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)slot0);
                 if (emit) {
                     const uint32_t slot = slot0 + lane_rank(m);
                     if (slot < a->out_cap) {
@@ -361,8 +432,8 @@ struct WaveScan {
                         o.context = k;
                         o.q_start = r.q_start;
                         o.s_start = r.s_start;
-                        o.q_off = (int32_t)jb.x;
-                        o.s_off = (int32_t)jb.y;
+                        o.q_off = q;
+                        o.s_off = so;
                         o.len = r.len;
                         o.score = r.score;
                         a->out[slot] = o;
@@ -385,14 +456,14 @@ struct WaveScan {
         for (uint32_t b0 = 0; b0 < nb; b0 += HB_WAVE) {
             const uint32_t b = b0 + (uint32_t)lane;
             const bool v = b < nb;
-            const uint4 e = v ? blk[b] : make_uint4(0, 0, 0, 0);
+            const uint2 e = v ? blk[b] : make_uint2(0, 0);
             wave_lds_sync();
-            decide(v, e.x, (int32_t)e.y, e.z);
+            decide(v, e.x, (int32_t)e.y, diag_of(e.x, (int32_t)e.y));
         }
     }
 
-    template <typename SubjPtr>
-    __device__ __forceinline__ void drain(SubjPtr S, int32_t slen, uint32_t oid)
+    template <typename SP>
+    __device__ __forceinline__ void drain(SP S, int32_t slen, uint32_t oid)
     {
         while (n_jobs) {
             run_jobs(S, slen, oid);
@@ -400,47 +471,52 @@ struct WaveScan {
         }
     }
 
-    template <typename SubjPtr>
-    __device__ __forceinline__ void subject(SubjPtr S, int32_t slen, uint32_t oid)
+    template <typename SP>
+    __device__ __forceinline__ void subject(SP S, int32_t slen, uint32_t oid)
     {
         uint32_t hits = 0;
         n_ext = 0;
         n_hsp = 0;
+        const uint32_t n_entries = a->n_entries;
         int32_t s0m = 0;  // s0 mod D
         for (int32_t s0 = 0; s0 + 3 <= slen; s0 += HB_WAVE) {
             const int32_t s = s0 + lane;
             uint32_t cnt = 0, cst = 0;
             if (s + 3 <= slen) {
                 const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
-                if ((sm_pv_of()[idx >> 5] >> (idx & 31)) & 1u) {
+                if ((pv[idx >> 5] >> (idx & 31)) & 1u) {
                     const uint2 c = a->cells[idx];
                     cst = c.x;
                     cnt = c.y;
                 }
             }
-            const uint32_t incl = wave_incl_scan(cnt, lane);
+            const uint32_t incl = wave_incl_add(cnt);
             const uint32_t excl = incl - cnt;
-            const uint32_t H = __shfl(incl, HB_WAVE - 1, HB_WAVE);
+            const uint32_t H = lane63(incl);
+            const int32_t ebase = (int32_t)cst - (int32_t)excl;  // entry index = flat index + ebase
             hits += H;
-            int32_t sm = s0m + lane;
-            if (sm >= D) sm -= D;
             uint32_t carry = 0;
             for (uint32_t r0 = 0; r0 < H; r0 += HB_WAVE) {
                 if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain(S, slen, oid);
-                seg[lane] = 0;
+                if (++rgen == (1u << 25)) {
+                    seg[lane] = 0;
+                    rgen = 1;
+                }
+                const uint32_t tag = rgen << 7;
+                if (cnt && excl >= r0 && excl < r0 + HB_WAVE) seg[excl - r0] = tag | ((uint32_t)lane + 1);
                 wave_lds_sync();
-                if (cnt && excl >= r0 && excl < r0 + HB_WAVE) seg[excl - r0] = (uint32_t)lane + 1;
-                wave_lds_sync();
-                const uint32_t v = wave_max_scan(seg[lane], lane);
-                const uint32_t p = v ? v - 1 : carry;
-                carry = __shfl(p, HB_WAVE - 1, HB_WAVE);
+                const uint32_t sv = seg[lane];
+                const uint32_t v = wave_incl_max((sv & ~127u) == tag ? (sv & 127u) : 0u);
+                const uint32_t p = min(v ? v - 1 : carry, 63u);
+                carry = lane63(p);
                 const uint32_t h = r0 + (uint32_t)lane;
                 const bool active = h < H;
-                const uint32_t ex_p = __shfl(excl, (int)p, HB_WAVE);
-                const uint32_t cs_p = __shfl(cst, (int)p, HB_WAVE);
-                const int32_t sm_p = __shfl(sm, (int)p, HB_WAVE);
-                const uint32_t q = active ? a->entries[cs_p + (h - ex_p)] : 0u;
-                int32_t d = (int32_t)q - sm_p;
+                const int32_t eb = __shfl(ebase, (int)p, HB_WAVE);
+                const uint32_t ei = min((uint32_t)((int32_t)h + eb), n_entries - 1);
+                const uint32_t q = active ? a->entries[ei] : 0u;
+                int32_t smp = s0m + (int32_t)p;
+                if (smp >= D) smp -= D;
+                int32_t d = (int32_t)q - smp;
                 if (d < 0) d += D;
                 wave_lds_sync();
                 decide(active, q, s0 + (int32_t)p, (uint32_t)d);
@@ -463,9 +539,6 @@ struct WaveScan {
             base += slen + window;
         }
     }
-
-    const uint32_t *pv;
-    __device__ __forceinline__ const uint32_t *sm_pv_of() const { return pv; }
 };
 
 __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
@@ -493,7 +566,7 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
     w.state = reinterpret_cast<int32_t *>(wb);
     w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)D);
     w.seg = reinterpret_cast<uint32_t *>(wb + 8 * (size_t)D);
-    w.jobs = reinterpret_cast<uint4 *>(wb + 8 * (size_t)D + 256);
+    w.jobs = reinterpret_cast<uint2 *>(wb + 8 * (size_t)D + 256);
     w.blk = w.jobs + kJobCap;
     w.sbuf = reinterpret_cast<uint8_t *>(w.blk + kBlkCap);
     w.lane = lane;
@@ -501,15 +574,16 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
     w.window = a.window;
     w.base = a.window;
     w.gen = 0;
+    w.rgen = 0;
     w.n_jobs = 0;
     w.n_blk = 0;
-    for (int i = lane; i < 2 * D; i += HB_WAVE) w.state[i] = 0;   // state and marker
+    for (int i = lane; i < 2 * D + 64; i += HB_WAVE) w.state[i] = 0;   // state, marker, seg
     __syncthreads();
 
     for (;;) {
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(a.tile_counter, 1u);
-        t = __shfl(t, 0, HB_WAVE);
+        t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
         if (t >= a.n_tiles) break;
         const uint32_t i_end = a.tiles[t + 1];
         for (uint32_t it = a.tiles[t]; it < i_end; ++it) {
@@ -521,6 +595,7 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
                 for (int i = lane; i < slen; i += HB_WAVE) w.sbuf[i] = S[i];
                 wave_lds_sync();
                 w.subject(w.sbuf, slen, oid);
+                wave_lds_sync();
             } else {
                 w.subject(S, slen, oid);
             }

@@ -324,10 +324,10 @@ int hb_query_free(hb_query *q)
 }
 
 // per wave: diagonal state + marker, 64 segment starts, job and blocked
-// queues (16 B entries), staged subject
+// queues (8 B entries), staged subject
 static size_t scan_wave_lds(int32_t diag_len)
 {
-    return 8 * (size_t)diag_len + 256 + 16 * (size_t)(HB_JOBCAP + HB_BLKCAP) + HB_SCAP;
+    return 8 * (size_t)diag_len + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + HB_SCAP;
 }
 static size_t scan_fixed_lds(int32_t qlen_pad) { return 1024 + 4 * HB_PV_WORDS + qlen_pad; }
 
@@ -342,8 +342,10 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
     hb_params_default(&p);
     if (params) p = *params;
     if (p.word_size != 3) return HB_ERR_UNSUPPORTED;
-    if (p.window <= 3 || p.window > (1 << 20)) return HB_ERR_ARG;
-    if (len > (1 << 25)) return HB_ERR_UNSUPPORTED;   // 26-bit query offsets in the hit list
+    // queued extensions pack the query offset in 24 bits and s - last_hit
+    // (< window) in 8 bits
+    if (p.window <= 3) return HB_ERR_ARG;
+    if (p.window > 255 || len >= (1 << 24)) return HB_ERR_UNSUPPORTED;
     for (int32_t i = 0; i < len; ++i)
         if (seq[i] >= HB_ALPHA) return HB_ERR_RESIDUE;
     std::vector<uint8_t> qflag((size_t)len);
@@ -497,6 +499,7 @@ static int launch_scan(hb_scan *s)
     a.n_tiles = db->n_tiles;
     a.cells = q->d_cells;
     a.entries = q->d_entries;
+    a.n_entries = std::max<uint32_t>(q->n_entries, 1);
     a.pv = q->d_pv;
     a.query = q->d_query;
     a.qlen = q->qlen;
