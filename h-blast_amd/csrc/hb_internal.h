@@ -35,6 +35,7 @@ struct hb_scan_args {
     const int32_t *ctx_xdrop;
     const int32_t *ctx_cutoff;
     int32_t n_ctx;
+    int32_t ctx_pad;             // n_ctx rounded up to 4 (LDS table stride)
     const int8_t *mat32;         // 32x32 scores, -128 padded
     int32_t window;
     int32_t diag_len;            // >= qlen + window, multiple of 64 (diagonal = (q - s) mod diag_len)

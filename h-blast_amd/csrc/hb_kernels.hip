@@ -323,6 +323,7 @@ struct WaveScan {
     const hb_scan_args *a;
     const int8_t *mat;
     const uint8_t *qry;
+    const int32_t *ctx;   // LDS: ctx_off[ctx_pad], xdrop[ctx_pad], cutoff[ctx_pad]
     const uint32_t *pv;
     int32_t *state;
     uint32_t *marker;
@@ -413,12 +414,12 @@ struct WaveScan {
                 q = (int32_t)(jb.x & 0xffffffu);
                 so = (int32_t)jb.y;
                 const int32_t last_s = so - (int32_t)(jb.x >> 24);
-                k = context_of(a->ctx_off, a->n_ctx, q);
-                r = ext_two_hit(mat, S, slen, qry, a->qlen, last_s + 3, so, q, a->ctx_xdrop[k]);
+                k = context_of(ctx, a->n_ctx, q);
+                r = ext_two_hit(mat, S, slen, qry, a->qlen, last_s + 3, so, q, ctx[a->ctx_pad + k]);
                 state[diag_of((uint32_t)q, so)] =
                     r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + base)) : so + base;
             }
-            const bool emit = v && r.score >= a->ctx_cutoff[k];
+            const bool emit = v && r.score >= ctx[2 * a->ctx_pad + k];
             const uint64_t m = __ballot(emit);
             if (m) {
                 uint32_t slot0 = 0;
@@ -548,7 +549,8 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
     const int wid = threadIdx.x / HB_WAVE;
     int8_t *sm_mat = reinterpret_cast<int8_t *>(lds);
     uint32_t *sm_pv = reinterpret_cast<uint32_t *>(lds + 1024);
-    uint8_t *sm_q = lds + 1024 + 4 * HB_PV_WORDS;
+    int32_t *sm_ctx = reinterpret_cast<int32_t *>(lds + 1024 + 4 * HB_PV_WORDS);
+    uint8_t *sm_q = reinterpret_cast<uint8_t *>(sm_ctx + 3 * a.ctx_pad);
     unsigned char *wb = sm_q + a.qlen_pad + (size_t)wid * a.wave_lds;
     const int32_t D = a.diag_len;
 
@@ -557,11 +559,17 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
     for (int i = threadIdx.x; i < HB_PV_WORDS; i += blockDim.x) sm_pv[i] = a.pv[i];
     for (int i = threadIdx.x; i < a.qlen_pad / 4; i += blockDim.x)
         reinterpret_cast<uint32_t *>(sm_q)[i] = reinterpret_cast<const uint32_t *>(a.query)[i];
+    for (int i = threadIdx.x; i < a.n_ctx; i += blockDim.x) {
+        sm_ctx[i] = a.ctx_off[i];
+        sm_ctx[a.ctx_pad + i] = a.ctx_xdrop[i];
+        sm_ctx[2 * a.ctx_pad + i] = a.ctx_cutoff[i];
+    }
 
     WaveScan w;
     w.a = &a;
     w.mat = sm_mat;
     w.qry = sm_q;
+    w.ctx = sm_ctx;
     w.pv = sm_pv;
     w.state = reinterpret_cast<int32_t *>(wb);
     w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)D);

@@ -329,7 +329,12 @@ static size_t scan_wave_lds(int32_t diag_len)
 {
     return 8 * (size_t)diag_len + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + HB_SCAP;
 }
-static size_t scan_fixed_lds(int32_t qlen_pad) { return 1024 + 4 * HB_PV_WORDS + qlen_pad; }
+// per workgroup: score matrix, presence vector, context tables, query
+static int32_t ctx_pad_of(int32_t n_ctx) { return (n_ctx + 3) & ~3; }
+static size_t scan_fixed_lds(int32_t qlen_pad, int32_t n_ctx)
+{
+    return 1024 + 4 * HB_PV_WORDS + 12 * (size_t)ctx_pad_of(n_ctx) + qlen_pad;
+}
 
 int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
                      const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
@@ -370,7 +375,7 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
     // gives the same two-hit decisions as the CPU word finder's power-of-two
     // table (two hits that alias are always more than a window apart).
     q->diag_len = (len + p.window + 63) & ~63;
-    if (scan_fixed_lds(q->qlen_pad) + scan_wave_lds(q->diag_len) > c->max_lds) {
+    if (scan_fixed_lds(q->qlen_pad, n_ctx) + scan_wave_lds(q->diag_len) > c->max_lds) {
         delete q;
         return HB_ERR_UNSUPPORTED;
     }
@@ -476,7 +481,7 @@ static int launch_scan(hb_scan *s)
     hb_db *db = s->db;
     hb_query *q = s->q;
     const size_t wave_lds = scan_wave_lds(q->diag_len);
-    const size_t fixed = scan_fixed_lds(q->qlen_pad);
+    const size_t fixed = scan_fixed_lds(q->qlen_pad, q->n_ctx);
     // 4 wavefronts per workgroup share the query tables; fewer when the
     // diagonal arrays of long (e.g. six-frame) queries do not fit.
     int waves = 4;
@@ -508,6 +513,7 @@ static int launch_scan(hb_scan *s)
     a.ctx_xdrop = q->d_ctx + q->n_ctx;
     a.ctx_cutoff = q->d_ctx + 2 * q->n_ctx;
     a.n_ctx = q->n_ctx;
+    a.ctx_pad = ctx_pad_of(q->n_ctx);
     a.mat32 = q->d_mat;
     a.window = q->window;
     a.diag_len = q->diag_len;
