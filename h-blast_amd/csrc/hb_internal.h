@@ -13,52 +13,87 @@
 #define HB_JOBCAP 128       // queued two-hit extensions per wave
 #define HB_BLKCAP 128       // hits waiting on a pending diagonal per wave
 #define HB_SCAP 1024        // subjects up to this length are staged in LDS
-#define HB_BLOCK 256        // 4 wavefronts per workgroup
+#define HB_BLOCK 256        // at most 4 wavefronts per workgroup
 
-// Kernel arguments of the scan (one query group against one database).
+// One query group: a concatenated query (one or more queries / frames as
+// contexts) with its lookup table.  The plain path scans one group; the
+// batched path scans many groups in one launch.
+struct hb_group_desc {
+    const uint2 *cells;          // HB_NCELLS cells {start, count}
+    const uint32_t *entries;     // query offsets, ascending per cell
+    const uint32_t *pv;          // HB_PV_WORDS presence bits
+    const uint8_t *query;        // qlen_pad bytes (zero padded)
+    const int32_t *ctx;          // ctx_off[n_ctx] | xdrop[n_ctx] | cutoff[n_ctx]
+    int32_t qlen;
+    int32_t qlen_pad;            // multiple of 16
+    int32_t n_ctx;
+    int32_t diag_len;            // >= qlen + window, multiple of 64
+    uint32_t n_entries;
+    int32_t ctx_base;            // added to context indices in HSP records
+    int32_t pad[2];
+};
+
+// Kernel arguments of the scan.
 struct hb_scan_args {
     // database (resident)
     const uint8_t *res;          // all residues
     const uint64_t *offs;        // n+1 subject offsets
     const uint32_t *order;       // processing order (length-descending oids)
-    const uint32_t *tiles;       // n_tiles+1 ranges into order
-    uint32_t n_tiles;
-    // query group
-    const uint2 *cells;          // HB_NCELLS cells {start, count}
-    const uint32_t *entries;     // query offsets
-    uint32_t n_entries;
-    const uint32_t *pv;          // HB_PV_WORDS presence bits
-    const uint8_t *query;        // qlen_pad bytes (zero padded)
-    int32_t qlen;
-    int32_t qlen_pad;
-    const int32_t *ctx_off;      // n_ctx
-    const int32_t *ctx_xdrop;
-    const int32_t *ctx_cutoff;
-    int32_t n_ctx;
-    int32_t ctx_pad;             // n_ctx rounded up to 4 (LDS table stride)
+    const uint32_t *tiles;       // tile boundaries into order
+    uint32_t tile_begin;         // tiles [tile_begin, tile_end) are scanned
+    uint32_t tile_end;
+    // query groups
+    const hb_group_desc *groups;
+    uint32_t n_groups;
+    // LDS layout (maxima over the groups of the launch)
+    int32_t qlen_pad_max;
+    int32_t ctx_pad_max;         // multiple of 4
+    int32_t diag_len_max;
+    uint32_t wave_lds;           // LDS bytes per wave
     const int8_t *mat32;         // 32x32 scores, -128 padded
     int32_t window;
-    int32_t diag_len;            // >= qlen + window, multiple of 64 (diagonal = (q - s) mod diag_len)
-    uint32_t wave_lds;           // LDS bytes per wave
     // work queue and output
-    uint32_t *tile_counter;
+    uint32_t *work_counter;
     hb_hsp *out;
     uint32_t out_cap;
     uint32_t *out_count;
-    uint32_t *stats;             // 3 per oid: lookup hits, extensions, HSPs
+    uint32_t *stats;             // plain: 3 per oid (lookup hits, extensions, HSPs); else null
+    unsigned long long *group_totals;  // batched: 3 per group; else null
 };
+
+// per wave: diagonal state + marker, 64 segment tags, job and blocked queues
+// (8 B entries), staged subject
+static inline size_t hb_scan_wave_lds(int32_t diag_len)
+{
+    return 8 * (size_t)diag_len + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + HB_SCAP;
+}
+static inline int32_t hb_ctx_pad(int32_t n_ctx) { return (n_ctx + 3) & ~3; }
+// per workgroup: score matrix, presence vector, context tables, query
+static inline size_t hb_scan_fixed_lds(int32_t qlen_pad, int32_t ctx_pad)
+{
+    return 1024 + 4 * HB_PV_WORDS + 12 * (size_t)ctx_pad + qlen_pad;
+}
 
 // Kernel launchers (hb_kernels.hip).
 hipError_t hb_launch_check_residues(const uint8_t *res, uint64_t n, uint32_t *flag,
                                     hipStream_t st);
-hipError_t hb_launch_lut_count(const uint8_t *query, int32_t qlen, const int32_t *word_ok,
-                               const int8_t *mat32, int32_t threshold, uint32_t *count,
-                               hipStream_t st);
-hipError_t hb_launch_lut_scan(uint32_t *count_to_start, uint32_t *total, hipStream_t st);
-hipError_t hb_launch_lut_fill(const uint8_t *query, int32_t qlen, const int32_t *word_ok,
-                              const int8_t *mat32, int32_t threshold, const uint32_t *start,
-                              uint32_t *entries, uint2 *cells, uint32_t *pv, hipStream_t st);
-hipError_t hb_launch_scan(const hb_scan_args &a, int n_blocks, int block_threads,
+// Lookup tables of n groups (grid.y = group).  qflag[q] = residue | 0x80 when
+// the 3-mer at q lies inside one context.
+struct hb_lut_job {
+    const uint8_t *qflag;
+    int32_t qlen;
+    int32_t pad;
+    uint32_t *count_start;       // HB_NCELLS + 1: counts, then exclusive starts
+    uint32_t *entries;
+    uint2 *cells;
+    uint32_t *pv;
+    uint32_t *total;             // total entries of the group
+};
+// pass 0: count; pass 1: exclusive scan + total; pass 2: fill cells, entries, pv
+hipError_t hb_launch_lut(const hb_lut_job *jobs_dev, int n_jobs, int32_t qlen_max,
+                         const int8_t *mat32, int32_t threshold, int pass, hipStream_t st);
+hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int block_threads,
                           size_t lds_bytes, hipStream_t st);
-hipError_t hb_scan_occupancy(size_t lds_bytes, int block_threads, int *blocks_per_cu);
+hipError_t hb_scan_occupancy(bool multi, size_t lds_bytes, int block_threads, int *blocks_per_cu);
 hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);
+hipError_t hb_launch_fill_u32(uint32_t *dst, uint32_t value, uint32_t n, hipStream_t st);

@@ -87,55 +87,57 @@ __global__ void check_residues_kernel(const uint8_t *res, uint64_t n, uint32_t *
 }
 
 // ---------------------------------------------------------------- lookup table
-// qflag[q] = residue | 0x80 when the 3-mer at q lies inside one context.
-// Cell w = (a<<10)|(b<<5)|c holds q when M[q0][a]+M[q1][b]+M[q2][c] >= T or
-// w is the query word itself (an exact word always seeds).
-template <bool FILL>
-__global__ __launch_bounds__(256) void lut_kernel(const uint8_t *qflag, int32_t qlen,
-                                                  const int8_t *mat32, int32_t threshold,
-                                                  uint32_t *count, const uint32_t *start,
-                                                  uint32_t *entries, uint2 *cells,
-                                                  uint32_t *pv)
+// Cell w = (a<<10)|(b<<5)|c holds query offset q (a word inside one context)
+// when M[q0][a]+M[q1][b]+M[q2][c] >= T, or when w is the query word itself
+// (an exact word always seeds, also below T).  One thread per cell, query
+// offsets visited ascending, so each cell lists them ascending (as
+// lut_sorting leaves them, h_blast.cu:165-194).  grid.y = group.
+template <int PASS>
+__global__ __launch_bounds__(256) void lut_kernel(const hb_lut_job *jobs, const int8_t *mat32,
+                                                  int32_t threshold)
 {
     extern __shared__ __align__(16) unsigned char lds[];
+    const hb_lut_job J = jobs[blockIdx.y];
     int8_t *sm = reinterpret_cast<int8_t *>(lds);        // 32x32
     uint8_t *sq = lds + 1024;                              // qlen bytes
     for (int i = threadIdx.x; i < 1024; i += blockDim.x) sm[i] = mat32[i];
-    for (int i = threadIdx.x; i < qlen; i += blockDim.x) sq[i] = qflag[i];
+    for (int i = threadIdx.x; i < J.qlen; i += blockDim.x) sq[i] = J.qflag[i];
     __syncthreads();
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= HB_NCELLS) return;
     const uint32_t a = w >> 10, b = (w >> 5) & 31, c = w & 31;
     uint32_t n = 0;
     if (a < HB_ALPHA && b < HB_ALPHA && c < HB_ALPHA) {
-        uint32_t k = FILL ? start[w] : 0;
-        for (int32_t q = 0; q + 3 <= qlen; ++q) {
+        uint32_t k = PASS == 2 ? J.count_start[w] : 0;
+        for (int32_t q = 0; q + 3 <= J.qlen; ++q) {
             const uint8_t f0 = sq[q];
             if (!(f0 & 0x80)) continue;
             const uint32_t q0 = f0 & 31, q1 = sq[q + 1] & 31, q2 = sq[q + 2] & 31;
-            const int32_t s = sm[(q0 << 5) | a] + sm[(q1 << 5) | b] + sm[(q2 << 5) | c];
+            const int32_t sc = sm[(q0 << 5) | a] + sm[(q1 << 5) | b] + sm[(q2 << 5) | c];
             const bool self = (q0 == a) & (q1 == b) & (q2 == c);
-            if (s >= threshold || self) {
-                if (FILL) entries[k++] = (uint32_t)q;
+            if (sc >= threshold || self) {
+                if (PASS == 2) J.entries[k++] = (uint32_t)q;
                 ++n;
             }
         }
     }
-    if (!FILL) {
-        count[w] = n;
+    if (PASS == 0) {
+        J.count_start[w] = n;
     } else {
-        cells[w] = make_uint2(start[w], n);
+        J.cells[w] = make_uint2(J.count_start[w], n);
         const uint64_t m = __ballot(n != 0);
         const int lane = threadIdx.x & 63;
-        if (lane == 0) pv[w >> 5] = (uint32_t)m;
-        if (lane == 32) pv[w >> 5] = (uint32_t)(m >> 32);
+        if (lane == 0) J.pv[w >> 5] = (uint32_t)m;
+        if (lane == 32) J.pv[w >> 5] = (uint32_t)(m >> 32);
     }
 }
 
-// Exclusive scan of HB_NCELLS counts in place; start[HB_NCELLS] = total.
-__global__ __launch_bounds__(1024) void lut_scan_kernel(uint32_t *cnt, uint32_t *total)
+// Exclusive scan of one group's HB_NCELLS counts in place; [HB_NCELLS] and
+// *total receive the number of entries.  grid.y = group.
+__global__ __launch_bounds__(1024) void lut_scan_kernel(const hb_lut_job *jobs)
 {
     __shared__ uint32_t part[1024];
+    const hb_lut_job J = jobs[blockIdx.y];
+    uint32_t *cnt = J.count_start;
     const int t = threadIdx.x;
     const int per = HB_NCELLS / 1024;
     uint32_t s = 0;
@@ -143,20 +145,20 @@ __global__ __launch_bounds__(1024) void lut_scan_kernel(uint32_t *cnt, uint32_t 
     part[t] = s;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
-        uint32_t v = t >= o ? part[t - o] : 0;
+        const uint32_t v = t >= o ? part[t - o] : 0;
         __syncthreads();
         part[t] += v;
         __syncthreads();
     }
     uint32_t run = part[t] - s;
     for (int i = 0; i < per; ++i) {
-        uint32_t v = cnt[t * per + i];
+        const uint32_t v = cnt[t * per + i];
         cnt[t * per + i] = run;
         run += v;
     }
     if (t == 1023) {
         cnt[HB_NCELLS] = run;
-        *total = run;
+        *J.total = run;
     }
 }
 
@@ -299,14 +301,16 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 }
 
 // ---------------------------------------------------------------- scan
-// Per-wavefront LDS of the scan:
+// Per-wavefront LDS of the scan (D = diagonal array length of the launch):
 //   state[D]   int32  diagonal state: bit 31 = extended-right flag,
 //                     bit 30 = extension pending, bits 0..29 = last hit + offset
 //   marker[D]  uint32 same-round ordering marker (generation << 6 | 63 - lane)
 //   seg[64]    uint32 tagged segment starts of the current hit round
-//   jobs[kJobCap] uint2 pending two-hit extensions (q | (s - last) << 24, s)
+//   jobs[kJobCap] uint2 queued two-hit extensions (q | (s - last) << 24, s)
 //   blk[kBlkCap]  uint2 hits waiting on a pending diagonal (q, s)
 //   sbuf[kSCap]   residues of the current subject (if it fits)
+// Per-workgroup LDS: score matrix, presence vector, context tables (stride
+// ctx_pad_max) and query of the group being scanned.
 constexpr int kJobCap = HB_JOBCAP;
 constexpr int kBlkCap = HB_BLKCAP;
 constexpr int kSCap = HB_SCAP;
@@ -321,9 +325,19 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m)
 
 struct WaveScan {
     const hb_scan_args *a;
+    // group being scanned
+    const uint2 *cells;
+    const uint32_t *entries;
+    uint32_t n_entries;
+    int32_t qlen;
+    int32_t n_ctx;
+    int32_t ctx_base;
+    int32_t D;
+    // LDS tables
     const int8_t *mat;
     const uint8_t *qry;
-    const int32_t *ctx;   // LDS: ctx_off[ctx_pad], xdrop[ctx_pad], cutoff[ctx_pad]
+    const int32_t *ctx;   // ctx_off[ctx_pad], xdrop[ctx_pad], cutoff[ctx_pad]
+    int32_t ctx_pad;
     const uint32_t *pv;
     int32_t *state;
     uint32_t *marker;
@@ -332,7 +346,6 @@ struct WaveScan {
     uint2 *blk;
     uint8_t *sbuf;
     int lane;
-    int32_t D;
     int32_t window;
     int32_t base;      // diagonal offset: stale entries read as "no recent hit"
     uint32_t gen;      // marker generation
@@ -341,11 +354,24 @@ struct WaveScan {
     uint32_t n_blk;    // wave-uniform
     uint32_t n_ext;    // per lane
     uint32_t n_hsp;    // per lane
+    uint64_t tot_hits, tot_ext, tot_hsp;  // wave-uniform running totals
 
     __device__ __forceinline__ uint32_t diag_of(uint32_t q, int32_t so) const
     {
         int32_t d = (int32_t)q - so % D;
         return (uint32_t)(d < 0 ? d + D : d);
+    }
+
+    // Starts a new group: clears the diagonal state and ordering markers.
+    __device__ __forceinline__ void reset()
+    {
+        for (int i = lane; i < D; i += HB_WAVE) {
+            state[i] = 0;
+            marker[i] = 0;
+        }
+        base = window;
+        gen = 0;
+        wave_lds_sync();
     }
 
     // Two-hit decisions for one set of hits (at most one per lane; active
@@ -414,12 +440,12 @@ struct WaveScan {
                 q = (int32_t)(jb.x & 0xffffffu);
                 so = (int32_t)jb.y;
                 const int32_t last_s = so - (int32_t)(jb.x >> 24);
-                k = context_of(ctx, a->n_ctx, q);
-                r = ext_two_hit(mat, S, slen, qry, a->qlen, last_s + 3, so, q, ctx[a->ctx_pad + k]);
+                k = context_of(ctx, n_ctx, q);
+                r = ext_two_hit(mat, S, slen, qry, qlen, last_s + 3, so, q, ctx[ctx_pad + k]);
                 state[diag_of((uint32_t)q, so)] =
                     r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + base)) : so + base;
             }
-            const bool emit = v && r.score >= ctx[2 * a->ctx_pad + k];
+            const bool emit = v && r.score >= ctx[2 * ctx_pad + k];
             const uint64_t m = __ballot(emit);
             if (m) {
                 uint32_t slot0 = 0;
@@ -430,7 +456,7 @@ struct WaveScan {
                     if (slot < a->out_cap) {
                         hb_hsp o;
                         o.oid = oid;
-                        o.context = k;
+                        o.context = ctx_base + k;
                         o.q_start = r.q_start;
                         o.s_start = r.s_start;
                         o.q_off = q;
@@ -478,7 +504,6 @@ struct WaveScan {
         uint32_t hits = 0;
         n_ext = 0;
         n_hsp = 0;
-        const uint32_t n_entries = a->n_entries;
         int32_t s0m = 0;  // s0 mod D
         for (int32_t s0 = 0; s0 + 3 <= slen; s0 += HB_WAVE) {
             const int32_t s = s0 + lane;
@@ -486,7 +511,7 @@ struct WaveScan {
             if (s + 3 <= slen) {
                 const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
                 if ((pv[idx >> 5] >> (idx & 31)) & 1u) {
-                    const uint2 c = a->cells[idx];
+                    const uint2 c = cells[idx];
                     cst = c.x;
                     cnt = c.y;
                 }
@@ -514,7 +539,7 @@ struct WaveScan {
                 const bool active = h < H;
                 const int32_t eb = __shfl(ebase, (int)p, HB_WAVE);
                 const uint32_t ei = min((uint32_t)((int32_t)h + eb), n_entries - 1);
-                const uint32_t q = active ? a->entries[ei] : 0u;
+                const uint32_t q = active ? entries[ei] : 0u;
                 int32_t smp = s0m + (int32_t)p;
                 if (smp >= D) smp -= D;
                 int32_t d = (int32_t)q - smp;
@@ -527,11 +552,14 @@ struct WaveScan {
         }
         drain(S, slen, oid);
         const uint32_t e = wave_sum(n_ext), hs = wave_sum(n_hsp);
-        if (lane == 0) {
+        if (a->stats && lane == 0) {
             a->stats[3 * (size_t)oid + 0] = hits;
             a->stats[3 * (size_t)oid + 1] = e;
             a->stats[3 * (size_t)oid + 2] = hs;
         }
+        tot_hits += hits;
+        tot_ext += e;
+        tot_hsp += hs;
         if (base + slen + window >= (1 << 29)) {
             for (int i = lane; i < D; i += HB_WAVE) state[i] = 0;
             wave_lds_sync();
@@ -540,74 +568,144 @@ struct WaveScan {
             base += slen + window;
         }
     }
+
+    __device__ __forceinline__ void scan_subject(uint32_t it)
+    {
+        const uint32_t oid = a->order[it];
+        const uint64_t sb = a->offs[oid];
+        const int32_t slen = (int32_t)(a->offs[oid + 1] - sb);
+        const uint8_t *S = a->res + sb;
+        if (slen <= kSCap) {
+            for (int i = lane; i < slen; i += HB_WAVE) sbuf[i] = S[i];
+            wave_lds_sync();
+            subject(sbuf, slen, oid);
+            wave_lds_sync();
+        } else {
+            subject(S, slen, oid);
+        }
+    }
+
+    __device__ __forceinline__ void flush_totals(uint32_t g)
+    {
+        if (a->group_totals && lane == 0 && (tot_hits | tot_ext | tot_hsp)) {
+            atomicAdd(&a->group_totals[3 * (size_t)g + 0], (unsigned long long)tot_hits);
+            atomicAdd(&a->group_totals[3 * (size_t)g + 1], (unsigned long long)tot_ext);
+            atomicAdd(&a->group_totals[3 * (size_t)g + 2], (unsigned long long)tot_hsp);
+        }
+        tot_hits = tot_ext = tot_hsp = 0;
+    }
 };
 
+// Copies group g's tables into the workgroup's LDS (all threads).
+__device__ __forceinline__ void load_group(const hb_scan_args &a, const hb_group_desc &G,
+                                           uint32_t *sm_pv, int32_t *sm_ctx, uint8_t *sm_q)
+{
+    for (int i = threadIdx.x; i < HB_PV_WORDS; i += blockDim.x) sm_pv[i] = G.pv[i];
+    for (int i = threadIdx.x; i < G.qlen_pad / 4; i += blockDim.x)
+        reinterpret_cast<uint32_t *>(sm_q)[i] = reinterpret_cast<const uint32_t *>(G.query)[i];
+    for (int i = threadIdx.x; i < G.n_ctx; i += blockDim.x) {
+        sm_ctx[i] = G.ctx[i];
+        sm_ctx[a.ctx_pad_max + i] = G.ctx[G.n_ctx + i];
+        sm_ctx[2 * a.ctx_pad_max + i] = G.ctx[2 * G.n_ctx + i];
+    }
+}
+
+__device__ __forceinline__ void bind_group(WaveScan &w, const hb_group_desc &G)
+{
+    w.cells = G.cells;
+    w.entries = G.entries;
+    w.n_entries = G.n_entries > 0 ? G.n_entries : 1u;
+    w.qlen = G.qlen;
+    w.n_ctx = G.n_ctx;
+    w.ctx_base = G.ctx_base;
+    w.D = G.diag_len;
+}
+
+// MULTI = false: one group; each wavefront pops subject tiles on its own.
+// MULTI = true: many groups; the workgroup pops (group, tile) items and its
+// wavefronts share the tile's subjects, so all of them use one group's LDS
+// tables at a time.
+template <bool MULTI>
 __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t s_item;
     const int lane = threadIdx.x & (HB_WAVE - 1);
     const int wid = threadIdx.x / HB_WAVE;
+    const int nwaves = blockDim.x / HB_WAVE;
     int8_t *sm_mat = reinterpret_cast<int8_t *>(lds);
     uint32_t *sm_pv = reinterpret_cast<uint32_t *>(lds + 1024);
     int32_t *sm_ctx = reinterpret_cast<int32_t *>(lds + 1024 + 4 * HB_PV_WORDS);
-    uint8_t *sm_q = reinterpret_cast<uint8_t *>(sm_ctx + 3 * a.ctx_pad);
-    unsigned char *wb = sm_q + a.qlen_pad + (size_t)wid * a.wave_lds;
-    const int32_t D = a.diag_len;
+    uint8_t *sm_q = reinterpret_cast<uint8_t *>(sm_ctx + 3 * a.ctx_pad_max);
+    unsigned char *wb = sm_q + a.qlen_pad_max + (size_t)wid * a.wave_lds;
+    const int32_t Dm = a.diag_len_max;
 
     for (int i = threadIdx.x; i < 256; i += blockDim.x)
         reinterpret_cast<uint32_t *>(sm_mat)[i] = reinterpret_cast<const uint32_t *>(a.mat32)[i];
-    for (int i = threadIdx.x; i < HB_PV_WORDS; i += blockDim.x) sm_pv[i] = a.pv[i];
-    for (int i = threadIdx.x; i < a.qlen_pad / 4; i += blockDim.x)
-        reinterpret_cast<uint32_t *>(sm_q)[i] = reinterpret_cast<const uint32_t *>(a.query)[i];
-    for (int i = threadIdx.x; i < a.n_ctx; i += blockDim.x) {
-        sm_ctx[i] = a.ctx_off[i];
-        sm_ctx[a.ctx_pad + i] = a.ctx_xdrop[i];
-        sm_ctx[2 * a.ctx_pad + i] = a.ctx_cutoff[i];
-    }
 
     WaveScan w;
     w.a = &a;
     w.mat = sm_mat;
     w.qry = sm_q;
-    w.ctx = sm_ctx;
     w.pv = sm_pv;
+    w.ctx = sm_ctx;
+    w.ctx_pad = a.ctx_pad_max;
     w.state = reinterpret_cast<int32_t *>(wb);
-    w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)D);
-    w.seg = reinterpret_cast<uint32_t *>(wb + 8 * (size_t)D);
-    w.jobs = reinterpret_cast<uint2 *>(wb + 8 * (size_t)D + 256);
+    w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)Dm);
+    w.seg = reinterpret_cast<uint32_t *>(wb + 8 * (size_t)Dm);
+    w.jobs = reinterpret_cast<uint2 *>(wb + 8 * (size_t)Dm + 256);
     w.blk = w.jobs + kJobCap;
     w.sbuf = reinterpret_cast<uint8_t *>(w.blk + kBlkCap);
     w.lane = lane;
-    w.D = D;
     w.window = a.window;
-    w.base = a.window;
-    w.gen = 0;
     w.rgen = 0;
     w.n_jobs = 0;
     w.n_blk = 0;
-    for (int i = lane; i < 2 * D + 64; i += HB_WAVE) w.state[i] = 0;   // state, marker, seg
-    __syncthreads();
+    w.tot_hits = w.tot_ext = w.tot_hsp = 0;
+    w.seg[lane] = 0;
+    const uint32_t n_tiles = a.tile_end - a.tile_begin;
 
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(a.tile_counter, 1u);
-        t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
-        if (t >= a.n_tiles) break;
-        const uint32_t i_end = a.tiles[t + 1];
-        for (uint32_t it = a.tiles[t]; it < i_end; ++it) {
-            const uint32_t oid = a.order[it];
-            const uint64_t sb = a.offs[oid];
-            const int32_t slen = (int32_t)(a.offs[oid + 1] - sb);
-            const uint8_t *S = a.res + sb;
-            if (slen <= kSCap) {
-                for (int i = lane; i < slen; i += HB_WAVE) w.sbuf[i] = S[i];
-                wave_lds_sync();
-                w.subject(w.sbuf, slen, oid);
-                wave_lds_sync();
-            } else {
-                w.subject(S, slen, oid);
-            }
+    if (!MULTI) {
+        const hb_group_desc G = a.groups[0];
+        load_group(a, G, sm_pv, sm_ctx, sm_q);
+        bind_group(w, G);
+        w.reset();
+        __syncthreads();
+        for (;;) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(a.work_counter, 1u);
+            t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+            if (t >= n_tiles) break;
+            t += a.tile_begin;
+            const uint32_t i_end = a.tiles[t + 1];
+            for (uint32_t it = a.tiles[t]; it < i_end; ++it) w.scan_subject(it);
         }
+        w.flush_totals(0);
+    } else {
+        uint32_t cur = 0xffffffffu;
+        const uint32_t n_items = n_tiles * a.n_groups;
+        for (;;) {
+            if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1u);
+            __syncthreads();
+            const uint32_t item = s_item;
+            if (item >= n_items) break;
+            const uint32_t g = item / n_tiles;
+            const uint32_t t = a.tile_begin + item % n_tiles;
+            if (g != cur) {
+                if (cur != 0xffffffffu) w.flush_totals(cur);
+                const hb_group_desc G = a.groups[g];
+                load_group(a, G, sm_pv, sm_ctx, sm_q);
+                bind_group(w, G);
+                w.reset();
+                cur = g;
+                __syncthreads();
+            }
+            const uint32_t i_end = a.tiles[t + 1];
+            for (uint32_t it = a.tiles[t] + (uint32_t)wid; it < i_end; it += (uint32_t)nwaves)
+                w.scan_subject(it);
+            __syncthreads();
+        }
+        if (cur != 0xffffffffu) w.flush_totals(cur);
     }
 }
 
@@ -618,6 +716,12 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4 *dst, const uint4 *src,
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
         dst[i] = src[i];
+}
+
+__global__ void fill_u32_kernel(uint32_t *dst, uint32_t value, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = value;
 }
 
 }  // namespace
@@ -633,6 +737,14 @@ hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_
     return hipGetLastError();
 }
 
+hipError_t hb_launch_fill_u32(uint32_t *dst, uint32_t value, uint32_t n, hipStream_t st)
+{
+    const uint32_t blocks = n ? (n + 255) / 256 : 1;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks < 1024 ? blocks : 1024), dim3(256), 0, st, dst,
+                       value, n);
+    return hipGetLastError();
+}
+
 hipError_t hb_launch_check_residues(const uint8_t *res, uint64_t n, uint32_t *flag,
                                     hipStream_t st)
 {
@@ -645,49 +757,56 @@ hipError_t hb_launch_check_residues(const uint8_t *res, uint64_t n, uint32_t *fl
     return hipGetLastError();
 }
 
-hipError_t hb_launch_lut_count(const uint8_t *qflag, int32_t qlen, const int32_t *,
-                               const int8_t *mat32, int32_t threshold, uint32_t *count,
-                               hipStream_t st)
+hipError_t hb_launch_lut(const hb_lut_job *jobs_dev, int n_jobs, int32_t qlen_max,
+                         const int8_t *mat32, int32_t threshold, int pass, hipStream_t st)
 {
-    const size_t lds = 1024 + (size_t)((qlen + 15) & ~15);
-    hipLaunchKernelGGL(lut_kernel<false>, dim3(HB_NCELLS / 256), dim3(256), lds, st, qflag, qlen,
-                       mat32, threshold, count, nullptr, nullptr, nullptr, nullptr);
-    return hipGetLastError();
-}
-
-hipError_t hb_launch_lut_scan(uint32_t *count_to_start, uint32_t *total, hipStream_t st)
-{
-    hipLaunchKernelGGL(lut_scan_kernel, dim3(1), dim3(1024), 0, st, count_to_start, total);
-    return hipGetLastError();
-}
-
-hipError_t hb_launch_lut_fill(const uint8_t *qflag, int32_t qlen, const int32_t *,
-                              const int8_t *mat32, int32_t threshold, const uint32_t *start,
-                              uint32_t *entries, uint2 *cells, uint32_t *pv, hipStream_t st)
-{
-    const size_t lds = 1024 + (size_t)((qlen + 15) & ~15);
-    hipLaunchKernelGGL(lut_kernel<true>, dim3(HB_NCELLS / 256), dim3(256), lds, st, qflag, qlen,
-                       mat32, threshold, nullptr, start, entries, cells, pv);
-    return hipGetLastError();
-}
-
-hipError_t hb_launch_scan(const hb_scan_args &a, int n_blocks, int block_threads, size_t lds_bytes,
-                          hipStream_t st)
-{
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(hb_scan_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (n_jobs <= 0) return hipSuccess;
+    const size_t lds = 1024 + (size_t)((qlen_max + 15) & ~15);
+    if (pass == 1) {
+        hipLaunchKernelGGL(lut_scan_kernel, dim3(1, n_jobs), dim3(1024), 0, st, jobs_dev);
+        return hipGetLastError();
+    }
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(pass == 0 ? lut_kernel<0>
+                                                                                : lut_kernel<2>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
+    if (pass == 0)
+        hipLaunchKernelGGL(lut_kernel<0>, dim3(HB_NCELLS / 256, n_jobs), dim3(256), lds, st,
+                           jobs_dev, mat32, threshold);
+    else
+        hipLaunchKernelGGL(lut_kernel<2>, dim3(HB_NCELLS / 256, n_jobs), dim3(256), lds, st,
+                           jobs_dev, mat32, threshold);
+    return hipGetLastError();
+}
+
+static const void *scan_fn(bool multi)
+{
+    return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true>)
+                 : reinterpret_cast<const void *>(hb_scan_kernel<false>);
+}
+
+hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int block_threads,
+                          size_t lds_bytes, hipStream_t st)
+{
     if (block_threads < HB_WAVE || block_threads > HB_BLOCK || block_threads % HB_WAVE)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(hb_scan_kernel, dim3(n_blocks), dim3(block_threads), lds_bytes, st, a);
+    hipError_t e = hipFuncSetAttribute(scan_fn(multi), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    if (multi)
+        hipLaunchKernelGGL(hb_scan_kernel<true>, dim3(n_blocks), dim3(block_threads), lds_bytes, st,
+                           a);
+    else
+        hipLaunchKernelGGL(hb_scan_kernel<false>, dim3(n_blocks), dim3(block_threads), lds_bytes,
+                           st, a);
     return hipGetLastError();
 }
 
-hipError_t hb_scan_occupancy(size_t lds_bytes, int block_threads, int *blocks_per_cu)
+hipError_t hb_scan_occupancy(bool multi, size_t lds_bytes, int block_threads, int *blocks_per_cu)
 {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(hb_scan_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    hipError_t e = hipFuncSetAttribute(scan_fn(multi), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hb_scan_kernel,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_fn(multi),
                                                         block_threads, lds_bytes);
 }

@@ -1,0 +1,93 @@
+// hb_runtime.h -- host-side objects shared by the plain (hb_seed.cpp) and
+// batched/served (hb_batch.cpp, hb_serve.cpp) entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "hb_internal.h"
+
+void hb_note_hip_error(hipError_t e, const char *what, int line);
+
+#define HB_CHECK(call)                                   \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) {                          \
+            hb_note_hip_error(e_, #call, __LINE__);      \
+            return HB_ERR_DEVICE;                        \
+        }                                                \
+    } while (0)
+
+extern const int8_t kHbBlosum62[HB_ALPHA][HB_ALPHA];
+
+struct hb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    size_t max_lds = 0;
+    std::mutex mu;
+    // Device buffers of finished scans, reused by later scans so that a
+    // per-query loop does not pay hipMalloc/hipFree per call.
+    std::mutex pool_mu;
+    std::vector<std::pair<void *, size_t>> pool;
+};
+
+void *hb_pool_get(hb_ctx *c, size_t bytes);
+void hb_pool_put(hb_ctx *c, void *p, size_t bytes);
+
+struct hb_db {
+    hb_ctx *ctx = nullptr;
+    uint32_t n = 0;
+    uint64_t total = 0;
+    uint8_t *d_res = nullptr;
+    uint64_t *d_offs = nullptr;
+    uint32_t *d_order = nullptr;
+    uint32_t *d_tiles = nullptr;
+    uint32_t n_tiles = 0;
+    std::vector<uint64_t> h_offs;
+};
+
+// Search parameters resolved from hb_params.
+struct hb_resolved_params {
+    int32_t threshold = 11;
+    int32_t window = 40;
+    int8_t mat32[32 * 32];
+};
+int hb_resolve_params(const hb_params *p, hb_resolved_params *out);
+
+// A concatenated query as built on the host: residues and per-context
+// (offset, length, X-drop, cutoff).
+struct hb_concat {
+    std::vector<uint8_t> seq;
+    std::vector<int32_t> ctx_off, ctx_len, xdrop, cutoff;
+};
+// Validates contexts and residues of one query.
+int hb_check_query(const uint8_t *seq, int32_t len, int32_t n_ctx, const int32_t *ctx_off,
+                   const int32_t *ctx_len, const int32_t *xdrop);
+
+// Device lookup tables and group descriptors for a set of concatenated
+// queries, all in two allocations (fixed-size tables + entries).
+struct hb_group_set {
+    std::vector<hb_group_desc> desc;   // host copy
+    hb_group_desc *d_desc = nullptr;   // device copy
+    void *d_fixed = nullptr;           // cells, starts, pv, query, ctx, qflag per group
+    uint32_t *d_entries = nullptr;
+    int8_t *d_mat = nullptr;
+    size_t fixed_bytes = 0, entries_bytes = 0;
+    int32_t qlen_pad_max = 0, ctx_pad_max = 4, diag_len_max = 64;
+    std::vector<uint32_t> n_entries, n_cells_used;
+};
+int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &queries, const std::vector<int32_t> &ctx_base,
+                       const hb_resolved_params &p, hb_group_set *out);
+void hb_group_set_free(hb_group_set *g);
+
+// Launch configuration of the scan for a given LDS layout.
+struct hb_launch_cfg {
+    int waves = 4;
+    size_t lds = 0;
+    int blocks = 0;
+};
+int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
+                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg);

@@ -1,4 +1,5 @@
-// hb_seed.cpp -- host runtime behind include/hb_seed.h.
+// hb_seed.cpp -- host runtime behind include/hb_seed.h (plain per-query path)
+// and the pieces the batched path shares (hb_runtime.h).
 //
 // Replaces the host half of H-BLAST (h_blast_v1.1/H-BLAST/h_blast.cu):
 // device setup, the resident database (instead of an mmap + H2D copy per
@@ -16,45 +17,24 @@
 #include <numeric>
 #include <vector>
 
-#include "hb_internal.h"
-
-#define HB_CHECK(call)                                   \
-    do {                                                 \
-        hipError_t e_ = (call);                          \
-        if (e_ != hipSuccess) {                          \
-            hb_note_hip_error(e_, #call, __LINE__);      \
-            return HB_ERR_DEVICE;                        \
-        }                                                \
-    } while (0)
+#include "hb_runtime.h"
 
 static thread_local char g_last_hip[256];
 
-static void hb_note_hip_error(hipError_t e, const char *what, int line)
+void hb_note_hip_error(hipError_t e, const char *what, int line)
 {
-    snprintf(g_last_hip, sizeof(g_last_hip), "%s (%s) at hb_seed.cpp:%d", hipGetErrorString(e),
-             what, line);
+    snprintf(g_last_hip, sizeof(g_last_hip), "%s (%s) at line %d", hipGetErrorString(e), what,
+             line);
 }
 
 // BLOSUM62 in ncbistdaa order -,A,B,C,D,E,F,G,H,I,K,L,M,N,P,Q,R,S,T,V,W,X,Y,Z,U,*,O,J.
 // Same values as the reference's table (h_blast_v1.1/H-BLAST/h_blast.c:42-71);
 // the gap letter scores -128 so that context sentinels stop every extension.
-static const int8_t kBlosum62[HB_ALPHA][HB_ALPHA] = {
+const int8_t kHbBlosum62[HB_ALPHA][HB_ALPHA] = {
 #include "hb_blosum62.inc"
 };
 
-struct hb_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int n_cu = 0;
-    size_t max_lds = 0;
-    std::mutex mu;
-    // Device buffers of finished scans, reused by later scans so that a
-    // per-query loop does not pay hipMalloc/hipFree per call.
-    std::mutex pool_mu;
-    std::vector<std::pair<void *, size_t>> pool;
-};
-
-static void *pool_get(hb_ctx *c, size_t bytes)
+void *hb_pool_get(hb_ctx *c, size_t bytes)
 {
     {
         std::lock_guard<std::mutex> lk(c->pool_mu);
@@ -74,7 +54,7 @@ static void *pool_get(hb_ctx *c, size_t bytes)
     return p;
 }
 
-static void pool_put(hb_ctx *c, void *p, size_t bytes)
+void hb_pool_put(hb_ctx *c, void *p, size_t bytes)
 {
     if (!p) return;
     std::lock_guard<std::mutex> lk(c->pool_mu);
@@ -85,29 +65,225 @@ static void pool_put(hb_ctx *c, void *p, size_t bytes)
     }
 }
 
-struct hb_db {
-    hb_ctx *ctx = nullptr;
-    uint32_t n = 0;
-    uint64_t total = 0;
-    uint8_t *d_res = nullptr;
-    uint64_t *d_offs = nullptr;
-    uint32_t *d_order = nullptr;
-    uint32_t *d_tiles = nullptr;
-    uint32_t n_tiles = 0;
-    std::vector<uint64_t> h_offs;
-};
+int hb_resolve_params(const hb_params *pp, hb_resolved_params *out)
+{
+    hb_params p;
+    hb_params_default(&p);
+    if (pp) p = *pp;
+    if (p.word_size != 3) return HB_ERR_UNSUPPORTED;
+    // queued extensions pack the query offset in 24 bits and s - last_hit
+    // (< window) in 8 bits
+    if (p.window <= 3) return HB_ERR_ARG;
+    if (p.window > 255) return HB_ERR_UNSUPPORTED;
+    out->threshold = p.threshold;
+    out->window = p.window;
+    for (int i = 0; i < 32; ++i)
+        for (int j = 0; j < 32; ++j)
+            out->mat32[i * 32 + j] =
+                (i < HB_ALPHA && j < HB_ALPHA)
+                    ? (p.matrix ? p.matrix[i * HB_ALPHA + j] : kHbBlosum62[i][j])
+                    : (int8_t)-128;
+    return HB_OK;
+}
+
+int hb_check_query(const uint8_t *seq, int32_t len, int32_t n_ctx, const int32_t *ctx_off,
+                   const int32_t *ctx_len, const int32_t *xdrop)
+{
+    if (!seq || len <= 0 || n_ctx <= 0 || !ctx_off || !ctx_len || !xdrop) return HB_ERR_ARG;
+    if (len >= (1 << 24)) return HB_ERR_UNSUPPORTED;
+    for (int32_t i = 0; i < len; ++i)
+        if (seq[i] >= HB_ALPHA) return HB_ERR_RESIDUE;
+    int32_t prev_end = 0;
+    for (int32_t k = 0; k < n_ctx; ++k) {
+        if (ctx_len[k] < 0 || ctx_off[k] < prev_end || ctx_off[k] + ctx_len[k] > len ||
+            xdrop[k] <= 0)
+            return HB_ERR_ARG;
+        prev_end = ctx_off[k] + ctx_len[k];
+    }
+    return HB_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+void hb_group_set_free(hb_group_set *g)
+{
+    if (!g) return;
+    hipFree(g->d_desc);
+    hipFree(g->d_fixed);
+    hipFree(g->d_entries);
+    hipFree(g->d_mat);
+    g->d_desc = nullptr;
+    g->d_fixed = nullptr;
+    g->d_entries = nullptr;
+    g->d_mat = nullptr;
+}
+
+// Lookup tables of all groups are built together: count pass, per-group
+// exclusive scan, one read-back of the totals, entries allocation, fill pass.
+int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
+                       const std::vector<int32_t> &ctx_base, const hb_resolved_params &p,
+                       hb_group_set *out)
+{
+    const size_t n = qs.size();
+    if (n == 0 || ctx_base.size() != n) return HB_ERR_ARG;
+    out->desc.assign(n, hb_group_desc{});
+    // per-group layout inside d_fixed
+    const size_t cells_b = HB_NCELLS * sizeof(uint2);
+    const size_t start_b = align256((HB_NCELLS + 1) * sizeof(uint32_t));
+    const size_t pv_b = HB_PV_WORDS * sizeof(uint32_t);
+    std::vector<size_t> off(n + 1, 0);
+    int32_t qlen_max = 0;
+    for (size_t g = 0; g < n; ++g) {
+        const int32_t qlen = (int32_t)qs[g].seq.size();
+        const int32_t qpad = (qlen + 15) & ~15;
+        const int32_t nctx = (int32_t)qs[g].ctx_off.size();
+        const size_t bytes = cells_b + start_b + pv_b + align256(qpad) +
+                             align256(3 * (size_t)nctx * sizeof(int32_t)) + align256(qlen) + 256;
+        off[g + 1] = off[g] + bytes;
+        qlen_max = std::max(qlen_max, qlen);
+        out->qlen_pad_max = std::max(out->qlen_pad_max, qpad);
+        out->ctx_pad_max = std::max(out->ctx_pad_max, hb_ctx_pad(nctx));
+        out->diag_len_max = std::max(out->diag_len_max, (qlen + p.window + 63) & ~63);
+    }
+    out->fixed_bytes = off[n];
+    // host image of the inputs (query, ctx, qflag) and the job table
+    std::vector<uint8_t> img(out->fixed_bytes, 0);
+    std::vector<hb_lut_job> jobs(n);
+    if (hipMalloc(&out->d_fixed, out->fixed_bytes) != hipSuccess ||
+        hipMalloc(&out->d_mat, 1024) != hipSuccess ||
+        hipMalloc(&out->d_desc, n * sizeof(hb_group_desc)) != hipSuccess) {
+        hb_group_set_free(out);
+        return HB_ERR_NOMEM;
+    }
+    uint8_t *base = static_cast<uint8_t *>(out->d_fixed);
+    for (size_t g = 0; g < n; ++g) {
+        const hb_concat &Q = qs[g];
+        const int32_t qlen = (int32_t)Q.seq.size();
+        const int32_t qpad = (qlen + 15) & ~15;
+        const int32_t nctx = (int32_t)Q.ctx_off.size();
+        size_t o = off[g];
+        uint2 *cells = reinterpret_cast<uint2 *>(base + o);
+        o += cells_b;
+        uint32_t *start = reinterpret_cast<uint32_t *>(base + o);
+        o += start_b;
+        uint32_t *pv = reinterpret_cast<uint32_t *>(base + o);
+        o += pv_b;
+        const size_t q_o = o;
+        memcpy(&img[q_o], Q.seq.data(), (size_t)qlen);
+        o += align256(qpad);
+        const size_t c_o = o;
+        int32_t *cimg = reinterpret_cast<int32_t *>(&img[c_o]);
+        for (int32_t k = 0; k < nctx; ++k) {
+            cimg[k] = Q.ctx_off[k];
+            cimg[nctx + k] = Q.xdrop[k];
+            cimg[2 * nctx + k] = Q.cutoff[k];
+        }
+        o += align256(3 * (size_t)nctx * sizeof(int32_t));
+        const size_t f_o = o;
+        uint8_t *qf = &img[f_o];
+        for (int32_t i = 0; i < qlen; ++i) qf[i] = Q.seq[i];
+        for (int32_t k = 0; k < nctx; ++k)
+            for (int32_t w = Q.ctx_off[k]; w + 3 <= Q.ctx_off[k] + Q.ctx_len[k]; ++w) qf[w] |= 0x80;
+        o += align256(qlen);
+        uint32_t *total = reinterpret_cast<uint32_t *>(base + o);
+        hb_group_desc &D = out->desc[g];
+        D.cells = cells;
+        D.pv = pv;
+        D.query = base + q_o;
+        D.ctx = reinterpret_cast<const int32_t *>(base + c_o);
+        D.qlen = qlen;
+        D.qlen_pad = qpad;
+        D.n_ctx = nctx;
+        D.diag_len = (qlen + p.window + 63) & ~63;
+        D.ctx_base = ctx_base[g];
+        jobs[g].qflag = base + f_o;
+        jobs[g].qlen = qlen;
+        jobs[g].count_start = start;
+        jobs[g].cells = cells;
+        jobs[g].pv = pv;
+        jobs[g].total = total;
+    }
+    hipStream_t st = c->stream;
+    hb_lut_job *d_jobs = nullptr;
+    auto fail = [&](int rc) {
+        hipFree(d_jobs);
+        hb_group_set_free(out);
+        return rc;
+    };
+    if (hipMalloc(&d_jobs, n * sizeof(hb_lut_job)) != hipSuccess) return fail(HB_ERR_NOMEM);
+    // inputs: copy each group's query/ctx/qflag span (the tables are written on the device)
+    for (size_t g = 0; g < n; ++g) {
+        // query, context table and qflag are contiguous in the group's span
+        const size_t q_o = (size_t)(out->desc[g].query - base);
+        const size_t end = (size_t)(jobs[g].qflag - base) + align256(jobs[g].qlen);
+        if (hipMemcpyAsync(base + q_o, &img[q_o], end - q_o, hipMemcpyHostToDevice, st) !=
+            hipSuccess)
+            return fail(HB_ERR_DEVICE);
+    }
+    if (hipMemcpyAsync(out->d_mat, p.mat32, 1024, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_jobs, jobs.data(), n * sizeof(hb_lut_job), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hb_launch_lut(d_jobs, (int)n, qlen_max, out->d_mat, p.threshold, 0, st) != hipSuccess ||
+        hb_launch_lut(d_jobs, (int)n, qlen_max, out->d_mat, p.threshold, 1, st) != hipSuccess)
+        return fail(HB_ERR_DEVICE);
+    std::vector<uint32_t> totals(n);
+    for (size_t g = 0; g < n; ++g)
+        if (hipMemcpyAsync(&totals[g], jobs[g].total, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           st) != hipSuccess)
+            return fail(HB_ERR_DEVICE);
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(HB_ERR_DEVICE);
+    size_t ent_total = 0;
+    for (uint32_t t : totals) ent_total += t;
+    out->entries_bytes = std::max<size_t>(ent_total, 1) * sizeof(uint32_t);
+    if (hipMalloc(&out->d_entries, out->entries_bytes) != hipSuccess) return fail(HB_ERR_NOMEM);
+    size_t e = 0;
+    out->n_entries.resize(n);
+    for (size_t g = 0; g < n; ++g) {
+        jobs[g].entries = out->d_entries + e;
+        out->desc[g].entries = out->d_entries + e;
+        out->desc[g].n_entries = totals[g];
+        out->n_entries[g] = totals[g];
+        e += totals[g];
+    }
+    if (hipMemcpyAsync(d_jobs, jobs.data(), n * sizeof(hb_lut_job), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hb_launch_lut(d_jobs, (int)n, qlen_max, out->d_mat, p.threshold, 2, st) != hipSuccess ||
+        hipMemcpyAsync(out->d_desc, out->desc.data(), n * sizeof(hb_group_desc),
+                       hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(HB_ERR_DEVICE);
+    hipFree(d_jobs);
+    return HB_OK;
+}
+
+int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
+                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg)
+{
+    const size_t wave_lds = hb_scan_wave_lds(diag_len_max);
+    const size_t fixed = hb_scan_fixed_lds(qlen_pad_max, ctx_pad_max);
+    // 4 wavefronts per workgroup share the query tables; fewer when the
+    // diagonal arrays of long (e.g. six-frame) queries do not fit.
+    int waves = 4;
+    while (waves > 1 && fixed + waves * wave_lds > c->max_lds) waves /= 2;
+    const size_t lds = fixed + waves * wave_lds;
+    if (lds > c->max_lds) return HB_ERR_UNSUPPORTED;
+    int per_cu = 0;
+    HB_CHECK(hb_scan_occupancy(multi, lds, waves * HB_WAVE, &per_cu));
+    if (per_cu < 1) per_cu = 1;
+    int blocks = c->n_cu * per_cu;
+    const uint32_t per_block = multi ? 1u : (uint32_t)waves;
+    const int need = (int)((work_items + per_block - 1) / per_block);
+    if (need < blocks) blocks = std::max(need, 1);
+    cfg->waves = waves;
+    cfg->lds = lds;
+    cfg->blocks = blocks;
+    return HB_OK;
+}
 
 struct hb_query {
     hb_ctx *ctx = nullptr;
-    int32_t qlen = 0, qlen_pad = 0, n_ctx = 0, window = 0, threshold = 0, diag_len = 0;
-    uint8_t *d_query = nullptr;
-    int32_t *d_ctx = nullptr;   // [off | xdrop | cutoff] x n_ctx
-    int8_t *d_mat = nullptr;    // 32x32
-    uint2 *d_cells = nullptr;
-    uint32_t *d_start = nullptr;
-    uint32_t *d_entries = nullptr;
-    uint32_t *d_pv = nullptr;
-    uint32_t n_entries = 0;
+    hb_resolved_params p;
+    hb_group_set gs;
     uint32_t n_cells_used = 0;
 };
 
@@ -142,7 +318,7 @@ const char *hb_strerror(int err)
     }
 }
 
-const int8_t *hb_blosum62(void) { return &kBlosum62[0][0]; }
+const int8_t *hb_blosum62(void) { return &kHbBlosum62[0][0]; }
 
 void hb_params_default(hb_params *p)
 {
@@ -312,143 +488,55 @@ int hb_query_free(hb_query *q)
 {
     if (!q) return HB_ERR_ARG;
     hipSetDevice(q->ctx->device);
-    hipFree(q->d_query);
-    hipFree(q->d_ctx);
-    hipFree(q->d_mat);
-    hipFree(q->d_cells);
-    hipFree(q->d_start);
-    hipFree(q->d_entries);
-    hipFree(q->d_pv);
+    hb_group_set_free(&q->gs);
     delete q;
     return HB_OK;
-}
-
-// per wave: diagonal state + marker, 64 segment starts, job and blocked
-// queues (8 B entries), staged subject
-static size_t scan_wave_lds(int32_t diag_len)
-{
-    return 8 * (size_t)diag_len + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + HB_SCAP;
-}
-// per workgroup: score matrix, presence vector, context tables, query
-static int32_t ctx_pad_of(int32_t n_ctx) { return (n_ctx + 3) & ~3; }
-static size_t scan_fixed_lds(int32_t qlen_pad, int32_t n_ctx)
-{
-    return 1024 + 4 * HB_PV_WORDS + 12 * (size_t)ctx_pad_of(n_ctx) + qlen_pad;
 }
 
 int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
                      const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
                      const int32_t *cutoff, const hb_params *params, hb_query **out)
 {
-    if (!c || !seq || len <= 0 || n_ctx <= 0 || !ctx_off || !ctx_len || !xdrop || !cutoff || !out)
-        return HB_ERR_ARG;
+    if (!c || !out || !cutoff) return HB_ERR_ARG;
     *out = nullptr;
-    hb_params p;
-    hb_params_default(&p);
-    if (params) p = *params;
-    if (p.word_size != 3) return HB_ERR_UNSUPPORTED;
-    // queued extensions pack the query offset in 24 bits and s - last_hit
-    // (< window) in 8 bits
-    if (p.window <= 3) return HB_ERR_ARG;
-    if (p.window > 255 || len >= (1 << 24)) return HB_ERR_UNSUPPORTED;
-    for (int32_t i = 0; i < len; ++i)
-        if (seq[i] >= HB_ALPHA) return HB_ERR_RESIDUE;
-    std::vector<uint8_t> qflag((size_t)len);
-    for (int32_t i = 0; i < len; ++i) qflag[i] = seq[i];
-    int32_t prev_end = 0;
-    for (int32_t k = 0; k < n_ctx; ++k) {
-        if (ctx_len[k] < 0 || ctx_off[k] < prev_end || ctx_off[k] + ctx_len[k] > len ||
-            xdrop[k] <= 0)
-            return HB_ERR_ARG;
-        prev_end = ctx_off[k] + ctx_len[k];
-        for (int32_t o = ctx_off[k]; o + 3 <= ctx_off[k] + ctx_len[k]; ++o) qflag[o] |= 0x80;
-    }
+    int rc = hb_check_query(seq, len, n_ctx, ctx_off, ctx_len, xdrop);
+    if (rc) return rc;
     hb_query *q = new (std::nothrow) hb_query;
     if (!q) return HB_ERR_NOMEM;
     q->ctx = c;
-    q->qlen = len;
-    q->qlen_pad = (len + 15) & ~15;
-    q->n_ctx = n_ctx;
-    q->window = p.window;
-    q->threshold = p.threshold;
-    // Diagonals are (q - s) mod diag_len.  Any diag_len >= qlen + window
-    // gives the same two-hit decisions as the CPU word finder's power-of-two
-    // table (two hits that alias are always more than a window apart).
-    q->diag_len = (len + p.window + 63) & ~63;
-    if (scan_fixed_lds(q->qlen_pad, n_ctx) + scan_wave_lds(q->diag_len) > c->max_lds) {
+    rc = hb_resolve_params(params, &q->p);
+    if (rc) {
+        delete q;
+        return rc;
+    }
+    std::vector<hb_concat> qs(1);
+    qs[0].seq.assign(seq, seq + len);
+    qs[0].ctx_off.assign(ctx_off, ctx_off + n_ctx);
+    qs[0].ctx_len.assign(ctx_len, ctx_len + n_ctx);
+    qs[0].xdrop.assign(xdrop, xdrop + n_ctx);
+    qs[0].cutoff.assign(cutoff, cutoff + n_ctx);
+    const int32_t D = (len + q->p.window + 63) & ~63;
+    if (hb_scan_fixed_lds((len + 15) & ~15, hb_ctx_pad(n_ctx)) + hb_scan_wave_lds(D) > c->max_lds) {
         delete q;
         return HB_ERR_UNSUPPORTED;
     }
-    int8_t mat32[32 * 32];
-    for (int i = 0; i < 32; ++i)
-        for (int j = 0; j < 32; ++j)
-            mat32[i * 32 + j] = (i < HB_ALPHA && j < HB_ALPHA)
-                                    ? (p.matrix ? p.matrix[i * HB_ALPHA + j] : kBlosum62[i][j])
-                                    : (int8_t)-128;
-    std::vector<int32_t> ctxv(3 * (size_t)n_ctx);
-    for (int32_t k = 0; k < n_ctx; ++k) {
-        ctxv[k] = ctx_off[k];
-        ctxv[n_ctx + k] = xdrop[k];
-        ctxv[2 * n_ctx + k] = cutoff[k];
-    }
-    std::vector<uint8_t> qpad((size_t)q->qlen_pad, 0);
-    memcpy(qpad.data(), seq, (size_t)len);
-
     std::lock_guard<std::mutex> lk(c->mu);
-    auto fail = [&](int r) {
+    if (hipSetDevice(c->device) != hipSuccess) {
+        delete q;
+        return HB_ERR_DEVICE;
+    }
+    rc = hb_group_set_build(c, qs, std::vector<int32_t>(1, 0), q->p, &q->gs);
+    if (rc) {
+        delete q;
+        return rc;
+    }
+    std::vector<uint32_t> pv(HB_PV_WORDS);
+    if (hipMemcpy(pv.data(), q->gs.desc[0].pv, HB_PV_WORDS * 4, hipMemcpyDeviceToHost) !=
+        hipSuccess) {
         hb_query_free(q);
-        return r;
-    };
-    if (hipSetDevice(c->device) != hipSuccess) return fail(HB_ERR_DEVICE);
-    uint8_t *d_qflag = nullptr;
-    uint32_t *d_total = nullptr;
-    if (hipMalloc(&q->d_query, q->qlen_pad) != hipSuccess ||
-        hipMalloc(&q->d_ctx, ctxv.size() * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&q->d_mat, sizeof(mat32)) != hipSuccess ||
-        hipMalloc(&q->d_cells, HB_NCELLS * sizeof(uint2)) != hipSuccess ||
-        hipMalloc(&q->d_start, (HB_NCELLS + 1) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&q->d_pv, HB_PV_WORDS * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&d_qflag, (size_t)len) != hipSuccess ||
-        hipMalloc(&d_total, sizeof(uint32_t)) != hipSuccess) {
-        hipFree(d_qflag);
-        hipFree(d_total);
-        return fail(HB_ERR_NOMEM);
+        return HB_ERR_DEVICE;
     }
-    hipStream_t st = c->stream;
-    bool ok = hipMemcpyAsync(q->d_query, qpad.data(), q->qlen_pad, hipMemcpyHostToDevice, st) ==
-                  hipSuccess &&
-              hipMemcpyAsync(q->d_ctx, ctxv.data(), ctxv.size() * sizeof(int32_t),
-                             hipMemcpyHostToDevice, st) == hipSuccess &&
-              hipMemcpyAsync(q->d_mat, mat32, sizeof(mat32), hipMemcpyHostToDevice, st) ==
-                  hipSuccess &&
-              hipMemcpyAsync(d_qflag, qflag.data(), (size_t)len, hipMemcpyHostToDevice, st) ==
-                  hipSuccess &&
-              hb_launch_lut_count(d_qflag, len, nullptr, q->d_mat, p.threshold, q->d_start, st) ==
-                  hipSuccess &&
-              hb_launch_lut_scan(q->d_start, d_total, st) == hipSuccess;
-    uint32_t total = 0;
-    ok = ok && hipMemcpyAsync(&total, d_total, sizeof(total), hipMemcpyDeviceToHost, st) ==
-                   hipSuccess &&
-         hipStreamSynchronize(st) == hipSuccess;
-    if (ok) {
-        q->n_entries = total;
-        ok = hipMalloc(&q->d_entries, (size_t)std::max<uint32_t>(total, 1) * sizeof(uint32_t)) ==
-                 hipSuccess &&
-             hb_launch_lut_fill(d_qflag, len, nullptr, q->d_mat, p.threshold, q->d_start,
-                                q->d_entries, q->d_cells, q->d_pv, st) == hipSuccess &&
-             hipStreamSynchronize(st) == hipSuccess;
-    }
-    hipFree(d_qflag);
-    hipFree(d_total);
-    if (!ok) return fail(HB_ERR_DEVICE);
-    {
-        std::vector<uint32_t> pv(HB_PV_WORDS);
-        if (hipMemcpy(pv.data(), q->d_pv, HB_PV_WORDS * 4, hipMemcpyDeviceToHost) != hipSuccess)
-            return fail(HB_ERR_DEVICE);
-        uint32_t used = 0;
-        for (uint32_t w : pv) used += (uint32_t)__builtin_popcount(w);
-        q->n_cells_used = used;
-    }
+    for (uint32_t w : pv) q->n_cells_used += (uint32_t)__builtin_popcount(w);
     *out = q;
     return HB_OK;
 }
@@ -456,20 +544,26 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
 int hb_query_lut_info(const hb_query *q, uint32_t *n_entries, uint32_t *n_cells_used)
 {
     if (!q) return HB_ERR_ARG;
-    if (n_entries) *n_entries = q->n_entries;
+    if (n_entries) *n_entries = q->gs.n_entries[0];
     if (n_cells_used) *n_cells_used = q->n_cells_used;
     return HB_OK;
 }
 
 int hb_query_lut_fetch(const hb_query *q, uint32_t *cell_start, uint32_t *entries, uint32_t cap)
 {
-    if (!q || !cell_start || (!entries && q->n_entries)) return HB_ERR_ARG;
-    if (cap < q->n_entries) return HB_ERR_CAPACITY;
+    if (!q || !cell_start) return HB_ERR_ARG;
+    const uint32_t ne = q->gs.n_entries[0];
+    if (!entries && ne) return HB_ERR_ARG;
+    if (cap < ne) return HB_ERR_CAPACITY;
     HB_CHECK(hipSetDevice(q->ctx->device));
-    HB_CHECK(hipMemcpy(cell_start, q->d_start, (HB_NCELLS + 1) * sizeof(uint32_t),
+    std::vector<uint2> cells(HB_NCELLS);
+    HB_CHECK(hipMemcpy(cells.data(), q->gs.desc[0].cells, HB_NCELLS * sizeof(uint2),
                        hipMemcpyDeviceToHost));
-    if (q->n_entries)
-        HB_CHECK(hipMemcpy(entries, q->d_entries, (size_t)q->n_entries * sizeof(uint32_t),
+    // every cell holds its exclusive start, empty ones included (CSR)
+    for (int i = 0; i < HB_NCELLS; ++i) cell_start[i] = cells[i].x;
+    cell_start[HB_NCELLS] = ne;
+    if (ne)
+        HB_CHECK(hipMemcpy(entries, q->gs.desc[0].entries, (size_t)ne * sizeof(uint32_t),
                            hipMemcpyDeviceToHost));
     return HB_OK;
 }
@@ -480,52 +574,35 @@ static int launch_scan(hb_scan *s)
     hb_ctx *c = s->ctx;
     hb_db *db = s->db;
     hb_query *q = s->q;
-    const size_t wave_lds = scan_wave_lds(q->diag_len);
-    const size_t fixed = scan_fixed_lds(q->qlen_pad, q->n_ctx);
-    // 4 wavefronts per workgroup share the query tables; fewer when the
-    // diagonal arrays of long (e.g. six-frame) queries do not fit.
-    int waves = 4;
-    while (waves > 1 && fixed + waves * wave_lds > c->max_lds) waves /= 2;
-    const size_t lds = fixed + waves * wave_lds;
-    if (lds > c->max_lds) return HB_ERR_UNSUPPORTED;
-    int per_cu = 0;
-    HB_CHECK(hb_scan_occupancy(lds, waves * HB_WAVE, &per_cu));
-    if (per_cu < 1) per_cu = 1;
-    int blocks = c->n_cu * per_cu;
-    const int need = (int)((db->n_tiles + waves - 1) / waves);
-    if (need < blocks) blocks = std::max(need, 1);
-
+    hb_launch_cfg cfg;
+    int rc = hb_scan_config(c, false, q->gs.qlen_pad_max, q->gs.ctx_pad_max, q->gs.diag_len_max,
+                            db->n_tiles, &cfg);
+    if (rc) return rc;
     hb_scan_args a;
     memset(&a, 0, sizeof(a));
     a.res = db->d_res;
     a.offs = db->d_offs;
     a.order = db->d_order;
     a.tiles = db->d_tiles;
-    a.n_tiles = db->n_tiles;
-    a.cells = q->d_cells;
-    a.entries = q->d_entries;
-    a.n_entries = std::max<uint32_t>(q->n_entries, 1);
-    a.pv = q->d_pv;
-    a.query = q->d_query;
-    a.qlen = q->qlen;
-    a.qlen_pad = q->qlen_pad;
-    a.ctx_off = q->d_ctx;
-    a.ctx_xdrop = q->d_ctx + q->n_ctx;
-    a.ctx_cutoff = q->d_ctx + 2 * q->n_ctx;
-    a.n_ctx = q->n_ctx;
-    a.ctx_pad = ctx_pad_of(q->n_ctx);
-    a.mat32 = q->d_mat;
-    a.window = q->window;
-    a.diag_len = q->diag_len;
-    a.wave_lds = (uint32_t)wave_lds;
-    a.tile_counter = s->d_misc + 1;
+    a.tile_begin = 0;
+    a.tile_end = db->n_tiles;
+    a.groups = q->gs.d_desc;
+    a.n_groups = 1;
+    a.qlen_pad_max = q->gs.qlen_pad_max;
+    a.ctx_pad_max = q->gs.ctx_pad_max;
+    a.diag_len_max = q->gs.diag_len_max;
+    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gs.diag_len_max);
+    a.mat32 = q->gs.d_mat;
+    a.window = q->p.window;
+    a.work_counter = s->d_misc + 1;
     a.out = s->d_out;
     a.out_cap = s->cap;
     a.out_count = s->d_misc;
     a.stats = s->d_stats;
+    a.group_totals = nullptr;
     HB_CHECK(hipMemsetAsync(s->d_misc, 0, 2 * sizeof(uint32_t), c->stream));
     HB_CHECK(hipEventRecord(s->ev0, c->stream));
-    if (db->n_tiles) HB_CHECK(hb_launch_scan(a, blocks, waves * HB_WAVE, lds, c->stream));
+    if (db->n_tiles) HB_CHECK(hb_launch_scan(a, false, cfg.blocks, cfg.waves * HB_WAVE, cfg.lds, c->stream));
     HB_CHECK(hipEventRecord(s->ev1, c->stream));
     return HB_OK;
 }
@@ -535,9 +612,9 @@ int hb_scan_free(hb_scan *s)
     if (!s) return HB_ERR_ARG;
     hipSetDevice(s->ctx->device);
     if (!s->waited) hipStreamSynchronize(s->ctx->stream);
-    pool_put(s->ctx, s->d_out, (size_t)s->cap * sizeof(hb_hsp));
-    pool_put(s->ctx, s->d_misc, 2 * sizeof(uint32_t));
-    pool_put(s->ctx, s->d_stats, (size_t)std::max<uint32_t>(s->db->n, 1) * 3 * sizeof(uint32_t));
+    hb_pool_put(s->ctx, s->d_out, (size_t)s->cap * sizeof(hb_hsp));
+    hb_pool_put(s->ctx, s->d_misc, 2 * sizeof(uint32_t));
+    hb_pool_put(s->ctx, s->d_stats, (size_t)std::max<uint32_t>(s->db->n, 1) * 3 * sizeof(uint32_t));
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     delete s;
@@ -556,9 +633,9 @@ int hb_scan_start(hb_ctx *c, hb_db *db, hb_query *q, hb_scan **out)
     s->db = db;
     s->q = q;
     s->cap = std::max<uint32_t>(1u << 18, db->n / 4);
-    s->d_out = (hb_hsp *)pool_get(c, (size_t)s->cap * sizeof(hb_hsp));
-    s->d_misc = (uint32_t *)pool_get(c, 2 * sizeof(uint32_t));
-    s->d_stats = (uint32_t *)pool_get(c, (size_t)std::max<uint32_t>(db->n, 1) * 3 * sizeof(uint32_t));
+    s->d_out = (hb_hsp *)hb_pool_get(c, (size_t)s->cap * sizeof(hb_hsp));
+    s->d_misc = (uint32_t *)hb_pool_get(c, 2 * sizeof(uint32_t));
+    s->d_stats = (uint32_t *)hb_pool_get(c, (size_t)std::max<uint32_t>(db->n, 1) * 3 * sizeof(uint32_t));
     if (!s->d_out || !s->d_misc || !s->d_stats) {
         hb_scan_free(s);
         return HB_ERR_NOMEM;
@@ -593,9 +670,9 @@ int hb_scan_wait(hb_scan *s)
         }
         // The kernel counted every HSP but stored only cap: rerun with room
         // for all of them (the stage is deterministic).
-        pool_put(c, s->d_out, (size_t)s->cap * sizeof(hb_hsp));
+        hb_pool_put(c, s->d_out, (size_t)s->cap * sizeof(hb_hsp));
         s->cap = n + n / 8 + 1024;
-        s->d_out = (hb_hsp *)pool_get(c, (size_t)s->cap * sizeof(hb_hsp));
+        s->d_out = (hb_hsp *)hb_pool_get(c, (size_t)s->cap * sizeof(hb_hsp));
         if (!s->d_out) {
             s->cap = 0;
             return HB_ERR_NOMEM;
