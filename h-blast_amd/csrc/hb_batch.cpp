@@ -1,0 +1,468 @@
+// hb_batch.cpp -- batched form: many queries, one persistent launch per
+// LDS size class, hipGraph capture/replay, autotuned group size.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "hb_runtime.h"
+
+namespace {
+
+struct Class {
+    int32_t g_begin = 0, g_end = 0;   // groups [g_begin, g_end) of the group set
+    int32_t qlen_pad_max = 0, ctx_pad_max = 4, diag_len_max = 64;
+    hb_launch_cfg cfg;
+    uint32_t tiles_per_item = 1;
+};
+
+}  // namespace
+
+struct hb_batch {
+    hb_ctx *ctx = nullptr;
+    hb_db *db = nullptr;
+    hb_resolved_params p;
+    hb_batch_opts o;
+    std::vector<hb_concat> queries;
+    bool built = false;
+    int32_t group_res = 0;
+    float build_ms = 0;
+    // groups
+    hb_group_set gs;
+    std::vector<Class> classes;
+    std::vector<int32_t> ctx_query, ctx_local, q_base;   // global context -> query, local ctx
+    // run state
+    hb_hsp *d_out = nullptr;
+    uint32_t cap = 0;
+    uint32_t *d_misc = nullptr;                // [0] HSP count, [1 + c] work counter of class c
+    unsigned long long *d_totals = nullptr;    // 3 per group
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float kernel_ms = 0;
+    uint64_t tot[3] = {0, 0, 0};
+    std::vector<hb_hsp> results;               // ordered by (query, oid, s_off, q_off)
+    std::vector<uint64_t> q_first, q_count;
+};
+
+static void free_graph(hb_batch *b)
+{
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+static void free_groups(hb_batch *b)
+{
+    free_graph(b);
+    hb_group_set_free(&b->gs);
+    hipFree(b->d_misc);
+    hipFree(b->d_totals);
+    b->d_misc = nullptr;
+    b->d_totals = nullptr;
+    b->classes.clear();
+    b->built = false;
+}
+
+// Packs queries into groups (first-fit decreasing by length; a query longer
+// than the target forms its own group), concatenates each group with
+// `window` code-0 residues between queries, and orders groups by diagonal
+// array size so that groups of one LDS configuration are contiguous.
+static void make_groups(hb_batch *b, int32_t target, std::vector<hb_concat> *groups,
+                        std::vector<int32_t> *ctx_base, std::vector<int32_t> *diag)
+{
+    const int32_t pad = b->p.window;
+    const int32_t nq = (int32_t)b->queries.size();
+    std::vector<int32_t> ord(nq);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+        return b->queries[x].seq.size() > b->queries[y].seq.size();
+    });
+    std::vector<std::vector<int32_t>> members;
+    std::vector<int32_t> used;
+    for (int32_t qi : ord) {
+        const int32_t L = (int32_t)b->queries[qi].seq.size();
+        int32_t slot = -1;
+        for (size_t gi = 0; gi < members.size(); ++gi)
+            if (used[gi] + pad + L <= target) {
+                slot = (int32_t)gi;
+                break;
+            }
+        if (slot < 0) {
+            members.push_back({qi});
+            used.push_back(L);
+        } else {
+            members[slot].push_back(qi);
+            used[slot] += pad + L;
+        }
+    }
+    // order groups by their diagonal array length (ascending)
+    std::vector<int32_t> gorder(members.size());
+    std::iota(gorder.begin(), gorder.end(), 0);
+    std::stable_sort(gorder.begin(), gorder.end(),
+                     [&](int32_t x, int32_t y) { return used[x] < used[y]; });
+    b->q_base.assign(nq, 0);
+    b->ctx_query.clear();
+    b->ctx_local.clear();
+    groups->clear();
+    ctx_base->clear();
+    diag->clear();
+    for (int32_t gi : gorder) {
+        hb_concat G;
+        ctx_base->push_back((int32_t)b->ctx_query.size());
+        for (size_t k = 0; k < members[gi].size(); ++k) {
+            const int32_t qi = members[gi][k];
+            const hb_concat &Q = b->queries[qi];
+            if (k) G.seq.insert(G.seq.end(), (size_t)pad, (uint8_t)0);
+            const int32_t base = (int32_t)G.seq.size();
+            b->q_base[qi] = base;
+            G.seq.insert(G.seq.end(), Q.seq.begin(), Q.seq.end());
+            for (size_t c = 0; c < Q.ctx_off.size(); ++c) {
+                G.ctx_off.push_back(base + Q.ctx_off[c]);
+                G.ctx_len.push_back(Q.ctx_len[c]);
+                G.xdrop.push_back(Q.xdrop[c]);
+                G.cutoff.push_back(Q.cutoff[c]);
+                b->ctx_query.push_back(qi);
+                b->ctx_local.push_back((int32_t)c);
+            }
+        }
+        diag->push_back(((int32_t)G.seq.size() + b->p.window + 63) & ~63);
+        groups->push_back(std::move(G));
+    }
+}
+
+// Splits the (diagonal-sorted) groups into classes: a new class starts when
+// the diagonal array would grow past twice the class's first one, so that
+// one launch's LDS layout does not waste occupancy on smaller groups.
+static int build(hb_batch *b, int32_t target)
+{
+    free_groups(b);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<hb_concat> groups;
+    std::vector<int32_t> ctx_base, diag;
+    make_groups(b, target, &groups, &ctx_base, &diag);
+    int rc = hb_group_set_build(b->ctx, groups, ctx_base, b->p, &b->gs);
+    if (rc) return rc;
+    const int32_t ng = (int32_t)groups.size();
+    for (int32_t g = 0; g < ng;) {
+        Class c;
+        c.g_begin = g;
+        const int32_t lim = std::max(diag[g] * 2, 1024);
+        while (g < ng && diag[g] <= lim) {
+            c.qlen_pad_max = std::max(c.qlen_pad_max, b->gs.desc[g].qlen_pad);
+            c.ctx_pad_max = std::max(c.ctx_pad_max, hb_ctx_pad(b->gs.desc[g].n_ctx));
+            c.diag_len_max = std::max(c.diag_len_max, diag[g]);
+            ++g;
+        }
+        c.g_end = g;
+        b->classes.push_back(c);
+    }
+    for (Class &c : b->classes) {
+        const uint32_t n_groups = (uint32_t)(c.g_end - c.g_begin);
+        rc = hb_scan_config(b->ctx, true, c.qlen_pad_max, c.ctx_pad_max, c.diag_len_max,
+                            n_groups * b->db->n_tiles, &c.cfg);
+        if (rc) return rc;
+        // about 64 items per workgroup for balance, few enough for the queue atomic
+        const uint64_t want = (uint64_t)c.cfg.blocks * 64;
+        const uint64_t per_group = std::max<uint64_t>(1, want / std::max<uint32_t>(n_groups, 1));
+        c.tiles_per_item = (uint32_t)std::max<uint64_t>(1, (b->db->n_tiles + per_group - 1) / per_group);
+    }
+    if (hipMalloc(&b->d_misc, (1 + b->classes.size()) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&b->d_totals, 3 * (size_t)ng * sizeof(unsigned long long)) != hipSuccess)
+        return HB_ERR_NOMEM;
+    b->group_res = target;
+    b->built = true;
+    b->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return HB_OK;
+}
+
+// Enqueues the whole scan (counter resets + one launch per class) on the
+// context's stream, over tiles [t_begin, t_end).
+static int enqueue(hb_batch *b, uint32_t t_begin, uint32_t t_end)
+{
+    hb_ctx *c = b->ctx;
+    hb_db *db = b->db;
+    const uint32_t nc = (uint32_t)b->classes.size();
+    HB_CHECK(hb_launch_fill_u32(b->d_misc, 0u, 1 + nc, c->stream));
+    HB_CHECK(hb_launch_fill_u32(reinterpret_cast<uint32_t *>(b->d_totals), 0u,
+                                (uint32_t)(6 * b->gs.desc.size()), c->stream));
+    for (uint32_t ci = 0; ci < nc; ++ci) {
+        const Class &C = b->classes[ci];
+        hb_scan_args a;
+        memset(&a, 0, sizeof(a));
+        a.res = db->d_res;
+        a.offs = db->d_offs;
+        a.order = db->d_order;
+        a.tiles = db->d_tiles;
+        a.tile_begin = t_begin;
+        a.tile_end = t_end;
+        a.tiles_per_item = C.tiles_per_item;
+        a.groups = b->gs.d_desc + C.g_begin;
+        a.n_groups = (uint32_t)(C.g_end - C.g_begin);
+        a.qlen_pad_max = C.qlen_pad_max;
+        a.ctx_pad_max = C.ctx_pad_max;
+        a.diag_len_max = C.diag_len_max;
+        a.wave_lds = (uint32_t)hb_scan_wave_lds(C.diag_len_max);
+        a.mat32 = b->gs.d_mat;
+        a.window = b->p.window;
+        a.work_counter = b->d_misc + 1 + ci;
+        a.out = b->d_out;
+        a.out_cap = b->cap;
+        a.out_count = b->d_misc;
+        a.stats = nullptr;
+        a.group_totals = b->d_totals + 3 * (size_t)C.g_begin;
+        if (t_end > t_begin)
+            HB_CHECK(hb_launch_scan(a, true, C.cfg.blocks, C.cfg.waves * HB_WAVE, C.cfg.lds,
+                                    c->stream));
+    }
+    return HB_OK;
+}
+
+static int ensure_out(hb_batch *b, uint32_t cap)
+{
+    if (b->d_out && b->cap >= cap) return HB_OK;
+    free_graph(b);   // the captured launches hold the old buffer
+    hipFree(b->d_out);
+    b->d_out = nullptr;
+    b->cap = 0;
+    if (hipMalloc(&b->d_out, (size_t)cap * sizeof(hb_hsp)) != hipSuccess) return HB_ERR_NOMEM;
+    b->cap = cap;
+    return HB_OK;
+}
+
+// One timed pass over tiles [t_begin, t_end); returns the HSP count.
+static int run_pass(hb_batch *b, uint32_t t_begin, uint32_t t_end, bool use_graph, uint32_t *n_out)
+{
+    hb_ctx *c = b->ctx;
+    const bool full = t_begin == 0 && t_end == b->db->n_tiles;
+    for (;;) {
+        HB_CHECK(hipEventRecord(b->ev0, c->stream));
+        if (use_graph && full) {
+            if (!b->gexec) {
+                HB_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int rc = enqueue(b, t_begin, t_end);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(c->stream, &g);
+                if (rc) {
+                    if (g) hipGraphDestroy(g);
+                    return rc;
+                }
+                HB_CHECK(e);
+                b->graph = g;
+                HB_CHECK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+            }
+            HB_CHECK(hipGraphLaunch(b->gexec, c->stream));
+        } else {
+            int rc = enqueue(b, t_begin, t_end);
+            if (rc) return rc;
+        }
+        HB_CHECK(hipEventRecord(b->ev1, c->stream));
+        uint32_t n = 0;
+        HB_CHECK(hipMemcpyAsync(&n, b->d_misc, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+        HB_CHECK(hipStreamSynchronize(c->stream));
+        if (n <= b->cap) {
+            *n_out = n;
+            HB_CHECK(hipEventElapsedTime(&b->kernel_ms, b->ev0, b->ev1));
+            return HB_OK;
+        }
+        // every HSP was counted but only cap stored: grow and run again
+        int rc = ensure_out(b, n + n / 8 + 4096);
+        if (rc) return rc;
+    }
+}
+
+static int autotune(hb_batch *b, int32_t *best)
+{
+    const uint32_t nt = b->db->n_tiles;
+    uint32_t span = b->o.tune_tiles ? b->o.tune_tiles : std::max<uint32_t>(nt / 50, 64);
+    span = std::min(span, nt);
+    const uint32_t t0 = (nt - span) / 2;   // subjects of median length
+    int32_t longest = 0;
+    for (const hb_concat &q : b->queries) longest = std::max(longest, (int32_t)q.seq.size());
+    float best_ms = 0;
+    *best = 0;
+    for (int32_t cand : {1024, 2048, 3072, 4096, 6144}) {
+        if (cand < 64) continue;
+        int rc = build(b, cand);
+        if (rc == HB_ERR_UNSUPPORTED) continue;
+        if (rc) return rc;
+        uint32_t n = 0;
+        rc = run_pass(b, t0, t0 + span, false, &n);   // warm
+        if (!rc) rc = run_pass(b, t0, t0 + span, false, &n);
+        if (rc == HB_ERR_UNSUPPORTED) continue;
+        if (rc) return rc;
+        if (*best == 0 || b->kernel_ms < best_ms) {
+            best_ms = b->kernel_ms;
+            *best = cand;
+        }
+        if (cand >= longest + b->p.window && cand > 2048) break;   // larger groups change nothing
+    }
+    return *best ? HB_OK : HB_ERR_UNSUPPORTED;
+}
+
+extern "C" {
+
+void hb_batch_opts_default(hb_batch_opts *o)
+{
+    if (!o) return;
+    o->group_residues = 0;
+    o->use_graph = 1;
+    o->tune_tiles = 0;
+}
+
+int hb_batch_create(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_batch_opts *opts,
+                    hb_batch **out)
+{
+    if (!ctx || !db || !out || db->ctx != ctx) return HB_ERR_ARG;
+    *out = nullptr;
+    hb_batch *b = new (std::nothrow) hb_batch;
+    if (!b) return HB_ERR_NOMEM;
+    b->ctx = ctx;
+    b->db = db;
+    hb_batch_opts_default(&b->o);
+    if (opts) b->o = *opts;
+    int rc = hb_resolve_params(params, &b->p);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+        hipEventCreate(&b->ev1) != hipSuccess) {
+        if (b->ev0) hipEventDestroy(b->ev0);
+        delete b;
+        return HB_ERR_DEVICE;
+    }
+    *out = b;
+    return HB_OK;
+}
+
+int hb_batch_add(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                 const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
+                 const int32_t *cutoff, int32_t *query_id)
+{
+    if (!b || !cutoff) return HB_ERR_ARG;
+    if (b->built) return HB_ERR_STATE;
+    int rc = hb_check_query(seq, len, n_ctx, ctx_off, ctx_len, xdrop);
+    if (rc) return rc;
+    hb_concat q;
+    q.seq.assign(seq, seq + len);
+    q.ctx_off.assign(ctx_off, ctx_off + n_ctx);
+    q.ctx_len.assign(ctx_len, ctx_len + n_ctx);
+    q.xdrop.assign(xdrop, xdrop + n_ctx);
+    q.cutoff.assign(cutoff, cutoff + n_ctx);
+    b->queries.push_back(std::move(q));
+    if (query_id) *query_id = (int32_t)b->queries.size() - 1;
+    return HB_OK;
+}
+
+int hb_batch_run(hb_batch *b)
+{
+    if (!b) return HB_ERR_ARG;
+    if (b->queries.empty()) return HB_ERR_STATE;
+    hb_ctx *c = b->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HB_CHECK(hipSetDevice(c->device));
+    int rc = HB_OK;
+    if (!b->built) {
+        int32_t target = b->o.group_residues;
+        if (target <= 0) {
+            rc = ensure_out(b, 1u << 20);
+            if (!rc) rc = autotune(b, &target);
+            if (rc) return rc;
+        }
+        rc = build(b, target);
+        if (rc) return rc;
+    }
+    rc = ensure_out(b, std::max<uint32_t>(b->cap, std::max<uint32_t>(1u << 20, b->db->n / 2)));
+    if (rc) return rc;
+    uint32_t n = 0;
+    rc = run_pass(b, 0, b->db->n_tiles, b->o.use_graph != 0, &n);
+    if (rc) return rc;
+    // fetch, map to queries, order by (query, oid, s_off, q_off)
+    b->results.resize(n);
+    if (n)
+        HB_CHECK(hipMemcpy(b->results.data(), b->d_out, (size_t)n * sizeof(hb_hsp),
+                           hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> tot(3 * b->gs.desc.size());
+    HB_CHECK(hipMemcpy(tot.data(), b->d_totals, tot.size() * sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost));
+    b->tot[0] = b->tot[1] = b->tot[2] = 0;
+    for (size_t g = 0; g < b->gs.desc.size(); ++g)
+        for (int k = 0; k < 3; ++k) b->tot[k] += tot[3 * g + k];
+    std::vector<int32_t> qid(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        hb_hsp &h = b->results[i];
+        const int32_t qi = b->ctx_query[h.context];
+        qid[i] = qi;
+        h.context = b->ctx_local[h.context];
+        h.q_start -= b->q_base[qi];
+        h.q_off -= b->q_base[qi];
+    }
+    std::vector<uint32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
+        const hb_hsp &a = b->results[x], &c2 = b->results[y];
+        if (qid[x] != qid[y]) return qid[x] < qid[y];
+        if (a.oid != c2.oid) return a.oid < c2.oid;
+        if (a.s_off != c2.s_off) return a.s_off < c2.s_off;
+        return a.q_off < c2.q_off;
+    });
+    std::vector<hb_hsp> sorted(n);
+    const size_t nq = b->queries.size();
+    b->q_first.assign(nq, 0);
+    b->q_count.assign(nq, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        sorted[i] = b->results[idx[i]];
+        b->q_count[qid[idx[i]]]++;
+    }
+    for (size_t q = 1; q < nq; ++q) b->q_first[q] = b->q_first[q - 1] + b->q_count[q - 1];
+    b->results.swap(sorted);
+    return HB_OK;
+}
+
+int hb_batch_results(hb_batch *b, int32_t query_id, hb_hsp *hsps, uint64_t cap, uint64_t *n_hsp)
+{
+    if (!b || query_id < 0 || (size_t)query_id >= b->queries.size()) return HB_ERR_ARG;
+    if (b->q_count.size() != b->queries.size()) return HB_ERR_STATE;
+    const uint64_t n = b->q_count[query_id];
+    if (n_hsp) *n_hsp = n;
+    if (!hsps) return HB_OK;
+    if (cap < n) return HB_ERR_CAPACITY;
+    if (n) memcpy(hsps, &b->results[b->q_first[query_id]], (size_t)n * sizeof(hb_hsp));
+    return HB_OK;
+}
+
+int hb_batch_info_get(hb_batch *b, hb_batch_info *info)
+{
+    if (!b || !info) return HB_ERR_ARG;
+    info->n_queries = (int32_t)b->queries.size();
+    info->n_groups = (int32_t)b->gs.desc.size();
+    info->n_classes = (int32_t)b->classes.size();
+    info->group_residues = b->group_res;
+    info->kernel_ms = b->kernel_ms;
+    info->build_ms = b->build_ms;
+    info->lookup_hits = b->tot[0];
+    info->extensions = b->tot[1];
+    info->hsps = b->tot[2];
+    return HB_OK;
+}
+
+int hb_batch_free(hb_batch *b)
+{
+    if (!b) return HB_ERR_ARG;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    free_groups(b);
+    hipFree(b->d_out);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    delete b;
+    return HB_OK;
+}
+
+}  // extern "C"

@@ -42,6 +42,7 @@ struct hb_scan_args {
     const uint32_t *tiles;       // tile boundaries into order
     uint32_t tile_begin;         // tiles [tile_begin, tile_end) are scanned
     uint32_t tile_end;
+    uint32_t tiles_per_item;     // batched: tiles per (group, tile chunk) work item
     // query groups
     const hb_group_desc *groups;
     uint32_t n_groups;
@@ -94,6 +95,8 @@ hipError_t hb_launch_lut(const hb_lut_job *jobs_dev, int n_jobs, int32_t qlen_ma
                          const int8_t *mat32, int32_t threshold, int pass, hipStream_t st);
 hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int block_threads,
                           size_t lds_bytes, hipStream_t st);
+// Sets the kernel's dynamic-LDS limit and returns its occupancy (call before
+// launching or capturing a launch with that LDS size).
 hipError_t hb_scan_occupancy(bool multi, size_t lds_bytes, int block_threads, int *blocks_per_cu);
 hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);
 hipError_t hb_launch_fill_u32(uint32_t *dst, uint32_t value, uint32_t n, hipStream_t st);

@@ -683,14 +683,17 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
         w.flush_totals(0);
     } else {
         uint32_t cur = 0xffffffffu;
-        const uint32_t n_items = n_tiles * a.n_groups;
+        const uint32_t tpi = a.tiles_per_item;
+        const uint32_t n_chunks = (n_tiles + tpi - 1) / tpi;
+        const uint32_t n_items = n_chunks * a.n_groups;
         for (;;) {
             if (threadIdx.x == 0) s_item = atomicAdd(a.work_counter, 1u);
             __syncthreads();
             const uint32_t item = s_item;
             if (item >= n_items) break;
-            const uint32_t g = item / n_tiles;
-            const uint32_t t = a.tile_begin + item % n_tiles;
+            const uint32_t g = item / n_chunks;
+            const uint32_t t0 = a.tile_begin + (item % n_chunks) * tpi;
+            const uint32_t t1 = min(t0 + tpi, a.tile_end);
             if (g != cur) {
                 if (cur != 0xffffffffu) w.flush_totals(cur);
                 const hb_group_desc G = a.groups[g];
@@ -700,8 +703,8 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
                 cur = g;
                 __syncthreads();
             }
-            const uint32_t i_end = a.tiles[t + 1];
-            for (uint32_t it = a.tiles[t] + (uint32_t)wid; it < i_end; it += (uint32_t)nwaves)
+            const uint32_t i_end = a.tiles[t1];
+            for (uint32_t it = a.tiles[t0] + (uint32_t)wid; it < i_end; it += (uint32_t)nwaves)
                 w.scan_subject(it);
             __syncthreads();
         }
@@ -790,9 +793,6 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 {
     if (block_threads < HB_WAVE || block_threads > HB_BLOCK || block_threads % HB_WAVE)
         return hipErrorInvalidValue;
-    hipError_t e = hipFuncSetAttribute(scan_fn(multi), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds_bytes);
-    if (e != hipSuccess) return e;
     if (multi)
         hipLaunchKernelGGL(hb_scan_kernel<true>, dim3(n_blocks), dim3(block_threads), lds_bytes, st,
                            a);

@@ -44,6 +44,8 @@ EXPORTS = [
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
+    "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
+    "hb_batch_results", "hb_batch_info_get", "hb_batch_free",
 ]
 
 
@@ -52,6 +54,17 @@ class HBError(RuntimeError):
         self.code = code
         msg = _lib().hb_strerror(code).decode() if _LIB is not None else ERRORS.get(code, code)
         super().__init__(f"{what}: {ERRORS.get(code, code)} ({msg})")
+
+
+class _BatchOpts(C.Structure):
+    _fields_ = [("group_residues", C.c_int32), ("use_graph", C.c_int32),
+                ("tune_tiles", C.c_uint32)]
+
+
+class _BatchInfo(C.Structure):
+    _fields_ = [("n_queries", C.c_int32), ("n_groups", C.c_int32), ("n_classes", C.c_int32),
+                ("group_residues", C.c_int32), ("kernel_ms", C.c_float), ("build_ms", C.c_float),
+                ("lookup_hits", C.c_uint64), ("extensions", C.c_uint64), ("hsps", C.c_uint64)]
 
 
 class _Params(C.Structure):
@@ -103,6 +116,13 @@ def _lib():
         L.hb_scan_elapsed_ms.argtypes = [vp, C.POINTER(C.c_float)]
         L.hb_scan_free.argtypes = [vp]
         L.hb_diag_copy.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
+        L.hb_batch_opts_default.argtypes = [vp]
+        L.hb_batch_create.argtypes = [vp, vp, vp, vp, pp]
+        L.hb_batch_add.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(i32)]
+        L.hb_batch_run.argtypes = [vp]
+        L.hb_batch_results.argtypes = [vp, i32, vp, u64, C.POINTER(u64)]
+        L.hb_batch_info_get.argtypes = [vp, vp]
+        L.hb_batch_free.argtypes = [vp]
         L.hb_search.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, u64,
                                 C.POINTER(u64)]
         _LIB = L
@@ -305,6 +325,50 @@ class Scan:
     def close(self):
         if self.h:
             _lib().hb_scan_free(self.h)
+            self.h = None
+
+
+class Batch:
+    """Many queries in one batched scan (hb_batch_*)."""
+
+    def __init__(self, ctx: Context, db: Database, group_residues=0, use_graph=True,
+                 tune_tiles=0, threshold=11, window=40, word_size=3, matrix=None):
+        self.ctx, self.db = ctx, db
+        p, self._keep = _params_struct(threshold, window, word_size, matrix)
+        o = _BatchOpts(group_residues, 1 if use_graph else 0, tune_tiles)
+        h = C.c_void_p()
+        _check(_lib().hb_batch_create(ctx.h, db.h, C.byref(p), C.byref(o), C.byref(h)),
+               "hb_batch_create")
+        self.h = h
+        self.specs = []
+
+    def add(self, spec: QuerySpec):
+        qid = C.c_int32()
+        _check(_lib().hb_batch_add(self.h, _ptr(spec.seq), len(spec.seq), spec.n_ctx,
+                                   _ptr(spec.ctx_off), _ptr(spec.ctx_len), _ptr(spec.xdrop),
+                                   _ptr(spec.cutoff), C.byref(qid)), "hb_batch_add")
+        self.specs.append(spec)
+        return qid.value
+
+    def run(self):
+        _check(_lib().hb_batch_run(self.h), "hb_batch_run")
+
+    def results(self, qid):
+        n = C.c_uint64()
+        _check(_lib().hb_batch_results(self.h, qid, None, 0, C.byref(n)), "hb_batch_results")
+        out = np.zeros(n.value, HSP_DTYPE)
+        _check(_lib().hb_batch_results(self.h, qid, _ptr(out), n.value, C.byref(n)),
+               "hb_batch_results")
+        return out
+
+    def info(self):
+        i = _BatchInfo()
+        _check(_lib().hb_batch_info_get(self.h, C.byref(i)), "hb_batch_info_get")
+        return {f: getattr(i, f) for f, _ in _BatchInfo._fields_}
+
+    def close(self):
+        if self.h:
+            _lib().hb_batch_free(self.h)
             self.h = None
 
 

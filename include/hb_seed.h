@@ -174,6 +174,55 @@ int hb_search(hb_ctx *ctx, hb_db *db, const uint8_t *seq, int32_t len,
               const int32_t *xdrop, const int32_t *cutoff, const hb_params *params,
               hb_subject_stats *stats, hb_hsp *hsps, uint64_t cap, uint64_t *n_hsp);
 
+/* --- batched form --------------------------------------------------------
+ * Many queries (BLASTP or six-frame BLASTX, mixed lengths) searched against
+ * the resident database in one persistent launch per size class.  Queries
+ * are packed into groups of about group_residues concatenated residues,
+ * separated by `window` code-0 residues so that hits of different queries
+ * never pair (each query's HSPs equal its HSPs searched alone); each group
+ * has its own lookup table in one device arena, and workgroups pop (group,
+ * tile chunk) items from one queue.  The launch sequence is captured in a
+ * hipGraph on the first run and replayed on later runs.  H-BLAST's closest
+ * counterpart is NCBI's concatenation of a query batch into one query
+ * (contexts) searched by H_BLAST_pre/H_BLAST_get_data
+ * (h_blast.cu:922-1157); here the concatenation is split into groups that
+ * fit on-chip memory. */
+typedef struct hb_batch hb_batch;
+typedef struct hb_batch_opts {
+    int32_t group_residues;  /* target concatenated residues per group; 0 = autotune */
+    int32_t use_graph;       /* 1: capture the launches in a hipGraph and replay it */
+    uint32_t tune_tiles;     /* database tiles timed per autotune candidate (0 = default) */
+} hb_batch_opts;
+
+typedef struct hb_batch_info {
+    int32_t n_queries;
+    int32_t n_groups;
+    int32_t n_classes;       /* launches (groups of one LDS configuration) */
+    int32_t group_residues;  /* chosen (autotuned or given) */
+    float kernel_ms;         /* last run: hipEvent time of the launches */
+    float build_ms;          /* grouping + lookup tables (host clock) */
+    uint64_t lookup_hits;    /* last run, all queries */
+    uint64_t extensions;
+    uint64_t hsps;
+} hb_batch_info;
+
+void hb_batch_opts_default(hb_batch_opts *o);
+int hb_batch_create(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_batch_opts *opts,
+                    hb_batch **out);
+/* Adds one query (same arguments as hb_query_prepare); *query_id is its
+ * index in the batch.  Queries can be added until the first hb_batch_run. */
+int hb_batch_add(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                 const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
+                 const int32_t *cutoff, int32_t *query_id);
+/* Builds groups and lookup tables on the first call (autotuning the group
+ * size if asked), then runs the scan; blocking. */
+int hb_batch_run(hb_batch *b);
+/* HSPs of one query from the last run, in (oid, s_off, q_off) order, with
+ * context and query offsets relative to that query alone. */
+int hb_batch_results(hb_batch *b, int32_t query_id, hb_hsp *hsps, uint64_t cap, uint64_t *n_hsp);
+int hb_batch_info_get(hb_batch *b, hb_batch_info *info);
+int hb_batch_free(hb_batch *b);
+
 /* --- diagnostics ---------------------------------------------------------
  * Copies `bytes` between two fresh device buffers `reps` times with a plain
  * 16-byte-per-lane copy kernel and returns the mean time per copy: the

@@ -1,0 +1,71 @@
+"""Batched form: every query's HSPs from one batched scan equal the oracle's
+for that query alone (bit-exact), across groups, size classes, BLASTP and
+BLASTX, graph replay and the autotuned group size."""
+import numpy as np
+import pytest
+
+import hbseed
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    pq = hbseed.random_protein(3001, 800)
+    res, offs = hbseed.make_db(3000, 6000, plant_query=pq, plant_frac=0.1)
+    ctx = hbseed.Context(0)
+    db = hbseed.Database(ctx, res, offs)
+    specs = []
+    lens = hbseed.lognormal_lengths(3002, 24, 350.0, 0.7, 50, 3000)
+    for i, L in enumerate(lens):
+        q = hbseed.random_protein(3100 + i, int(L))
+        if i % 5 == 0:   # share a planted segment so that there are real homologs
+            q[:min(200, len(q))] = pq[100:100 + min(200, len(q))]
+        specs.append(hbseed.blastp_query(q))
+    specs.append(hbseed.blastp_query(hbseed.random_protein(3200, 5000)))   # longer than a group
+    for i in range(3):
+        seg = hbseed.homolog(3300 + i, pq[200 + 50 * i: 300 + 50 * i], 0.7, 1)
+        specs.append(hbseed.blastx_query(hbseed.blastx_nt(3310 + i, 900, seg, seg[::-1].copy())))
+    yield ctx, db, res, offs, specs
+    db.close()
+    ctx.close()
+
+
+def oracle(spec, res, offs):
+    _, h = oracle_lib.OracleQuery.from_spec(spec).scan(res, offs, threads=8)
+    return h.view(np.int32).reshape(-1, 8)
+
+
+@pytest.mark.parametrize("group_residues", [1024, 2048])
+def test_batch_equals_alone(setup, group_residues):
+    ctx, db, res, offs, specs = setup
+    b = hbseed.Batch(ctx, db, group_residues=group_residues)
+    ids = [b.add(s) for s in specs]
+    b.run()
+    info = b.info()
+    assert info["n_groups"] < len(specs) and info["n_classes"] >= 2
+    total = 0
+    first = {}
+    for qid, spec in zip(ids, specs):
+        got = b.results(qid).view(np.int32).reshape(-1, 8)
+        exp = oracle(spec, res, offs)
+        assert np.array_equal(got, exp), (qid, len(got), len(exp))
+        first[qid] = got.copy()
+        total += len(got)
+    assert total == info["hsps"] and total > 0
+    b.run()   # graph replay
+    for qid in ids:
+        assert np.array_equal(b.results(qid).view(np.int32).reshape(-1, 8), first[qid])
+    b.close()
+
+
+def test_batch_autotune(setup):
+    ctx, db, res, offs, specs = setup
+    b = hbseed.Batch(ctx, db, group_residues=0, tune_tiles=40)
+    ids = [b.add(s) for s in specs[:10]]
+    b.run()
+    assert b.info()["group_residues"] in (1024, 2048, 3072, 4096, 6144)
+    for qid, spec in zip(ids, specs[:10]):
+        assert np.array_equal(b.results(qid).view(np.int32).reshape(-1, 8), oracle(spec, res, offs))
+    b.close()
