@@ -33,6 +33,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def workload_c3(n_queries):
+    """2,048 BLASTP queries, log-normal lengths (median 350, sigma 0.7,
+    clipped 50-3,000), residues from background frequencies (seed 1300)."""
+    lens = hbseed.lognormal_lengths(1300, n_queries, 350.0, 0.7, 50, 3000)
+    return [hbseed.random_protein(1300 * 100003 + i, int(L)) for i, L in enumerate(lens)]
+
+
 def workload_c1(n_subjects):
     q = hbseed.random_protein(1101, 1000)
     res, offs = hbseed.make_db(1100, n_subjects, plant_query=q, plant_frac=0.05)
@@ -79,12 +86,30 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the known-size copy kernel (rocprof calibration)")
+    ap.add_argument("--config", default="c1", choices=["c1", "c3", "c4"],
+                    help="c1: plain 1000-aa query (default); c3: batched 2,048 queries; "
+                         "c4: served Poisson stream over 20M subjects")
+    ap.add_argument("--queries", type=int, default=2048, help="c3: number of queries")
+    ap.add_argument("--plain-sample", type=int, default=48,
+                    help="c3: queries timed one by one through the plain path")
+    ap.add_argument("--qps", type=float, default=2.5, help="c4: Poisson arrival rate")
+    ap.add_argument("--duration", type=float, default=40.0,
+                    help="c4: seconds of arrivals")
+    ap.add_argument("--serve-subjects", type=int, default=20_000_000,
+                    help="c4: database subjects")
+    ap.add_argument("--max-batch-residues", type=int, default=16384,
+                    help="c4: query residues per served batch")
+    ap.add_argument("--max-inflight", type=int, default=2, help="c4: batches in flight")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; --gpus must be 1 "
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "c3":
+        return bench_c3(args)
+    if args.config == "c4":
+        return bench_c4(args)
     log("generating workload")
     q, res, offs = workload_c1(args.subjects)
     n_subj = len(offs) - 1
@@ -176,6 +201,217 @@ def main():
         out["copy_calibration"] = copy
     print(json.dumps(out), flush=True)
     query.close()
+    db.close()
+    ctx.close()
+
+
+def bench_c3(args):
+    """Batched BLASTP: all queries in one batched scan (autotuned groups,
+    hipGraph replay) against the C1 database, beside the plain per-query
+    loop (prepare + scan + results per query) on a sample of the same
+    queries.  One step = one batched pass over the database for all queries."""
+    log("generating workload")
+    _, res, offs = workload_c1(args.subjects)
+    qs = workload_c3(args.queries)
+    n_res = int(offs[-1])
+    ctx = hbseed.Context(0)
+    db = hbseed.Database(ctx, res, offs)
+    log(f"{len(offs) - 1} subjects resident; {len(qs)} queries, "
+        f"{sum(len(q) for q in qs)} query residues")
+    b = hbseed.Batch(ctx, db, group_residues=0)
+    for q in qs:
+        b.add(hbseed.blastp_query(q))
+    t0 = time.perf_counter()
+    b.run()    # builds groups + lookup tables (autotuned), captures the graph
+    first_ms = (time.perf_counter() - t0) * 1e3
+    info = b.info()
+    log(f"first run {first_ms:.0f} ms: {info}")
+    for _ in range(max(0, args.warmup - 1)):
+        b.run()
+    ctx.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+        kms.append(b.info()["kernel_ms"])
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    info = b.info()
+    b.close()
+    # plain per-query loop on a sample (same queries, one at a time)
+    sample = qs[:args.plain_sample]
+    for q in sample[:2]:
+        hbseed.search(ctx, db, hbseed.blastp_query(q))
+    ctx.synchronize()
+    tp = time.perf_counter()
+    for q in sample:
+        hbseed.search(ctx, db, hbseed.blastp_query(q))
+    ctx.synchronize()
+    plain_wall = time.perf_counter() - tp
+    plain_qps = len(sample) / plain_wall
+    qps = len(qs) * args.steps / wall
+    kernel_ms = float(np.mean(kms))
+    abytes = n_res + 24 * (len(offs) - 1) + 32 * info["hsps"]
+    out = {
+        "metric": "queries/s (batched vs plain)",
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"batched BLASTP: {len(qs)} queries (log-normal median 350, "
+                               "sigma 0.7, clipped 50-3000) vs 2,000,000 subjects",
+                   "queries": len(qs), "query_residues": int(sum(len(q) for q in qs)),
+                   "subjects": len(offs) - 1, "residues": n_res,
+                   "group_residues": info["group_residues"], "groups": info["n_groups"],
+                   "launches": info["n_classes"], "seeds": {"db": 1100, "queries": 1300}},
+        "effective_subject_residues_per_s": qps * n_res,
+        "plain_queries_per_s": plain_qps,
+        "plain_sample": len(sample),
+        "speedup_vs_plain": qps / plain_qps,
+        "kernel_ms": kernel_ms,
+        "first_run_ms": first_ms,
+        "build_ms": info["build_ms"],
+        "hsps": info["hsps"], "lookup_hits": info["lookup_hits"],
+        "extensions": info["extensions"],
+        "roofline": {"bound": "hbm", "achieved": abytes / (kernel_ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": abytes / (kernel_ms * 1e-3) / HBM_PEAK, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    db.close()
+    ctx.close()
+
+
+def workload_c4(n, seed=1400):
+    """Arrival offsets (Poisson process at rate qps per second is applied by
+    the caller) and the query mix: 80% BLASTP with lengths as the batched
+    workload (log-normal median 350, sigma 0.7, clipped 50-3,000), 20%
+    BLASTX of U[600, 3000] nt with two planted coding regions."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0, n)
+    kinds = rng.random(n) < 0.8
+    lens = hbseed.lognormal_lengths(seed, n, 350.0, 0.7, 50, 3000)
+    nts = rng.integers(600, 3001, n)
+    src = hbseed.random_protein(1101, 1000)
+    specs = []
+    for i in range(n):
+        if kinds[i]:
+            specs.append(("blastp", hbseed.blastp_query(
+                hbseed.random_protein(seed * 100003 + i, int(lens[i])))))
+        else:
+            seg_len = int(min(100, nts[i] // 6 - 2))
+            a = int(rng.integers(0, 1000 - seg_len))
+            seg = hbseed.homolog(seed * 7 + i, src[a:a + seg_len], 0.7, 1)
+            nt = hbseed.blastx_nt(seed * 13 + i, int(nts[i]), seg, seg[::-1].copy())
+            specs.append(("blastx", hbseed.blastx_query(nt)))
+    return gaps, specs
+
+
+def bench_c4(args):
+    """Served stream: Poisson arrivals at --qps for --duration seconds,
+    submitted from a client thread to the server (hb_serve_*), which folds
+    them into batched scans over the resident database.  Reports completed
+    queries/s, subject residues/s (queries x database residues / wall) and
+    p50/p99 latency from submit to results ready; a sample of the served
+    results is checked against the plain per-query path."""
+    import threading
+    q1 = hbseed.random_protein(1101, 1000)
+    log(f"generating {args.serve_subjects} subjects")
+    res, offs = hbseed.make_db(1400, args.serve_subjects, plant_query=q1, plant_frac=0.05)
+    n_res = int(offs[-1])
+    n_arr = max(1, int(args.qps * args.duration * 1.5) + 20)
+    gaps, specs = workload_c4(n_arr)
+    arrivals = np.cumsum(gaps / args.qps)
+    n_sub = int(np.searchsorted(arrivals, args.duration))
+    n_sub = max(n_sub, 1)
+    specs, arrivals = specs[:n_sub], arrivals[:n_sub]
+    ctx = hbseed.Context(0)
+    db = hbseed.Database(ctx, res, offs)
+    del res
+    log(f"{len(offs) - 1} subjects, {n_res} residues resident; {n_sub} arrivals over "
+        f"{args.duration:.0f} s at {args.qps} q/s")
+    srv = hbseed.Server(ctx, db, max_batch_residues=args.max_batch_residues,
+                        max_inflight=args.max_inflight)
+    # warm-up request (first launch, code-object load), not counted
+    t = srv.submit(hbseed.blastp_query(hbseed.random_protein(1499, 300)))
+    srv.wait(t)
+    srv.results(t)
+    st0 = srv.stats()
+    tickets = [None] * n_sub
+    done_t = [0.0] * n_sub
+    n_hsp = [0] * n_sub
+    sample = {0, n_sub // 2, n_sub - 1}
+    kept = {}
+    t0 = time.perf_counter()
+
+    def client():
+        for i in range(n_sub):
+            d = t0 + arrivals[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(specs[i][1])
+
+    th = threading.Thread(target=client)
+    th.start()
+    last_log = time.perf_counter()
+    for i in range(n_sub):
+        while tickets[i] is None:
+            time.sleep(0.001)
+        srv.wait(tickets[i])
+        done_t[i] = time.perf_counter()
+        h = srv.results(tickets[i])
+        n_hsp[i] = len(h)
+        if i in sample:
+            kept[i] = h
+        if time.perf_counter() - last_log > 20:
+            log(f"{i + 1}/{n_sub} done")
+            last_log = time.perf_counter()
+    th.join()
+    wall = max(done_t) - t0
+    st = srv.stats()
+    srv.stop()
+    ok = True
+    for i, h in kept.items():
+        _, ref = hbseed.search(ctx, db, specs[i][1])
+        ok = ok and np.array_equal(h.view(np.int32), ref.view(np.int32))
+    log(f"served results equal plain path on {len(kept)} sampled queries: {ok}")
+    n_done = st["completed"] - st0["completed"]
+    qres = st["query_residues"] - st0["query_residues"]
+    gpu_ms = st["gpu_ms"] - st0["gpu_ms"]
+    batches = st["batches"] - st0["batches"]
+    abytes = batches * (n_res + 24 * (len(offs) - 1)) + 32 * sum(n_hsp)
+    n_bx = sum(1 for k, _ in specs if k == "blastx")
+    out = {
+        "metric": "queries/s served (p50/p99 latency)",
+        "value": n_done / wall,
+        "unit": "queries/s",
+        "n_gpus": 1, "steps": n_sub, "warmup": 1,
+        "ms_per_step": wall / max(n_done, 1) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"served stream: Poisson arrivals at {args.qps} q/s for "
+                               f"{args.duration:.0f} s, 80% BLASTP (log-normal median 350) / "
+                               f"20% BLASTX (600-3000 nt) vs {len(offs) - 1} subjects",
+                   "subjects": len(offs) - 1, "residues": n_res, "arrivals": n_sub,
+                   "blastx": n_bx, "max_batch_residues": args.max_batch_residues,
+                   "max_inflight": args.max_inflight, "seeds": {"db": 1400, "stream": 1400}},
+        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+        "max_ms": st["max_ms"],
+        "effective_subject_residues_per_s": n_done * n_res / wall,
+        "query_residues_per_s": qres / wall,
+        "batches": batches, "peak_inflight": st["peak_inflight"], "reruns": st["reruns"],
+        "gpu_busy_frac": gpu_ms / (wall * 1e3),
+        "hsps": int(sum(n_hsp)),
+        "sample_equal_plain": ok,
+        "roofline": {"bound": "hbm", "achieved": abytes / (gpu_ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": abytes / (gpu_ms * 1e-3) / HBM_PEAK, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
     db.close()
     ctx.close()
 

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "hb_runtime.h"
@@ -24,6 +25,9 @@ struct Class {
 struct hb_batch {
     hb_ctx *ctx = nullptr;
     hb_db *db = nullptr;
+    hipStream_t st = nullptr;                  // the context stream, or the caller's
+    bool launched = false;                     // hb_batch_launch'ed, not yet finished
+    bool reran = false;                        // the last finish reran an overflowed pass
     hb_resolved_params p;
     hb_batch_opts o;
     std::vector<hb_concat> queries;
@@ -60,8 +64,8 @@ static void free_groups(hb_batch *b)
 {
     free_graph(b);
     hb_group_set_free(&b->gs);
-    hipFree(b->d_misc);
-    hipFree(b->d_totals);
+    hb_pool_put(b->ctx, b->d_misc, (1 + b->classes.size()) * sizeof(uint32_t));
+    hb_pool_put(b->ctx, b->d_totals, 3 * b->gs.desc.size() * sizeof(unsigned long long));
     b->d_misc = nullptr;
     b->d_totals = nullptr;
     b->classes.clear();
@@ -145,7 +149,7 @@ static int build(hb_batch *b, int32_t target)
     std::vector<hb_concat> groups;
     std::vector<int32_t> ctx_base, diag;
     make_groups(b, target, &groups, &ctx_base, &diag);
-    int rc = hb_group_set_build(b->ctx, groups, ctx_base, b->p, &b->gs);
+    int rc = hb_group_set_build(b->ctx, groups, ctx_base, b->p, b->st, &b->gs);
     if (rc) return rc;
     const int32_t ng = (int32_t)groups.size();
     for (int32_t g = 0; g < ng;) {
@@ -171,9 +175,10 @@ static int build(hb_batch *b, int32_t target)
         const uint64_t per_group = std::max<uint64_t>(1, want / std::max<uint32_t>(n_groups, 1));
         c.tiles_per_item = (uint32_t)std::max<uint64_t>(1, (b->db->n_tiles + per_group - 1) / per_group);
     }
-    if (hipMalloc(&b->d_misc, (1 + b->classes.size()) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&b->d_totals, 3 * (size_t)ng * sizeof(unsigned long long)) != hipSuccess)
-        return HB_ERR_NOMEM;
+    b->d_misc = static_cast<uint32_t *>(hb_pool_get(b->ctx, (1 + b->classes.size()) * sizeof(uint32_t)));
+    b->d_totals = static_cast<unsigned long long *>(
+        hb_pool_get(b->ctx, 3 * (size_t)ng * sizeof(unsigned long long)));
+    if (!b->d_misc || !b->d_totals) return HB_ERR_NOMEM;
     b->group_res = target;
     b->built = true;
     b->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -181,15 +186,14 @@ static int build(hb_batch *b, int32_t target)
 }
 
 // Enqueues the whole scan (counter resets + one launch per class) on the
-// context's stream, over tiles [t_begin, t_end).
+// batch's stream, over tiles [t_begin, t_end).
 static int enqueue(hb_batch *b, uint32_t t_begin, uint32_t t_end)
 {
-    hb_ctx *c = b->ctx;
     hb_db *db = b->db;
     const uint32_t nc = (uint32_t)b->classes.size();
-    HB_CHECK(hb_launch_fill_u32(b->d_misc, 0u, 1 + nc, c->stream));
+    HB_CHECK(hb_launch_fill_u32(b->d_misc, 0u, 1 + nc, b->st));
     HB_CHECK(hb_launch_fill_u32(reinterpret_cast<uint32_t *>(b->d_totals), 0u,
-                                (uint32_t)(6 * b->gs.desc.size()), c->stream));
+                                (uint32_t)(6 * b->gs.desc.size()), b->st));
     for (uint32_t ci = 0; ci < nc; ++ci) {
         const Class &C = b->classes[ci];
         hb_scan_args a;
@@ -217,7 +221,7 @@ static int enqueue(hb_batch *b, uint32_t t_begin, uint32_t t_end)
         a.group_totals = b->d_totals + 3 * (size_t)C.g_begin;
         if (t_end > t_begin)
             HB_CHECK(hb_launch_scan(a, true, C.cfg.blocks, C.cfg.waves * HB_WAVE, C.cfg.lds,
-                                    c->stream));
+                                    b->st));
     }
     return HB_OK;
 }
@@ -226,52 +230,68 @@ static int ensure_out(hb_batch *b, uint32_t cap)
 {
     if (b->d_out && b->cap >= cap) return HB_OK;
     free_graph(b);   // the captured launches hold the old buffer
-    hipFree(b->d_out);
-    b->d_out = nullptr;
+    hb_pool_put(b->ctx, b->d_out, (size_t)b->cap * sizeof(hb_hsp));
     b->cap = 0;
-    if (hipMalloc(&b->d_out, (size_t)cap * sizeof(hb_hsp)) != hipSuccess) return HB_ERR_NOMEM;
+    b->d_out = static_cast<hb_hsp *>(hb_pool_get(b->ctx, (size_t)cap * sizeof(hb_hsp)));
+    if (!b->d_out) return HB_ERR_NOMEM;
     b->cap = cap;
     return HB_OK;
+}
+
+// Enqueues one pass over tiles [t_begin, t_end) between the two timing
+// events (replaying the captured graph for a full pass when enabled).
+static int launch_pass(hb_batch *b, uint32_t t_begin, uint32_t t_end, bool use_graph)
+{
+    const bool full = t_begin == 0 && t_end == b->db->n_tiles;
+    HB_CHECK(hipEventRecord(b->ev0, b->st));
+    if (use_graph && full) {
+        if (!b->gexec) {
+            HB_CHECK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue(b, t_begin, t_end);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->st, &g);
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            HB_CHECK(e);
+            b->graph = g;
+            HB_CHECK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        }
+        HB_CHECK(hipGraphLaunch(b->gexec, b->st));
+    } else {
+        int rc = enqueue(b, t_begin, t_end);
+        if (rc) return rc;
+    }
+    HB_CHECK(hipEventRecord(b->ev1, b->st));
+    return HB_OK;
+}
+
+// Waits for the pass enqueued last; returns its HSP count, or grows the
+// output buffer and returns *fits = false when the records did not fit
+// (every HSP is counted, only cap are stored).
+static int end_pass(hb_batch *b, uint32_t *n_out, bool *fits)
+{
+    uint32_t n = 0;
+    HB_CHECK(hipMemcpyAsync(&n, b->d_misc, sizeof(n), hipMemcpyDeviceToHost, b->st));
+    HB_CHECK(hipStreamSynchronize(b->st));
+    *fits = n <= b->cap;
+    if (*fits) {
+        *n_out = n;
+        HB_CHECK(hipEventElapsedTime(&b->kernel_ms, b->ev0, b->ev1));
+        return HB_OK;
+    }
+    return ensure_out(b, n + n / 8 + 4096);
 }
 
 // One timed pass over tiles [t_begin, t_end); returns the HSP count.
 static int run_pass(hb_batch *b, uint32_t t_begin, uint32_t t_end, bool use_graph, uint32_t *n_out)
 {
-    hb_ctx *c = b->ctx;
-    const bool full = t_begin == 0 && t_end == b->db->n_tiles;
     for (;;) {
-        HB_CHECK(hipEventRecord(b->ev0, c->stream));
-        if (use_graph && full) {
-            if (!b->gexec) {
-                HB_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                int rc = enqueue(b, t_begin, t_end);
-                hipGraph_t g = nullptr;
-                hipError_t e = hipStreamEndCapture(c->stream, &g);
-                if (rc) {
-                    if (g) hipGraphDestroy(g);
-                    return rc;
-                }
-                HB_CHECK(e);
-                b->graph = g;
-                HB_CHECK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-            }
-            HB_CHECK(hipGraphLaunch(b->gexec, c->stream));
-        } else {
-            int rc = enqueue(b, t_begin, t_end);
-            if (rc) return rc;
-        }
-        HB_CHECK(hipEventRecord(b->ev1, c->stream));
-        uint32_t n = 0;
-        HB_CHECK(hipMemcpyAsync(&n, b->d_misc, sizeof(n), hipMemcpyDeviceToHost, c->stream));
-        HB_CHECK(hipStreamSynchronize(c->stream));
-        if (n <= b->cap) {
-            *n_out = n;
-            HB_CHECK(hipEventElapsedTime(&b->kernel_ms, b->ev0, b->ev1));
-            return HB_OK;
-        }
-        // every HSP was counted but only cap stored: grow and run again
-        int rc = ensure_out(b, n + n / 8 + 4096);
-        if (rc) return rc;
+        int rc = launch_pass(b, t_begin, t_end, use_graph);
+        bool fits = false;
+        if (!rc) rc = end_pass(b, n_out, &fits);
+        if (rc || fits) return rc;
     }
 }
 
@@ -304,18 +324,67 @@ static int autotune(hb_batch *b, int32_t *best)
     return *best ? HB_OK : HB_ERR_UNSUPPORTED;
 }
 
-extern "C" {
-
-void hb_batch_opts_default(hb_batch_opts *o)
+// Fetches the n records and per-group counters of the last pass, maps
+// records back to their queries (local context and offsets) and orders each
+// query's records by (oid, s_off, q_off).
+static int collect(hb_batch *b, uint32_t n)
 {
-    if (!o) return;
-    o->group_residues = 0;
-    o->use_graph = 1;
-    o->tune_tiles = 0;
+    b->results.resize(n);
+    std::vector<unsigned long long> tot(3 * b->gs.desc.size());
+    if (n)
+        HB_CHECK(hipMemcpyAsync(b->results.data(), b->d_out, (size_t)n * sizeof(hb_hsp),
+                                hipMemcpyDeviceToHost, b->st));
+    HB_CHECK(hipMemcpyAsync(tot.data(), b->d_totals, tot.size() * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, b->st));
+    HB_CHECK(hipStreamSynchronize(b->st));
+    b->tot[0] = b->tot[1] = b->tot[2] = 0;
+    for (size_t g = 0; g < b->gs.desc.size(); ++g)
+        for (int k = 0; k < 3; ++k) b->tot[k] += tot[3 * g + k];
+    // partition by query (counting), then order each query's records on
+    // host threads
+    const size_t nq = b->queries.size();
+    std::vector<int32_t> qid(n);
+    b->q_first.assign(nq + 1, 0);
+    b->q_count.assign(nq, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        hb_hsp &h = b->results[i];
+        const int32_t qi = b->ctx_query[h.context];
+        qid[i] = qi;
+        h.context = b->ctx_local[h.context];
+        h.q_start -= b->q_base[qi];
+        h.q_off -= b->q_base[qi];
+        b->q_count[qi]++;
+    }
+    for (size_t q = 0; q < nq; ++q) b->q_first[q + 1] = b->q_first[q] + b->q_count[q];
+    std::vector<hb_hsp> sorted(n);
+    {
+        std::vector<uint64_t> cur(b->q_first.begin(), b->q_first.end() - 1);
+        for (uint32_t i = 0; i < n; ++i) sorted[cur[qid[i]]++] = b->results[i];
+    }
+    const unsigned nt = std::max(1u, std::min<unsigned>({16u, std::thread::hardware_concurrency(),
+                                                         (unsigned)std::max<size_t>(1, nq)}));
+    auto order = [&](size_t t) {
+        for (size_t q = t; q < nq; q += nt)
+            std::sort(sorted.begin() + (long)b->q_first[q], sorted.begin() + (long)b->q_first[q + 1],
+                      [](const hb_hsp &x, const hb_hsp &y) {
+                          if (x.oid != y.oid) return x.oid < y.oid;
+                          if (x.s_off != y.s_off) return x.s_off < y.s_off;
+                          return x.q_off < y.q_off;
+                      });
+    };
+    if (nt == 1 || n < 65536) {
+        for (size_t t = 0; t < nt; ++t) order(t);
+    } else {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t) th.emplace_back(order, (size_t)t);
+        for (auto &x : th) x.join();
+    }
+    b->results.swap(sorted);
+    return HB_OK;
 }
 
-int hb_batch_create(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_batch_opts *opts,
-                    hb_batch **out)
+int hb_batch_create_on(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_batch_opts *opts,
+                       hipStream_t st, hb_batch **out)
 {
     if (!ctx || !db || !out || db->ctx != ctx) return HB_ERR_ARG;
     *out = nullptr;
@@ -331,6 +400,7 @@ int hb_batch_create(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_ba
         return rc;
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
+    b->st = st ? st : ctx->stream;
     if (hipSetDevice(ctx->device) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
         hipEventCreate(&b->ev1) != hipSuccess) {
         if (b->ev0) hipEventDestroy(b->ev0);
@@ -339,6 +409,88 @@ int hb_batch_create(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_ba
     }
     *out = b;
     return HB_OK;
+}
+
+// Builds the groups (fixed group size: no autotuning on this path) and
+// enqueues one pass on the batch's stream without waiting for it.
+int hb_batch_launch(hb_batch *b)
+{
+    if (!b) return HB_ERR_ARG;
+    if (b->queries.empty() || b->launched) return HB_ERR_STATE;
+    hb_ctx *c = b->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HB_CHECK(hipSetDevice(c->device));
+    int rc = HB_OK;
+    if (!b->built) {
+        rc = build(b, b->o.group_residues > 0 ? b->o.group_residues : 2048);
+        if (rc) return rc;
+    }
+    rc = ensure_out(b, std::max<uint32_t>(b->cap, 4096));
+    if (rc) return rc;
+    rc = launch_pass(b, 0, b->db->n_tiles, false);
+    if (rc) return rc;
+    b->launched = true;
+    return HB_OK;
+}
+
+bool hb_batch_ready(hb_batch *b)
+{
+    return !b->launched || hipEventQuery(b->ev1) != hipErrorNotReady;
+}
+
+// Waits for the launched pass, reruns it once with a larger buffer when
+// the records did not fit, and collects the results.
+int hb_batch_finish(hb_batch *b)
+{
+    if (!b) return HB_ERR_ARG;
+    if (!b->launched) return HB_ERR_STATE;
+    b->launched = false;
+    hb_ctx *c = b->ctx;
+    HB_CHECK(hipSetDevice(c->device));
+    uint32_t n = 0;
+    bool fits = false;
+    int rc = end_pass(b, &n, &fits);
+    if (rc) return rc;
+    b->reran = !fits;
+    if (!fits) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        rc = run_pass(b, 0, b->db->n_tiles, false, &n);
+        if (rc) return rc;
+    }
+    return collect(b, n);
+}
+
+bool hb_batch_rerun_needed(const hb_batch *b) { return b->reran; }
+
+size_t hb_batch_device_bytes(const hb_batch *b)
+{
+    return b->gs.fixed_bytes + b->gs.entries_bytes + (size_t)b->cap * sizeof(hb_hsp) +
+           b->gs.desc.size() * (sizeof(hb_group_desc) + 3 * sizeof(unsigned long long));
+}
+
+// Sets the output capacity (records) before launch.
+int hb_batch_reserve(hb_batch *b, uint32_t cap)
+{
+    if (!b || b->launched) return HB_ERR_STATE;
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    HB_CHECK(hipSetDevice(b->ctx->device));
+    return ensure_out(b, cap);
+}
+
+extern "C" {
+
+void hb_batch_opts_default(hb_batch_opts *o)
+{
+    if (!o) return;
+    o->group_residues = 0;
+    o->use_graph = 1;
+    o->tune_tiles = 0;
+}
+
+int hb_batch_create(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_batch_opts *opts,
+                    hb_batch **out)
+{
+    return hb_batch_create_on(ctx, db, params, opts, nullptr, out);
 }
 
 int hb_batch_add(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_ctx,
@@ -363,7 +515,7 @@ int hb_batch_add(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_ctx,
 int hb_batch_run(hb_batch *b)
 {
     if (!b) return HB_ERR_ARG;
-    if (b->queries.empty()) return HB_ERR_STATE;
+    if (b->queries.empty() || b->launched) return HB_ERR_STATE;
     hb_ctx *c = b->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
     HB_CHECK(hipSetDevice(c->device));
@@ -383,46 +535,7 @@ int hb_batch_run(hb_batch *b)
     uint32_t n = 0;
     rc = run_pass(b, 0, b->db->n_tiles, b->o.use_graph != 0, &n);
     if (rc) return rc;
-    // fetch, map to queries, order by (query, oid, s_off, q_off)
-    b->results.resize(n);
-    if (n)
-        HB_CHECK(hipMemcpy(b->results.data(), b->d_out, (size_t)n * sizeof(hb_hsp),
-                           hipMemcpyDeviceToHost));
-    std::vector<unsigned long long> tot(3 * b->gs.desc.size());
-    HB_CHECK(hipMemcpy(tot.data(), b->d_totals, tot.size() * sizeof(unsigned long long),
-                       hipMemcpyDeviceToHost));
-    b->tot[0] = b->tot[1] = b->tot[2] = 0;
-    for (size_t g = 0; g < b->gs.desc.size(); ++g)
-        for (int k = 0; k < 3; ++k) b->tot[k] += tot[3 * g + k];
-    std::vector<int32_t> qid(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        hb_hsp &h = b->results[i];
-        const int32_t qi = b->ctx_query[h.context];
-        qid[i] = qi;
-        h.context = b->ctx_local[h.context];
-        h.q_start -= b->q_base[qi];
-        h.q_off -= b->q_base[qi];
-    }
-    std::vector<uint32_t> idx(n);
-    std::iota(idx.begin(), idx.end(), 0u);
-    std::sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
-        const hb_hsp &a = b->results[x], &c2 = b->results[y];
-        if (qid[x] != qid[y]) return qid[x] < qid[y];
-        if (a.oid != c2.oid) return a.oid < c2.oid;
-        if (a.s_off != c2.s_off) return a.s_off < c2.s_off;
-        return a.q_off < c2.q_off;
-    });
-    std::vector<hb_hsp> sorted(n);
-    const size_t nq = b->queries.size();
-    b->q_first.assign(nq, 0);
-    b->q_count.assign(nq, 0);
-    for (uint32_t i = 0; i < n; ++i) {
-        sorted[i] = b->results[idx[i]];
-        b->q_count[qid[idx[i]]]++;
-    }
-    for (size_t q = 1; q < nq; ++q) b->q_first[q] = b->q_first[q - 1] + b->q_count[q - 1];
-    b->results.swap(sorted);
-    return HB_OK;
+    return collect(b, n);
 }
 
 int hb_batch_results(hb_batch *b, int32_t query_id, hb_hsp *hsps, uint64_t cap, uint64_t *n_hsp)
@@ -456,9 +569,9 @@ int hb_batch_free(hb_batch *b)
 {
     if (!b) return HB_ERR_ARG;
     hipSetDevice(b->ctx->device);
-    hipStreamSynchronize(b->ctx->stream);
+    hipStreamSynchronize(b->st);
     free_groups(b);
-    hipFree(b->d_out);
+    hb_pool_put(b->ctx, b->d_out, (size_t)b->cap * sizeof(hb_hsp));
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
     delete b;

@@ -32,6 +32,7 @@ struct hb_ctx {
     // per-query loop does not pay hipMalloc/hipFree per call.
     std::mutex pool_mu;
     std::vector<std::pair<void *, size_t>> pool;
+    size_t pool_bytes = 0;
 };
 
 void *hb_pool_get(hb_ctx *c, size_t bytes);
@@ -68,8 +69,10 @@ int hb_check_query(const uint8_t *seq, int32_t len, int32_t n_ctx, const int32_t
                    const int32_t *ctx_len, const int32_t *xdrop);
 
 // Device lookup tables and group descriptors for a set of concatenated
-// queries, all in two allocations (fixed-size tables + entries).
+// queries, all in two allocations (fixed-size tables + entries), taken from
+// and returned to the context's buffer pool.
 struct hb_group_set {
+    hb_ctx *ctx = nullptr;
     std::vector<hb_group_desc> desc;   // host copy
     hb_group_desc *d_desc = nullptr;   // device copy
     void *d_fixed = nullptr;           // cells, starts, pv, query, ctx, qflag per group
@@ -79,8 +82,9 @@ struct hb_group_set {
     int32_t qlen_pad_max = 0, ctx_pad_max = 4, diag_len_max = 64;
     std::vector<uint32_t> n_entries, n_cells_used;
 };
-int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &queries, const std::vector<int32_t> &ctx_base,
-                       const hb_resolved_params &p, hb_group_set *out);
+int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &queries,
+                       const std::vector<int32_t> &ctx_base, const hb_resolved_params &p,
+                       hipStream_t st, hb_group_set *out);
 void hb_group_set_free(hb_group_set *g);
 
 // Launch configuration of the scan for a given LDS layout.
@@ -91,3 +95,17 @@ struct hb_launch_cfg {
 };
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg);
+
+// Asynchronous steps of a batch (hb_batch.cpp), used by the serving loop:
+// the batch runs on the given stream (nullptr: the context's); launch
+// returns at once, ready polls, finish collects (rerunning the pass if the
+// HSP buffer was too small).
+struct hb_batch;
+int hb_batch_create_on(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_batch_opts *opts,
+                       hipStream_t st, hb_batch **out);
+bool hb_batch_rerun_needed(const hb_batch *b);
+int hb_batch_launch(hb_batch *b);
+bool hb_batch_ready(hb_batch *b);
+int hb_batch_finish(hb_batch *b);
+size_t hb_batch_device_bytes(const hb_batch *b);
+int hb_batch_reserve(hb_batch *b, uint32_t cap);

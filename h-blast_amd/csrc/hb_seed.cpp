@@ -45,6 +45,7 @@ void *hb_pool_get(hb_ctx *c, size_t bytes)
                 best = i;
         if (best != (size_t)-1) {
             void *p = c->pool[best].first;
+            c->pool_bytes -= c->pool[best].second;
             c->pool.erase(c->pool.begin() + (long)best);
             return p;
         }
@@ -54,13 +55,17 @@ void *hb_pool_get(hb_ctx *c, size_t bytes)
     return p;
 }
 
+// Returns a buffer to the pool.  hipFree waits for the whole device, so the
+// pool frees (oldest first) only past 64 buffers or 16 GiB.
 void hb_pool_put(hb_ctx *c, void *p, size_t bytes)
 {
     if (!p) return;
     std::lock_guard<std::mutex> lk(c->pool_mu);
     c->pool.emplace_back(p, bytes);
-    if (c->pool.size() > 32) {
+    c->pool_bytes += bytes;
+    while (c->pool.size() > 64 || c->pool_bytes > ((size_t)16 << 30)) {
         hipFree(c->pool.front().first);
+        c->pool_bytes -= c->pool.front().second;
         c->pool.erase(c->pool.begin());
     }
 }
@@ -107,11 +112,11 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 void hb_group_set_free(hb_group_set *g)
 {
-    if (!g) return;
-    hipFree(g->d_desc);
-    hipFree(g->d_fixed);
-    hipFree(g->d_entries);
-    hipFree(g->d_mat);
+    if (!g || !g->ctx) return;
+    hb_pool_put(g->ctx, g->d_desc, g->desc.size() * sizeof(hb_group_desc));
+    hb_pool_put(g->ctx, g->d_fixed, g->fixed_bytes);
+    hb_pool_put(g->ctx, g->d_entries, g->entries_bytes);
+    hb_pool_put(g->ctx, g->d_mat, 1024);
     g->d_desc = nullptr;
     g->d_fixed = nullptr;
     g->d_entries = nullptr;
@@ -122,10 +127,11 @@ void hb_group_set_free(hb_group_set *g)
 // exclusive scan, one read-back of the totals, entries allocation, fill pass.
 int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
                        const std::vector<int32_t> &ctx_base, const hb_resolved_params &p,
-                       hb_group_set *out)
+                       hipStream_t st, hb_group_set *out)
 {
     const size_t n = qs.size();
     if (n == 0 || ctx_base.size() != n) return HB_ERR_ARG;
+    out->ctx = c;
     out->desc.assign(n, hb_group_desc{});
     // per-group layout inside d_fixed
     const size_t cells_b = HB_NCELLS * sizeof(uint2);
@@ -149,9 +155,10 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
     // host image of the inputs (query, ctx, qflag) and the job table
     std::vector<uint8_t> img(out->fixed_bytes, 0);
     std::vector<hb_lut_job> jobs(n);
-    if (hipMalloc(&out->d_fixed, out->fixed_bytes) != hipSuccess ||
-        hipMalloc(&out->d_mat, 1024) != hipSuccess ||
-        hipMalloc(&out->d_desc, n * sizeof(hb_group_desc)) != hipSuccess) {
+    out->d_fixed = hb_pool_get(c, out->fixed_bytes);
+    out->d_mat = static_cast<int8_t *>(hb_pool_get(c, 1024));
+    out->d_desc = static_cast<hb_group_desc *>(hb_pool_get(c, n * sizeof(hb_group_desc)));
+    if (!out->d_fixed || !out->d_mat || !out->d_desc) {
         hb_group_set_free(out);
         return HB_ERR_NOMEM;
     }
@@ -203,14 +210,14 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
         jobs[g].pv = pv;
         jobs[g].total = total;
     }
-    hipStream_t st = c->stream;
-    hb_lut_job *d_jobs = nullptr;
+    hb_lut_job *d_jobs = static_cast<hb_lut_job *>(hb_pool_get(c, n * sizeof(hb_lut_job)));
     auto fail = [&](int rc) {
-        hipFree(d_jobs);
+        hipStreamSynchronize(st);
+        hb_pool_put(c, d_jobs, n * sizeof(hb_lut_job));
         hb_group_set_free(out);
         return rc;
     };
-    if (hipMalloc(&d_jobs, n * sizeof(hb_lut_job)) != hipSuccess) return fail(HB_ERR_NOMEM);
+    if (!d_jobs) return fail(HB_ERR_NOMEM);
     // inputs: copy each group's query/ctx/qflag span (the tables are written on the device)
     for (size_t g = 0; g < n; ++g) {
         // query, context table and qflag are contiguous in the group's span
@@ -235,7 +242,8 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
     size_t ent_total = 0;
     for (uint32_t t : totals) ent_total += t;
     out->entries_bytes = std::max<size_t>(ent_total, 1) * sizeof(uint32_t);
-    if (hipMalloc(&out->d_entries, out->entries_bytes) != hipSuccess) return fail(HB_ERR_NOMEM);
+    out->d_entries = static_cast<uint32_t *>(hb_pool_get(c, out->entries_bytes));
+    if (!out->d_entries) return fail(HB_ERR_NOMEM);
     size_t e = 0;
     out->n_entries.resize(n);
     for (size_t g = 0; g < n; ++g) {
@@ -252,7 +260,7 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
                        hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return fail(HB_ERR_DEVICE);
-    hipFree(d_jobs);
+    hb_pool_put(c, d_jobs, n * sizeof(hb_lut_job));
     return HB_OK;
 }
 
@@ -488,6 +496,9 @@ int hb_query_free(hb_query *q)
 {
     if (!q) return HB_ERR_ARG;
     hipSetDevice(q->ctx->device);
+    // scans of this query run on the context stream; the tables go back to
+    // the pool, so they must not be reused while one still reads them
+    hipStreamSynchronize(q->ctx->stream);
     hb_group_set_free(&q->gs);
     delete q;
     return HB_OK;
@@ -525,7 +536,7 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
         delete q;
         return HB_ERR_DEVICE;
     }
-    rc = hb_group_set_build(c, qs, std::vector<int32_t>(1, 0), q->p, &q->gs);
+    rc = hb_group_set_build(c, qs, std::vector<int32_t>(1, 0), q->p, c->stream, &q->gs);
     if (rc) {
         delete q;
         return rc;
@@ -703,29 +714,37 @@ int hb_scan_results(hb_scan *s, hb_subject_stats *stats, hb_hsp *hsps, uint64_t 
     if (n_hsp) *n_hsp = n;
     if (!hsps && !stats) return HB_OK;
     if (hsps && cap < n) return HB_ERR_CAPACITY;
-    std::vector<hb_hsp> v((size_t)n);
     HB_CHECK(hipSetDevice(s->ctx->device));
-    if (n) HB_CHECK(hipMemcpy(v.data(), s->d_out, (size_t)n * sizeof(hb_hsp), hipMemcpyDeviceToHost));
-    std::sort(v.begin(), v.end(), [](const hb_hsp &x, const hb_hsp &y) {
-        if (x.oid != y.oid) return x.oid < y.oid;
-        if (x.s_off != y.s_off) return x.s_off < y.s_off;
-        return x.q_off < y.q_off;
-    });
-    if (hsps && n) memcpy(hsps, v.data(), (size_t)n * sizeof(hb_hsp));
+    const uint32_t ns = s->db->n;
+    std::vector<uint32_t> raw((size_t)ns * 3);
+    if (ns)
+        HB_CHECK(hipMemcpy(raw.data(), s->d_stats, raw.size() * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost));
+    // place every record at its subject's slot (the kernel counted HSPs per
+    // subject), then order each subject's few records by (s_off, q_off): the
+    // order in which the CPU word finder discovers them
+    std::vector<uint64_t> first((size_t)ns + 1, 0);
+    for (uint32_t i = 0; i < ns; ++i) first[i + 1] = first[i] + raw[3 * (size_t)i + 2];
+    if (first[ns] != n) return HB_ERR_STATE;
+    if (hsps && n) {
+        std::vector<hb_hsp> v((size_t)n);
+        HB_CHECK(hipMemcpy(v.data(), s->d_out, (size_t)n * sizeof(hb_hsp), hipMemcpyDeviceToHost));
+        std::vector<uint64_t> cur(first.begin(), first.end() - 1);
+        for (const hb_hsp &h : v) hsps[cur[h.oid]++] = h;
+        for (uint32_t i = 0; i < ns; ++i)
+            if (first[i + 1] - first[i] > 1)
+                std::sort(hsps + first[i], hsps + first[i + 1], [](const hb_hsp &x, const hb_hsp &y) {
+                    if (x.s_off != y.s_off) return x.s_off < y.s_off;
+                    return x.q_off < y.q_off;
+                });
+    }
     if (stats) {
-        const uint32_t ns = s->db->n;
-        std::vector<uint32_t> raw((size_t)ns * 3);
-        if (ns)
-            HB_CHECK(hipMemcpy(raw.data(), s->d_stats, raw.size() * sizeof(uint32_t),
-                               hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < ns; ++i) {
             stats[i].lookup_hits = raw[3 * (size_t)i];
             stats[i].extensions = raw[3 * (size_t)i + 1];
             stats[i].n_hsp = raw[3 * (size_t)i + 2];
-            stats[i].first = -1;
+            stats[i].first = stats[i].n_hsp ? (int64_t)first[i] : -1;
         }
-        for (uint64_t k = 0; k < n; ++k)
-            if (k == 0 || v[k].oid != v[k - 1].oid) stats[v[k].oid].first = (int64_t)k;
     }
     return HB_OK;
 }

@@ -46,6 +46,8 @@ EXPORTS = [
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
     "hb_batch_results", "hb_batch_info_get", "hb_batch_free",
+    "hb_serve_opts_default", "hb_serve_start", "hb_serve_submit", "hb_serve_wait",
+    "hb_serve_results", "hb_serve_stats_get", "hb_serve_stop",
 ]
 
 
@@ -65,6 +67,22 @@ class _BatchInfo(C.Structure):
     _fields_ = [("n_queries", C.c_int32), ("n_groups", C.c_int32), ("n_classes", C.c_int32),
                 ("group_residues", C.c_int32), ("kernel_ms", C.c_float), ("build_ms", C.c_float),
                 ("lookup_hits", C.c_uint64), ("extensions", C.c_uint64), ("hsps", C.c_uint64)]
+
+
+class _ServeOpts(C.Structure):
+    _fields_ = [("max_batch_queries", C.c_int32), ("max_inflight", C.c_int32),
+                ("max_batch_residues", C.c_int64), ("mem_budget", C.c_int64),
+                ("group_residues", C.c_int32), ("max_pending", C.c_int32),
+                ("hsp_per_cell", C.c_double), ("linger_us", C.c_int64)]
+
+
+class _ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("batches", C.c_uint64), ("query_residues", C.c_uint64),
+                ("peak_bytes", C.c_uint64), ("peak_inflight", C.c_int32),
+                ("reruns", C.c_int32), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("gpu_ms", C.c_double),
+                ("wall_ms", C.c_double)]
 
 
 class _Params(C.Structure):
@@ -123,6 +141,14 @@ def _lib():
         L.hb_batch_results.argtypes = [vp, i32, vp, u64, C.POINTER(u64)]
         L.hb_batch_info_get.argtypes = [vp, vp]
         L.hb_batch_free.argtypes = [vp]
+        L.hb_serve_opts_default.argtypes = [vp]
+        L.hb_serve_opts_default.restype = None
+        L.hb_serve_start.argtypes = [vp, vp, vp, vp, pp]
+        L.hb_serve_submit.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(u64)]
+        L.hb_serve_wait.argtypes = [vp, u64, i32]
+        L.hb_serve_results.argtypes = [vp, u64, vp, u64, C.POINTER(u64)]
+        L.hb_serve_stats_get.argtypes = [vp, vp]
+        L.hb_serve_stop.argtypes = [vp]
         L.hb_search.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, u64,
                                 C.POINTER(u64)]
         _LIB = L
@@ -370,6 +396,72 @@ class Batch:
         if self.h:
             _lib().hb_batch_free(self.h)
             self.h = None
+
+
+class Server:
+    """The served form (hb_serve_*): submit queries from any thread, wait for
+    each ticket, fetch its HSPs."""
+
+    def __init__(self, ctx: Context, db: Database, max_batch_queries=None, max_inflight=None,
+                 max_batch_residues=None, mem_budget=None, group_residues=None,
+                 max_pending=None, hsp_per_cell=None, linger_us=None, threshold=11, window=40,
+                 word_size=3, matrix=None):
+        self.ctx, self.db = ctx, db
+        p, self._keep = _params_struct(threshold, window, word_size, matrix)
+        o = _ServeOpts()
+        _lib().hb_serve_opts_default(C.byref(o))
+        for k, v in (("max_batch_queries", max_batch_queries), ("max_inflight", max_inflight),
+                     ("max_batch_residues", max_batch_residues), ("mem_budget", mem_budget),
+                     ("group_residues", group_residues), ("max_pending", max_pending),
+                     ("hsp_per_cell", hsp_per_cell), ("linger_us", linger_us)):
+            if v is not None:
+                setattr(o, k, v)
+        h = C.c_void_p()
+        _check(_lib().hb_serve_start(ctx.h, db.h, C.byref(p), C.byref(o), C.byref(h)),
+               "hb_serve_start")
+        self.h = h
+
+    def submit(self, spec: QuerySpec):
+        """Returns the request's ticket; raises HBError (code -8 busy, -4
+        capacity) when the request is not admitted."""
+        t = C.c_uint64()
+        _check(_lib().hb_serve_submit(self.h, _ptr(spec.seq), len(spec.seq), spec.n_ctx,
+                                      _ptr(spec.ctx_off), _ptr(spec.ctx_len), _ptr(spec.xdrop),
+                                      _ptr(spec.cutoff), C.byref(t)), "hb_serve_submit")
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        """True when ready, False on time-out."""
+        rc = _lib().hb_serve_wait(self.h, ticket, timeout_ms)
+        if rc == -8:
+            return False
+        _check(rc, "hb_serve_wait")
+        return True
+
+    def results(self, ticket):
+        """HSPs of a finished request (releases the ticket)."""
+        n = C.c_uint64()
+        _check(_lib().hb_serve_results(self.h, ticket, None, 0, C.byref(n)), "hb_serve_results")
+        out = np.zeros(n.value, HSP_DTYPE)
+        _check(_lib().hb_serve_results(self.h, ticket, _ptr(out), n.value, C.byref(n)),
+               "hb_serve_results")
+        return out
+
+    def stats(self):
+        st = _ServeStats()
+        _check(_lib().hb_serve_stats_get(self.h, C.byref(st)), "hb_serve_stats_get")
+        return {f: getattr(st, f) for f, _ in _ServeStats._fields_}
+
+    def stop(self):
+        if self.h:
+            _check(_lib().hb_serve_stop(self.h), "hb_serve_stop")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
 
 
 def search(ctx, db, spec: QuerySpec, **kw):

@@ -223,6 +223,59 @@ int hb_batch_results(hb_batch *b, int32_t query_id, hb_hsp *hsps, uint64_t cap, 
 int hb_batch_info_get(hb_batch *b, hb_batch_info *info);
 int hb_batch_free(hb_batch *b);
 
+/* --- served form ---------------------------------------------------------
+ * A request loop over the resident database: queries are submitted one at
+ * a time from any thread, admitted by a device-memory budget, and folded
+ * into batched scans (the batched form above, fixed group size) that run on
+ * their own streams, up to max_inflight at once, so that a batch formed
+ * while another is on the GPU starts as soon as device resources free up.
+ * Each query's HSPs are identical to those of the query searched alone.
+ * No counterpart in the reference: an H-BLAST process searches the query
+ * batch it was started with (H_BLAST_Execute, functions.cpp:1167-1217). */
+typedef struct hb_server hb_server;
+typedef struct hb_serve_opts {
+    int32_t max_batch_queries;   /* queries per batch (default 64) */
+    int32_t max_inflight;        /* batches on the device at once (default 2) */
+    int64_t max_batch_residues;  /* query residues per batch (default 16384) */
+    int64_t mem_budget;          /* device bytes for admitted batches; 0 = half the free memory */
+    int32_t group_residues;      /* group size of the batched scans (default 2048) */
+    int32_t max_pending;         /* queued requests beyond which submit is refused (default 65536) */
+    double hsp_per_cell;         /* expected HSPs per (query residue x subject residue), for
+                                    the output reservation (default 5e-7) */
+    int64_t linger_us;           /* a batch that is not full waits this long after its oldest
+                                    request for more (default 0: launch at once) */
+} hb_serve_opts;
+
+typedef struct hb_serve_stats {
+    uint64_t submitted, completed, rejected, batches;
+    uint64_t query_residues;     /* completed */
+    uint64_t peak_bytes;         /* largest admitted device footprint */
+    int32_t peak_inflight;       /* most batches on the device at once */
+    int32_t reruns;              /* batches rerun because their output overflowed */
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> results ready, completed queries */
+    double gpu_ms;               /* sum of batch kernel times */
+    double wall_ms;              /* since start */
+} hb_serve_stats;
+
+void hb_serve_opts_default(hb_serve_opts *o);
+int hb_serve_start(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_serve_opts *opts,
+                   hb_server **out);
+/* Queues one query (same arguments as hb_query_prepare).  Returns
+ * HB_ERR_BUSY when max_pending requests are queued and HB_ERR_CAPACITY when
+ * the query alone exceeds the memory budget; *ticket names the request. */
+int hb_serve_submit(hb_server *s, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                    const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
+                    const int32_t *cutoff, uint64_t *ticket);
+/* Waits up to timeout_ms (< 0: no limit) for the request: HB_OK when its
+ * results are ready, HB_ERR_BUSY on time-out, the batch's error otherwise. */
+int hb_serve_wait(hb_server *s, uint64_t ticket, int32_t timeout_ms);
+/* Results of a finished request in (oid, s_off, q_off) order.  With hsps
+ * NULL only *n_hsp is set; a successful copy releases the ticket. */
+int hb_serve_results(hb_server *s, uint64_t ticket, hb_hsp *hsps, uint64_t cap, uint64_t *n_hsp);
+int hb_serve_stats_get(hb_server *s, hb_serve_stats *st);
+/* Finishes every queued and running request, then stops the loop. */
+int hb_serve_stop(hb_server *s);
+
 /* --- diagnostics ---------------------------------------------------------
  * Copies `bytes` between two fresh device buffers `reps` times with a plain
  * 16-byte-per-lane copy kernel and returns the mean time per copy: the
