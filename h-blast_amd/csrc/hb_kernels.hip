@@ -167,40 +167,35 @@ __global__ __launch_bounds__(1024) void lut_scan_kernel(const hb_lut_job *jobs)
 // (h_blast_kernel.cu:1329-1540): positions k = 0..n, n = min(s_off, q_off);
 // strict '>' keeps the first best; stops once max - score >= X.  Returns the
 // best score; *len = positions kept (0 if no prefix improved on maxscore).
-// Residues are fetched four steps at a time so that the LDS/L1 latency is
-// paid once per four steps.
+// The walk is branch-free per lane: eight residue pairs are fetched at once
+// (indices clamped into the sequences, so a lane past its stop reads valid
+// memory and ignores it), a lane that stopped keeps its values, and the
+// loop ends when no lane of the wavefront is still walking.
 template <typename SP>
 __device__ __forceinline__ int32_t ext_left(const int8_t *m, SP S, const uint8_t *Q,
                                             int32_t s_off, int32_t q_off, int32_t X,
                                             int32_t maxscore, int32_t *len)
 {
     const int32_t n = min(s_off, q_off);
-    int32_t best = -1, sc = maxscore, k = 0;
-    bool done = false;
-    for (; k + 4 <= n + 1; k += 4) {
-        const int32_t m0 = m[((uint32_t)Q[q_off - k] << 5) | S[s_off - k]];
-        const int32_t m1 = m[((uint32_t)Q[q_off - k - 1] << 5) | S[s_off - k - 1]];
-        const int32_t m2 = m[((uint32_t)Q[q_off - k - 2] << 5) | S[s_off - k - 2]];
-        const int32_t m3 = m[((uint32_t)Q[q_off - k - 3] << 5) | S[s_off - k - 3]];
-        sc += m0;
-        if (sc > maxscore) { maxscore = sc; best = k; }
-        if (maxscore - sc >= X) { done = true; break; }
-        sc += m1;
-        if (sc > maxscore) { maxscore = sc; best = k + 1; }
-        if (maxscore - sc >= X) { done = true; break; }
-        sc += m2;
-        if (sc > maxscore) { maxscore = sc; best = k + 2; }
-        if (maxscore - sc >= X) { done = true; break; }
-        sc += m3;
-        if (sc > maxscore) { maxscore = sc; best = k + 3; }
-        if (maxscore - sc >= X) { done = true; break; }
-    }
-    if (!done) {
-        for (; k <= n; ++k) {
-            sc += m[((uint32_t)Q[q_off - k] << 5) | S[s_off - k]];
-            if (sc > maxscore) { maxscore = sc; best = k; }
-            if (maxscore - sc >= X) break;
+    int32_t best = -1, sc = maxscore;
+    bool done = n < 0;
+    for (int32_t k0 = 0;; k0 += 8) {
+        int32_t mv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int32_t k = min(k0 + j, n);
+            mv[j] = m[((uint32_t)Q[q_off - k] << 5) | S[s_off - k]];
         }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool live = !done && k0 + j <= n;
+            sc = live ? sc + mv[j] : sc;
+            const bool imp = live && sc > maxscore;
+            maxscore = imp ? sc : maxscore;
+            best = imp ? k0 + j : best;
+            done = !live || maxscore - sc >= X;
+        }
+        if (!__ballot(!done)) break;
     }
     *len = best + 1;
     return maxscore;
@@ -208,42 +203,38 @@ __device__ __forceinline__ int32_t ext_left(const int8_t *m, SP S, const uint8_t
 
 // Right X-drop walk (h_blast_kernel.cu:1133-1326): positions i = 0..n-1,
 // n = min(slen - s_off, qlen - q_off); also stops at score <= 0;
-// *s_last = s_off + stopping index (n if the walk ran out).
+// *s_last = s_off + stopping index (n if the walk ran out).  Branch-free
+// per lane like ext_left.
 template <typename SP>
 __device__ __forceinline__ int32_t ext_right(const int8_t *m, SP S, int32_t slen, const uint8_t *Q,
                                              int32_t qlen, int32_t s_off, int32_t q_off, int32_t X,
                                              int32_t maxscore, int32_t *len, int32_t *s_last)
 {
     const int32_t n = min(slen - s_off, qlen - q_off);
-    int32_t best = -1, sc = maxscore, i = 0;
-    bool done = false;
-    for (; i + 4 <= n; i += 4) {
-        const int32_t m0 = m[((uint32_t)Q[q_off + i] << 5) | S[s_off + i]];
-        const int32_t m1 = m[((uint32_t)Q[q_off + i + 1] << 5) | S[s_off + i + 1]];
-        const int32_t m2 = m[((uint32_t)Q[q_off + i + 2] << 5) | S[s_off + i + 2]];
-        const int32_t m3 = m[((uint32_t)Q[q_off + i + 3] << 5) | S[s_off + i + 3]];
-        sc += m0;
-        if (sc > maxscore) { maxscore = sc; best = i; }
-        if (sc <= 0 || maxscore - sc >= X) { done = true; break; }
-        sc += m1;
-        if (sc > maxscore) { maxscore = sc; best = i + 1; }
-        if (sc <= 0 || maxscore - sc >= X) { i += 1; done = true; break; }
-        sc += m2;
-        if (sc > maxscore) { maxscore = sc; best = i + 2; }
-        if (sc <= 0 || maxscore - sc >= X) { i += 2; done = true; break; }
-        sc += m3;
-        if (sc > maxscore) { maxscore = sc; best = i + 3; }
-        if (sc <= 0 || maxscore - sc >= X) { i += 3; done = true; break; }
-    }
-    if (!done) {
-        for (; i < n; ++i) {
-            sc += m[((uint32_t)Q[q_off + i] << 5) | S[s_off + i]];
-            if (sc > maxscore) { maxscore = sc; best = i; }
-            if (sc <= 0 || maxscore - sc >= X) break;
+    int32_t best = -1, sc = maxscore, stop = n;
+    bool done = n <= 0;
+    for (int32_t i0 = 0;; i0 += 8) {
+        int32_t mv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int32_t i = max(min(i0 + j, n - 1), 0);
+            mv[j] = m[((uint32_t)Q[q_off + i] << 5) | S[s_off + i]];
         }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool live = !done && i0 + j < n;
+            sc = live ? sc + mv[j] : sc;
+            const bool imp = live && sc > maxscore;
+            maxscore = imp ? sc : maxscore;
+            best = imp ? i0 + j : best;
+            const bool brk = live && (sc <= 0 || maxscore - sc >= X);
+            stop = brk ? i0 + j : stop;
+            done = !live || brk;
+        }
+        if (!__ballot(!done)) break;
     }
     *len = best + 1;
-    *s_last = s_off + i;
+    *s_last = s_off + stop;
     return maxscore;
 }
 
