@@ -365,55 +365,64 @@ struct WaveScan {
         wave_lds_sync();
     }
 
-    // Two-hit decisions for one set of hits (at most one per lane; active
-    // lanes are in subject order by lane).  Hits of the set that share a
-    // diagonal are taken one per pass, lowest lane first (LDS atomic max on
-    // the marker), so every diagonal sees its hits in subject order.  A hit
-    // whose diagonal waits on a pending extension joins the blocked list.
+    // One pass of two-hit decisions over a set of hits (at most one per
+    // lane; active lanes are in subject order by lane).  Among the hits of
+    // the set that share a diagonal only the lowest lane decides (LDS atomic
+    // max on the marker); a hit whose diagonal waits on a pending extension
+    // joins the blocked list, also a losing hit whose diagonal became
+    // pending in this pass (behind the winner, in lane order).  Returns true
+    // for the other losing hits: they decide in a later pass, before any
+    // later hit of their diagonal.
     // Per hit (state per diagonal, flagged = extended to the right):
     //   pending            -> wait
     //   flagged            -> s < last: skip; else last = s, unflag
     //   diff >= window     -> last = s
     //   diff < 3           -> skip
     //   otherwise          -> extension (diagonal pending until it ran)
+    __device__ __forceinline__ bool decide_pass(bool active, uint32_t q, int32_t so, uint32_t d)
+    {
+        if (++gen == (1u << 26)) {
+            for (int i = lane; i < D; i += HB_WAVE) marker[i] = 0;
+            wave_lds_sync();
+            gen = 1;
+        }
+        const uint32_t key = (gen << 6) | (uint32_t)(63 - lane);
+        if (active) atomicMax(&marker[d], key);
+        wave_lds_sync();
+        const uint32_t mk = marker[d];
+        const int32_t st = state[d];
+        const bool win = active && mk == key;
+        const uint32_t ust = (uint32_t)st;
+        const bool pend = (ust & kPend) != 0;
+        const bool flag = st < 0;
+        const int32_t sb = so + base;
+        const int32_t diff = (int32_t)((uint32_t)sb - ust);  // s - last if unflagged
+        const bool job = win && !pend && !flag && diff >= 3 && diff < window;
+        const bool wait = win && pend;
+        int32_t nst;
+        if (flag) nst = sb >= (int32_t)(ust & kVal) ? sb : st;
+        else nst = diff >= window ? sb : (job ? (int32_t)kPend : st);
+        if (win && !pend && nst != st) state[d] = nst;
+        wave_lds_sync();
+        const bool lose = active && !win;
+        const bool lwait = lose && ((uint32_t)state[d] & kPend) != 0;
+        const uint64_t mj = __ballot(job), mb = __ballot(wait || lwait);
+        if (job) {
+            const uint32_t delta = (uint32_t)diff;  // 3..window-1
+            jobs[n_jobs + lane_rank(mj)] = make_uint2(q | (delta << 24), (uint32_t)so);
+        }
+        if (wait || lwait) blk[n_blk + lane_rank(mb)] = make_uint2(q, (uint32_t)so);
+        n_jobs += (uint32_t)__popcll(mj);
+        n_blk += (uint32_t)__popcll(mb);
+        n_ext += job ? 1u : 0u;
+        wave_lds_sync();
+        return lose && !lwait;
+    }
+
+    // Decides a set of hits completely (passes until no hit is left).
     __device__ __forceinline__ void decide(bool active, uint32_t q, int32_t so, uint32_t d)
     {
-        for (;;) {
-            if (!__ballot(active)) break;
-            if (++gen == (1u << 26)) {
-                for (int i = lane; i < D; i += HB_WAVE) marker[i] = 0;
-                wave_lds_sync();
-                gen = 1;
-            }
-            const uint32_t key = (gen << 6) | (uint32_t)(63 - lane);
-            if (active) atomicMax(&marker[d], key);
-            wave_lds_sync();
-            const uint32_t mk = marker[d];
-            const int32_t st = state[d];
-            const bool win = active && mk == key;
-            const uint32_t ust = (uint32_t)st;
-            const bool pend = (ust & kPend) != 0;
-            const bool flag = st < 0;
-            const int32_t sb = so + base;
-            const int32_t diff = (int32_t)((uint32_t)sb - ust);  // s - last if unflagged
-            const bool job = win && !pend && !flag && diff >= 3 && diff < window;
-            const bool wait = win && pend;
-            int32_t nst;
-            if (flag) nst = sb >= (int32_t)(ust & kVal) ? sb : st;
-            else nst = diff >= window ? sb : (job ? (int32_t)kPend : st);
-            if (win && !pend && nst != st) state[d] = nst;
-            const uint64_t mj = __ballot(job), mb = __ballot(wait);
-            if (job) {
-                const uint32_t delta = (uint32_t)diff;  // 3..window-1
-                jobs[n_jobs + lane_rank(mj)] = make_uint2(q | (delta << 24), (uint32_t)so);
-            }
-            if (wait) blk[n_blk + lane_rank(mb)] = make_uint2(q, (uint32_t)so);
-            n_jobs += (uint32_t)__popcll(mj);
-            n_blk += (uint32_t)__popcll(mb);
-            n_ext += job ? 1u : 0u;
-            active = active && !win;
-            wave_lds_sync();
-        }
+        while (__ballot(active)) active = decide_pass(active, q, so, d);
     }
 
     // Runs every queued extension (one per lane), writes HSPs with score >=
@@ -489,6 +498,30 @@ struct WaveScan {
         }
     }
 
+    // Moves the hits that lost a pass (lost = true) to lanes 0..n-1 in lane
+    // order; returns n.  Their values travel in cq/cs/cd.
+    __device__ __forceinline__ uint32_t carry_losers(bool lost, uint32_t q, int32_t so, uint32_t d,
+                                                     uint32_t &cq, int32_t &cs, uint32_t &cd)
+    {
+        const uint64_t ml = __ballot(lost);
+        const uint32_t n = (uint32_t)__popcll(ml);
+        if (n) {
+            if (lost) seg[lane_rank(ml)] = (uint32_t)lane;   // source lane of each rank
+            wave_lds_sync();
+            const int src = (uint32_t)lane < n ? (int)(seg[lane] & 63u) : lane;
+            cq = (uint32_t)__shfl((int)q, src, HB_WAVE);
+            cs = __shfl(so, src, HB_WAVE);
+            cd = (uint32_t)__shfl((int)d, src, HB_WAVE);
+            wave_lds_sync();
+        }
+        return n;
+    }
+
+    // Scans one subject.  Hits are taken in (s, q) order, up to 64 per round;
+    // a hit that lost its diagonal to a lower lane in a round is carried to
+    // the front lanes of the next round (ahead of every newer hit, so each
+    // diagonal still sees its hits in subject order), and a round makes a
+    // single decision pass.
     template <typename SP>
     __device__ __forceinline__ void subject(SP S, int32_t slen, uint32_t oid)
     {
@@ -496,15 +529,18 @@ struct WaveScan {
         n_ext = 0;
         n_hsp = 0;
         int32_t s0m = 0;  // s0 mod D
+        uint32_t c = 0;   // carried hits, in lanes 0..c-1
+        uint32_t cq = 0, cd = 0;
+        int32_t cs = 0;
         for (int32_t s0 = 0; s0 + 3 <= slen; s0 += HB_WAVE) {
             const int32_t s = s0 + lane;
             uint32_t cnt = 0, cst = 0;
             if (s + 3 <= slen) {
                 const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
                 if ((pv[idx >> 5] >> (idx & 31)) & 1u) {
-                    const uint2 c = cells[idx];
-                    cst = c.x;
-                    cnt = c.y;
+                    const uint2 c2 = cells[idx];
+                    cst = c2.x;
+                    cnt = c2.y;
                 }
             }
             const uint32_t incl = wave_incl_add(cnt);
@@ -513,33 +549,47 @@ struct WaveScan {
             const int32_t ebase = (int32_t)cst - (int32_t)excl;  // entry index = flat index + ebase
             hits += H;
             uint32_t carry = 0;
-            for (uint32_t r0 = 0; r0 < H; r0 += HB_WAVE) {
+            for (uint32_t r0 = 0; r0 < H;) {
                 if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain(S, slen, oid);
+                // new hits r0 .. r0 + take - 1 go to lanes c .. c + take - 1
+                const uint32_t take = min((uint32_t)HB_WAVE - c, H - r0);
                 if (++rgen == (1u << 25)) {
                     seg[lane] = 0;
                     rgen = 1;
                 }
                 const uint32_t tag = rgen << 7;
-                if (cnt && excl >= r0 && excl < r0 + HB_WAVE) seg[excl - r0] = tag | ((uint32_t)lane + 1);
+                if (cnt && excl >= r0 && excl < r0 + take) seg[excl - r0 + c] = tag | ((uint32_t)lane + 1);
                 wave_lds_sync();
                 const uint32_t sv = seg[lane];
                 const uint32_t v = wave_incl_max((sv & ~127u) == tag ? (sv & 127u) : 0u);
                 const uint32_t p = min(v ? v - 1 : carry, 63u);
-                carry = lane63(p);
-                const uint32_t h = r0 + (uint32_t)lane;
-                const bool active = h < H;
+                carry = (uint32_t)__builtin_amdgcn_readlane((int)p, (int)(c + take - 1));
+                const bool isnew = (uint32_t)lane >= c && (uint32_t)lane < c + take;
+                const uint32_t h = r0 + (uint32_t)lane - c;
                 const int32_t eb = __shfl(ebase, (int)p, HB_WAVE);
                 const uint32_t ei = min((uint32_t)((int32_t)h + eb), n_entries - 1);
-                const uint32_t q = active ? entries[ei] : 0u;
+                const uint32_t qn = isnew ? entries[ei] : 0u;
                 int32_t smp = s0m + (int32_t)p;
                 if (smp >= D) smp -= D;
-                int32_t d = (int32_t)q - smp;
-                if (d < 0) d += D;
+                int32_t dn = (int32_t)qn - smp;
+                if (dn < 0) dn += D;
+                const bool isc = (uint32_t)lane < c;
+                const uint32_t q = isc ? cq : qn;
+                const int32_t so = isc ? cs : s0 + (int32_t)p;
+                const uint32_t d = isc ? cd : (uint32_t)dn;
                 wave_lds_sync();
-                decide(active, q, s0 + (int32_t)p, (uint32_t)d);
+                const bool lost = decide_pass(isc || isnew, q, so, d);
+                c = carry_losers(lost, q, so, d, cq, cs, cd);
+                r0 += take;
             }
             s0m += HB_WAVE;
             if (s0m >= D) s0m -= D;
+        }
+        while (c) {   // hits still carried at the end of the subject
+            if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain(S, slen, oid);
+            const bool isc = (uint32_t)lane < c;
+            const bool lost = decide_pass(isc, cq, cs, cd);
+            c = carry_losers(lost, cq, cs, cd, cq, cs, cd);
         }
         drain(S, slen, oid);
         const uint32_t e = wave_sum(n_ext), hs = wave_sum(n_hsp);
