@@ -210,7 +210,8 @@ static int enqueue(hb_batch *b, uint32_t t_begin, uint32_t t_end)
         a.qlen_pad_max = C.qlen_pad_max;
         a.ctx_pad_max = C.ctx_pad_max;
         a.diag_len_max = C.diag_len_max;
-        a.wave_lds = (uint32_t)hb_scan_wave_lds(C.diag_len_max);
+        a.wave_lds = (uint32_t)hb_scan_wave_lds(C.diag_len_max, C.cfg.scap);
+        a.scap = (uint32_t)C.cfg.scap;
         a.mat32 = b->gs.d_mat;
         a.window = b->p.window;
         a.work_counter = b->d_misc + 1 + ci;

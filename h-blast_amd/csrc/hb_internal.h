@@ -12,7 +12,10 @@
 #define HB_WAVE 64
 #define HB_JOBCAP 128       // queued two-hit extensions per wave
 #define HB_BLKCAP 128       // hits waiting on a pending diagonal per wave
-#define HB_SCAP 1024        // subjects up to this length are staged in LDS
+#define HB_SCAP_MAX 1024    // subjects up to the launch's staging size (<= this) are staged in LDS
+#ifndef HB_MARK
+#define HB_MARK 256         // same-round ordering markers per wave (power of two)
+#endif
 #define HB_BLOCK 256        // at most 4 wavefronts per workgroup
 
 // One query group: a concatenated query (one or more queries / frames as
@@ -51,6 +54,7 @@ struct hb_scan_args {
     int32_t ctx_pad_max;         // multiple of 4
     int32_t diag_len_max;
     uint32_t wave_lds;           // LDS bytes per wave
+    uint32_t scap;               // subjects up to this length are staged in LDS
     const int8_t *mat32;         // 32x32 scores, -128 padded
     int32_t window;
     // work queue and output
@@ -62,11 +66,11 @@ struct hb_scan_args {
     unsigned long long *group_totals;  // batched: 3 per group; else null
 };
 
-// per wave: diagonal state + marker, 64 segment tags, job and blocked queues
-// (8 B entries), staged subject
-static inline size_t hb_scan_wave_lds(int32_t diag_len)
+// per wave: diagonal state, ordering markers, 64 segment tags, job and
+// blocked queues (8 B entries), staged subject
+static inline size_t hb_scan_wave_lds(int32_t diag_len, int32_t scap)
 {
-    return 8 * (size_t)diag_len + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + HB_SCAP;
+    return 4 * (size_t)diag_len + 4 * HB_MARK + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + scap;
 }
 static inline int32_t hb_ctx_pad(int32_t n_ctx) { return (n_ctx + 3) & ~3; }
 // per workgroup: score matrix, presence vector, context tables, query
@@ -97,6 +101,8 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
                           size_t lds_bytes, hipStream_t st);
 // Sets the kernel's dynamic-LDS limit and returns its occupancy (call before
 // launching or capturing a launch with that LDS size).
+// Raises both scan kernels' dynamic LDS limit to max_lds (once per context).
+hipError_t hb_scan_allow_lds(size_t max_lds);
 hipError_t hb_scan_occupancy(bool multi, size_t lds_bytes, int block_threads, int *blocks_per_cu);
 hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);
 hipError_t hb_launch_fill_u32(uint32_t *dst, uint32_t value, uint32_t n, hipStream_t st);

@@ -295,16 +295,21 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 // Per-wavefront LDS of the scan (D = diagonal array length of the launch):
 //   state[D]   int32  diagonal state: bit 31 = extended-right flag,
 //                     bit 30 = extension pending, bits 0..29 = last hit + offset
-//   marker[D]  uint32 same-round ordering marker (generation << 6 | 63 - lane)
+//   marker[HB_MARK] uint32 same-pass ordering marker (generation << 6 | 63 - lane)
+//                     of the diagonals d with d mod HB_MARK = slot; hits of
+//                     different diagonals that share a slot are ordered like
+//                     hits of one diagonal, which is safe (only slower)
 //   seg[64]    uint32 tagged segment starts of the current hit round
 //   jobs[kJobCap] uint2 queued two-hit extensions (q | (s - last) << 24, s)
 //   blk[kBlkCap]  uint2 hits waiting on a pending diagonal (q, s)
-//   sbuf[kSCap]   residues of the current subject (if it fits)
+//   sbuf[scap]    residues of the current subject (if it fits; the host picks
+//                 scap <= HB_SCAP_MAX so that the LDS layout gives the most
+//                 wavefronts per CU)
 // Per-workgroup LDS: score matrix, presence vector, context tables (stride
 // ctx_pad_max) and query of the group being scanned.
 constexpr int kJobCap = HB_JOBCAP;
 constexpr int kBlkCap = HB_BLKCAP;
-constexpr int kSCap = HB_SCAP;
+static_assert((HB_MARK & (HB_MARK - 1)) == 0 && HB_MARK >= HB_WAVE, "HB_MARK: power of two >= 64");
 constexpr uint32_t kFlag = 0x80000000u;
 constexpr uint32_t kPend = 0x40000000u;
 constexpr uint32_t kVal = 0x3fffffffu;
@@ -356,10 +361,8 @@ struct WaveScan {
     // Starts a new group: clears the diagonal state and ordering markers.
     __device__ __forceinline__ void reset()
     {
-        for (int i = lane; i < D; i += HB_WAVE) {
-            state[i] = 0;
-            marker[i] = 0;
-        }
+        for (int i = lane; i < D; i += HB_WAVE) state[i] = 0;
+        for (int i = lane; i < HB_MARK; i += HB_WAVE) marker[i] = 0;
         base = window;
         gen = 0;
         wave_lds_sync();
@@ -382,14 +385,15 @@ struct WaveScan {
     __device__ __forceinline__ bool decide_pass(bool active, uint32_t q, int32_t so, uint32_t d)
     {
         if (++gen == (1u << 26)) {
-            for (int i = lane; i < D; i += HB_WAVE) marker[i] = 0;
+            for (int i = lane; i < HB_MARK; i += HB_WAVE) marker[i] = 0;
             wave_lds_sync();
             gen = 1;
         }
         const uint32_t key = (gen << 6) | (uint32_t)(63 - lane);
-        if (active) atomicMax(&marker[d], key);
+        const uint32_t slot = d & (HB_MARK - 1);
+        if (active) atomicMax(&marker[slot], key);
         wave_lds_sync();
-        const uint32_t mk = marker[d];
+        const uint32_t mk = marker[slot];
         const int32_t st = state[d];
         const bool win = active && mk == key;
         const uint32_t ust = (uint32_t)st;
@@ -616,7 +620,7 @@ struct WaveScan {
         const uint64_t sb = a->offs[oid];
         const int32_t slen = (int32_t)(a->offs[oid + 1] - sb);
         const uint8_t *S = a->res + sb;
-        if (slen <= kSCap) {
+        if ((uint32_t)slen <= a->scap) {
             for (int i = lane; i < slen; i += HB_WAVE) sbuf[i] = S[i];
             wave_lds_sync();
             subject(sbuf, slen, oid);
@@ -693,8 +697,8 @@ __global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
     w.ctx_pad = a.ctx_pad_max;
     w.state = reinterpret_cast<int32_t *>(wb);
     w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)Dm);
-    w.seg = reinterpret_cast<uint32_t *>(wb + 8 * (size_t)Dm);
-    w.jobs = reinterpret_cast<uint2 *>(wb + 8 * (size_t)Dm + 256);
+    w.seg = w.marker + HB_MARK;
+    w.jobs = reinterpret_cast<uint2 *>(w.seg + HB_WAVE);
     w.blk = w.jobs + kJobCap;
     w.sbuf = reinterpret_cast<uint8_t *>(w.blk + kBlkCap);
     w.lane = lane;
@@ -843,11 +847,22 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
     return hipGetLastError();
 }
 
+hipError_t hb_scan_allow_lds(size_t max_lds)
+{
+    for (bool multi : {false, true}) {
+        hipFuncAttributes fa;
+        hipError_t e = hipFuncGetAttributes(&fa, scan_fn(multi));
+        if (e != hipSuccess) return e;
+        // the limit covers dynamic LDS only; the kernel's static LDS comes on top
+        e = hipFuncSetAttribute(scan_fn(multi), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(max_lds - fa.sharedSizeBytes));
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t hb_scan_occupancy(bool multi, size_t lds_bytes, int block_threads, int *blocks_per_cu)
 {
-    hipError_t e = hipFuncSetAttribute(scan_fn(multi), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds_bytes);
-    if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_fn(multi),
                                                         block_threads, lds_bytes);
 }

@@ -92,6 +92,7 @@ struct hb_launch_cfg {
     int waves = 4;
     size_t lds = 0;
     int blocks = 0;
+    int32_t scap = 1024;   // staged-subject capacity per wave
 };
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg);

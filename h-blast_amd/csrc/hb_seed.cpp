@@ -264,26 +264,37 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
     return HB_OK;
 }
 
+// Picks wavefronts per workgroup and the staged-subject capacity: the most
+// wavefronts per CU first (the scan is latency-bound: LDS round trips and
+// dependent lane operations), then the largest staging buffer (subjects
+// above it read their residues from global memory), then more wavefronts
+// per workgroup (they share the query tables).
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg)
 {
-    const size_t wave_lds = hb_scan_wave_lds(diag_len_max);
     const size_t fixed = hb_scan_fixed_lds(qlen_pad_max, ctx_pad_max);
-    // 4 wavefronts per workgroup share the query tables; fewer when the
-    // diagonal arrays of long (e.g. six-frame) queries do not fit.
-    int waves = 4;
-    while (waves > 1 && fixed + waves * wave_lds > c->max_lds) waves /= 2;
-    const size_t lds = fixed + waves * wave_lds;
-    if (lds > c->max_lds) return HB_ERR_UNSUPPORTED;
-    int per_cu = 0;
-    HB_CHECK(hb_scan_occupancy(multi, lds, waves * HB_WAVE, &per_cu));
-    if (per_cu < 1) per_cu = 1;
-    int blocks = c->n_cu * per_cu;
-    const uint32_t per_block = multi ? 1u : (uint32_t)waves;
+    int best_wpc = 0;
+    for (int waves : {4, 2, 1}) {
+        for (int32_t scap = HB_SCAP_MAX; scap >= 256; scap -= 64) {
+            const size_t lds = fixed + waves * hb_scan_wave_lds(diag_len_max, scap);
+            if (lds > c->max_lds) continue;
+            int per_cu = 0;
+            HB_CHECK(hb_scan_occupancy(multi, lds, waves * HB_WAVE, &per_cu));
+            const int wpc = per_cu * waves;
+            if (wpc > best_wpc || (wpc == best_wpc && scap > cfg->scap)) {
+                best_wpc = wpc;
+                cfg->waves = waves;
+                cfg->lds = lds;
+                cfg->scap = scap;
+                cfg->blocks = per_cu;
+            }
+        }
+    }
+    if (best_wpc == 0) return HB_ERR_UNSUPPORTED;
+    int blocks = c->n_cu * std::max(cfg->blocks, 1);
+    const uint32_t per_block = multi ? 1u : (uint32_t)cfg->waves;
     const int need = (int)((work_items + per_block - 1) / per_block);
     if (need < blocks) blocks = std::max(need, 1);
-    cfg->waves = waves;
-    cfg->lds = lds;
     cfg->blocks = blocks;
     return HB_OK;
 }
@@ -360,6 +371,13 @@ int hb_create(int device, hb_ctx **out)
     if (hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, device) ==
             hipSuccess && (size_t)lds_optin > c->max_lds)
         c->max_lds = (size_t)lds_optin;
+    // one dynamic-LDS limit for every launch of the scan kernels, so that
+    // launches of different sizes on different streams never disagree
+    if (hb_scan_allow_lds(c->max_lds) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return HB_ERR_DEVICE;
+    }
     *out = c;
     return HB_OK;
 }
@@ -527,7 +545,7 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
     qs[0].xdrop.assign(xdrop, xdrop + n_ctx);
     qs[0].cutoff.assign(cutoff, cutoff + n_ctx);
     const int32_t D = (len + q->p.window + 63) & ~63;
-    if (hb_scan_fixed_lds((len + 15) & ~15, hb_ctx_pad(n_ctx)) + hb_scan_wave_lds(D) > c->max_lds) {
+    if (hb_scan_fixed_lds((len + 15) & ~15, hb_ctx_pad(n_ctx)) + hb_scan_wave_lds(D, 256) > c->max_lds) {
         delete q;
         return HB_ERR_UNSUPPORTED;
     }
@@ -602,7 +620,8 @@ static int launch_scan(hb_scan *s)
     a.qlen_pad_max = q->gs.qlen_pad_max;
     a.ctx_pad_max = q->gs.ctx_pad_max;
     a.diag_len_max = q->gs.diag_len_max;
-    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gs.diag_len_max);
+    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gs.diag_len_max, cfg.scap);
+    a.scap = (uint32_t)cfg.scap;
     a.mat32 = q->gs.d_mat;
     a.window = q->p.window;
     a.work_counter = s->d_misc + 1;
