@@ -670,8 +670,11 @@ __device__ __forceinline__ void bind_group(WaveScan &w, const hb_group_desc &G)
 // MULTI = true: many groups; the workgroup pops (group, tile) items and its
 // wavefronts share the tile's subjects, so all of them use one group's LDS
 // tables at a time.
+// Register budget for five wavefronts per SIMD (no spills at this limit):
+// the scan is latency-bound, and short queries or small groups leave LDS
+// for that many.
 template <bool MULTI>
-__global__ __launch_bounds__(HB_BLOCK) void hb_scan_kernel(hb_scan_args a)
+__global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void hb_scan_kernel(hb_scan_args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t s_item;
