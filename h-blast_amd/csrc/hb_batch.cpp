@@ -299,14 +299,14 @@ static int run_pass(hb_batch *b, uint32_t t_begin, uint32_t t_end, bool use_grap
 static int autotune(hb_batch *b, int32_t *best)
 {
     const uint32_t nt = b->db->n_tiles;
-    uint32_t span = b->o.tune_tiles ? b->o.tune_tiles : std::max<uint32_t>(nt / 50, 64);
+    uint32_t span = b->o.tune_tiles ? b->o.tune_tiles : std::max<uint32_t>(nt / 25, 64);
     span = std::min(span, nt);
     const uint32_t t0 = (nt - span) / 2;   // subjects of median length
     int32_t longest = 0;
     for (const hb_concat &q : b->queries) longest = std::max(longest, (int32_t)q.seq.size());
     float best_ms = 0;
     *best = 0;
-    for (int32_t cand : {1024, 2048, 3072, 4096, 6144}) {
+    for (int32_t cand : {512, 768, 1024, 1536, 2048, 3072, 4096}) {
         if (cand < 64) continue;
         int rc = build(b, cand);
         if (rc == HB_ERR_UNSUPPORTED) continue;
@@ -423,7 +423,7 @@ int hb_batch_launch(hb_batch *b)
     HB_CHECK(hipSetDevice(c->device));
     int rc = HB_OK;
     if (!b->built) {
-        rc = build(b, b->o.group_residues > 0 ? b->o.group_residues : 2048);
+        rc = build(b, b->o.group_residues > 0 ? b->o.group_residues : 768);
         if (rc) return rc;
     }
     rc = ensure_out(b, std::max<uint32_t>(b->cap, 4096));

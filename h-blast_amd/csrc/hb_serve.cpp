@@ -234,7 +234,7 @@ void hb_serve_opts_default(hb_serve_opts *o)
     o->max_inflight = 2;
     o->max_batch_residues = 16384;
     o->mem_budget = 0;
-    o->group_residues = 2048;
+    o->group_residues = 768;
     o->max_pending = 65536;
     o->hsp_per_cell = 5e-7;
     o->linger_us = 0;
@@ -255,7 +255,7 @@ int hb_serve_start(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_ser
     hb_resolved_params rp;
     int rc = hb_resolve_params(params, &rp);
     if (rc) return rc;
-    if (o.group_residues == 0) o.group_residues = 2048;
+    if (o.group_residues == 0) o.group_residues = 768;
     HB_CHECK(hipSetDevice(ctx->device));
     if (o.mem_budget == 0) {
         size_t fr = 0, tot = 0;

@@ -238,7 +238,7 @@ typedef struct hb_serve_opts {
     int32_t max_inflight;        /* batches on the device at once (default 2) */
     int64_t max_batch_residues;  /* query residues per batch (default 16384) */
     int64_t mem_budget;          /* device bytes for admitted batches; 0 = half the free memory */
-    int32_t group_residues;      /* group size of the batched scans (default 2048) */
+    int32_t group_residues;      /* group size of the batched scans (default 768) */
     int32_t max_pending;         /* queued requests beyond which submit is refused (default 65536) */
     double hsp_per_cell;         /* expected HSPs per (query residue x subject residue), for
                                     the output reservation (default 5e-7) */
