@@ -677,10 +677,9 @@ template <bool MULTI>
 __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void hb_scan_kernel(hb_scan_args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ uint32_t s_item;
+    __shared__ uint32_t s_item, s_next;
     const int lane = threadIdx.x & (HB_WAVE - 1);
     const int wid = threadIdx.x / HB_WAVE;
-    const int nwaves = blockDim.x / HB_WAVE;
     int8_t *sm_mat = reinterpret_cast<int8_t *>(lds);
     uint32_t *sm_pv = reinterpret_cast<uint32_t *>(lds + 1024);
     int32_t *sm_ctx = reinterpret_cast<int32_t *>(lds + 1024 + 4 * HB_PV_WORDS);
@@ -751,9 +750,19 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
                 cur = g;
                 __syncthreads();
             }
+            // the chunk's subjects are taken one at a time by whichever
+            // wavefront is free (LDS counter), so the workgroup's wavefronts
+            // reach the next barrier at most one subject apart
+            if (threadIdx.x == 0) s_next = a.tiles[t0];
+            __syncthreads();
             const uint32_t i_end = a.tiles[t1];
-            for (uint32_t it = a.tiles[t0] + (uint32_t)wid; it < i_end; it += (uint32_t)nwaves)
+            for (;;) {
+                uint32_t it = 0;
+                if (lane == 0) it = atomicAdd(&s_next, 1u);
+                it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
+                if (it >= i_end) break;
                 w.scan_subject(it);
+            }
             __syncthreads();
         }
         if (cur != 0xffffffffu) w.flush_totals(cur);
