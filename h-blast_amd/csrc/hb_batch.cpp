@@ -332,12 +332,13 @@ static int collect(hb_batch *b, uint32_t n)
 {
     b->results.resize(n);
     std::vector<unsigned long long> tot(3 * b->gs.desc.size());
-    if (n)
-        HB_CHECK(hipMemcpyAsync(b->results.data(), b->d_out, (size_t)n * sizeof(hb_hsp),
-                                hipMemcpyDeviceToHost, b->st));
     HB_CHECK(hipMemcpyAsync(tot.data(), b->d_totals, tot.size() * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, b->st));
     HB_CHECK(hipStreamSynchronize(b->st));
+    if (n) {
+        const int rc = hb_copy_out(b->ctx, b->results.data(), b->d_out, (size_t)n * sizeof(hb_hsp), b->st);
+        if (rc) return rc;
+    }
     b->tot[0] = b->tot[1] = b->tot[2] = 0;
     for (size_t g = 0; g < b->gs.desc.size(); ++g)
         for (int k = 0; k < 3; ++k) b->tot[k] += tot[3 * g + k];

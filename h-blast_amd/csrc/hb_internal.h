@@ -106,3 +106,17 @@ hipError_t hb_scan_allow_lds(size_t max_lds);
 hipError_t hb_scan_occupancy(bool multi, size_t lds_bytes, int block_threads, int *blocks_per_cu);
 hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);
 hipError_t hb_launch_fill_u32(uint32_t *dst, uint32_t value, uint32_t n, hipStream_t st);
+
+// Plain-form result layout on the device (hb_kernels.hip): per-subject stats
+// with first-record indices, records grouped by subject and ordered by
+// (s_off, q_off).  Subjects with more than HB_FIN_SORT_MAX records are left
+// unordered and listed in big[] (count in *n_big) for the host.
+#define HB_FIN_SORT_MAX 256
+struct hb_fin_big {
+    uint32_t oid, n;
+    uint64_t first;
+};
+// scratch: ns + ns / 4096 + 1 uint32
+hipError_t hb_launch_finalize(const uint32_t *st3, uint32_t ns, const hb_hsp *recs, uint32_t n,
+                              hb_subject_stats *stats_out, hb_hsp *recs_out, uint32_t *scratch,
+                              hb_fin_big *big, uint32_t big_cap, uint32_t *n_big, hipStream_t st);

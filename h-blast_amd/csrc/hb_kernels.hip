@@ -769,6 +769,134 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     }
 }
 
+// ---------------------------------------------------------------- results
+// Device-side result layout for the plain form: per-subject counters become
+// hb_subject_stats with the index of the subject's first record (exclusive
+// prefix of HSP counts over oids), records are scattered to their subject's
+// slot and each subject's records are ordered by (s_off, q_off) -- the
+// order in which the CPU word finder discovers them.
+constexpr int kFinThreads = 1024;
+constexpr int kFinPer = 4;
+constexpr uint32_t kFinSpan = kFinThreads * kFinPer;   // subjects per block
+
+// Block-wide exclusive scan of v (kFinThreads threads); returns the block total in *total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sm, uint32_t *total)
+{
+    const int lane = threadIdx.x & (HB_WAVE - 1), wid = threadIdx.x / HB_WAVE;
+    const uint32_t incl = wave_incl_add(v);
+    if (lane == HB_WAVE - 1) sm[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+        const uint32_t w = lane < kFinThreads / HB_WAVE ? sm[lane] : 0u;
+        const uint32_t wi = wave_incl_add(w);
+        if (lane < kFinThreads / HB_WAVE) sm[lane] = wi - w;
+        if (lane == HB_WAVE - 1) sm[kFinThreads / HB_WAVE] = wi;
+    }
+    __syncthreads();
+    const uint32_t r = sm[wid] + incl - v;
+    *total = sm[kFinThreads / HB_WAVE];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kFinThreads) void fin_sum_kernel(const uint32_t *st3, uint32_t ns,
+                                                              uint32_t *part)
+{
+    __shared__ uint32_t sm[kFinThreads / HB_WAVE + 1];
+    uint32_t v = 0;
+    for (int k = 0; k < kFinPer; ++k) {
+        const uint32_t i = blockIdx.x * kFinSpan + (uint32_t)k * kFinThreads + threadIdx.x;
+        if (i < ns) v += st3[3 * (size_t)i + 2];
+    }
+    uint32_t total = 0;
+    block_excl_scan(v, sm, &total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the nb block totals in place.
+__global__ __launch_bounds__(kFinThreads) void fin_part_kernel(uint32_t *part, uint32_t nb)
+{
+    __shared__ uint32_t sm[kFinThreads / HB_WAVE + 1];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kFinThreads) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < nb ? part[i] : 0u;
+        uint32_t total = 0;
+        const uint32_t e = block_excl_scan(v, sm, &total);
+        if (i < nb) part[i] = carry + e;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(kFinThreads) void fin_write_kernel(const uint32_t *st3, uint32_t ns,
+                                                                const uint32_t *part,
+                                                                hb_subject_stats *out,
+                                                                uint32_t *cursor)
+{
+    __shared__ uint32_t sm[kFinThreads / HB_WAVE + 1];
+    const uint32_t i0 = blockIdx.x * kFinSpan + (uint32_t)threadIdx.x * kFinPer;
+    uint32_t cnt[kFinPer];
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kFinPer; ++k) {
+        cnt[k] = i0 + k < ns ? st3[3 * (size_t)(i0 + k) + 2] : 0u;
+        v += cnt[k];
+    }
+    uint32_t total = 0;
+    uint32_t first = part[blockIdx.x] + block_excl_scan(v, sm, &total);
+#pragma unroll
+    for (int k = 0; k < kFinPer; ++k) {
+        const uint32_t i = i0 + k;
+        if (i < ns) {
+            hb_subject_stats o;
+            o.lookup_hits = st3[3 * (size_t)i];
+            o.extensions = st3[3 * (size_t)i + 1];
+            o.n_hsp = cnt[k];
+            o.first = cnt[k] ? (int64_t)first : -1;
+            out[i] = o;
+            cursor[i] = first;
+        }
+        first += cnt[k];
+    }
+}
+
+__global__ void fin_scatter_kernel(const hb_hsp *in, uint32_t n, uint32_t *cursor, hb_hsp *out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const hb_hsp h = in[i];
+        out[atomicAdd(&cursor[h.oid], 1u)] = h;
+    }
+}
+
+// One thread per subject: insertion sort of its records; subjects with more
+// than HB_FIN_SORT_MAX records are listed for the host.
+__global__ void fin_sort_kernel(const hb_subject_stats *st, uint32_t ns, hb_hsp *recs,
+                                hb_fin_big *big, uint32_t big_cap, uint32_t *n_big)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const uint32_t n = st[i].n_hsp;
+    if (n < 2) return;
+    const uint64_t f = (uint64_t)st[i].first;
+    if (n > HB_FIN_SORT_MAX) {
+        const uint32_t k = atomicAdd(n_big, 1u);
+        if (k < big_cap) big[k] = hb_fin_big{i, n, f};
+        return;
+    }
+    hb_hsp *r = recs + f;
+    for (uint32_t a = 1; a < n; ++a) {
+        const hb_hsp x = r[a];
+        uint32_t b = a;
+        while (b > 0 && (r[b - 1].s_off > x.s_off ||
+                         (r[b - 1].s_off == x.s_off && r[b - 1].q_off > x.q_off))) {
+            r[b] = r[b - 1];
+            --b;
+        }
+        r[b] = x;
+    }
+}
+
 // Plain 16-byte-per-lane copy: the known-size reference kernel for
 // calibrating HBM byte counters and the achievable copy bandwidth.
 __global__ __launch_bounds__(256) void copy_kernel(uint4 *dst, const uint4 *src, uint64_t n16)
@@ -794,6 +922,27 @@ hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st,
                        reinterpret_cast<uint4 *>(dst), reinterpret_cast<const uint4 *>(src), n16);
+    return hipGetLastError();
+}
+
+hipError_t hb_launch_finalize(const uint32_t *st3, uint32_t ns, const hb_hsp *recs, uint32_t n,
+                              hb_subject_stats *stats_out, hb_hsp *recs_out, uint32_t *scratch,
+                              hb_fin_big *big, uint32_t big_cap, uint32_t *n_big, hipStream_t st)
+{
+    if (ns == 0) return hipSuccess;
+    const uint32_t nb = (ns + kFinSpan - 1) / kFinSpan;
+    uint32_t *part = scratch;        // nb
+    uint32_t *cursor = scratch + nb; // ns
+    hipLaunchKernelGGL(fin_sum_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, part);
+    hipLaunchKernelGGL(fin_part_kernel, dim3(1), dim3(kFinThreads), 0, st, part, nb);
+    hipLaunchKernelGGL(fin_write_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, part,
+                       stats_out, cursor);
+    if (n) {
+        hipLaunchKernelGGL(fin_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, st, recs, n,
+                           cursor, recs_out);
+        hipLaunchKernelGGL(fin_sort_kernel, dim3((ns + 255) / 256), dim3(256), 0, st, stats_out,
+                           ns, recs_out, big, big_cap, n_big);
+    }
     return hipGetLastError();
 }
 

@@ -33,7 +33,15 @@ struct hb_ctx {
     std::mutex pool_mu;
     std::vector<std::pair<void *, size_t>> pool;
     size_t pool_bytes = 0;
+    // pinned host staging for result copies (grown on demand, kept)
+    std::mutex pin_mu;
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
 };
+
+// Copies device bytes to pageable host memory through the context's pinned
+// staging buffer (DMA at full link speed, then a multi-threaded host copy).
+int hb_copy_out(hb_ctx *c, void *dst, const void *src, size_t bytes, hipStream_t st);
 
 void *hb_pool_get(hb_ctx *c, size_t bytes);
 void hb_pool_put(hb_ctx *c, void *p, size_t bytes);

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "hb_runtime.h"
@@ -68,6 +69,38 @@ void hb_pool_put(hb_ctx *c, void *p, size_t bytes)
         c->pool_bytes -= c->pool.front().second;
         c->pool.erase(c->pool.begin());
     }
+}
+
+int hb_copy_out(hb_ctx *c, void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    if (bytes == 0) return HB_OK;
+    std::lock_guard<std::mutex> lk(c->pin_mu);
+    if (c->pinned_bytes < bytes) {
+        if (c->pinned) hipHostFree(c->pinned);
+        c->pinned = nullptr;
+        c->pinned_bytes = 0;
+        const size_t want = std::max(bytes, (size_t)64 << 20);
+        HB_CHECK(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
+        c->pinned_bytes = want;
+    }
+    HB_CHECK(hipMemcpyAsync(c->pinned, src, bytes, hipMemcpyDeviceToHost, st));
+    HB_CHECK(hipStreamSynchronize(st));
+    const unsigned nt = bytes < ((size_t)4 << 20)
+                            ? 1u
+                            : std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const size_t per = (bytes + nt - 1) / nt;
+    auto part = [&](unsigned t) {
+        const size_t b = (size_t)t * per, e = std::min(bytes, b + per);
+        if (b < e) memcpy(static_cast<char *>(dst) + b, static_cast<const char *>(c->pinned) + b, e - b);
+    };
+    if (nt == 1) {
+        part(0);
+    } else {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t) th.emplace_back(part, t);
+        for (auto &x : th) x.join();
+    }
+    return HB_OK;
 }
 
 int hb_resolve_params(const hb_params *pp, hb_resolved_params *out)
@@ -388,6 +421,7 @@ int hb_destroy(hb_ctx *c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto &b : c->pool) hipFree(b.first);
+    if (c->pinned) hipHostFree(c->pinned);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return HB_OK;
@@ -733,38 +767,75 @@ int hb_scan_results(hb_scan *s, hb_subject_stats *stats, hb_hsp *hsps, uint64_t 
     if (n_hsp) *n_hsp = n;
     if (!hsps && !stats) return HB_OK;
     if (hsps && cap < n) return HB_ERR_CAPACITY;
-    HB_CHECK(hipSetDevice(s->ctx->device));
     const uint32_t ns = s->db->n;
-    std::vector<uint32_t> raw((size_t)ns * 3);
-    if (ns)
-        HB_CHECK(hipMemcpy(raw.data(), s->d_stats, raw.size() * sizeof(uint32_t),
-                           hipMemcpyDeviceToHost));
-    // place every record at its subject's slot (the kernel counted HSPs per
-    // subject), then order each subject's few records by (s_off, q_off): the
-    // order in which the CPU word finder discovers them
-    std::vector<uint64_t> first((size_t)ns + 1, 0);
-    for (uint32_t i = 0; i < ns; ++i) first[i + 1] = first[i] + raw[3 * (size_t)i + 2];
-    if (first[ns] != n) return HB_ERR_STATE;
-    if (hsps && n) {
-        std::vector<hb_hsp> v((size_t)n);
-        HB_CHECK(hipMemcpy(v.data(), s->d_out, (size_t)n * sizeof(hb_hsp), hipMemcpyDeviceToHost));
-        std::vector<uint64_t> cur(first.begin(), first.end() - 1);
-        for (const hb_hsp &h : v) hsps[cur[h.oid]++] = h;
-        for (uint32_t i = 0; i < ns; ++i)
-            if (first[i + 1] - first[i] > 1)
-                std::sort(hsps + first[i], hsps + first[i + 1], [](const hb_hsp &x, const hb_hsp &y) {
-                    if (x.s_off != y.s_off) return x.s_off < y.s_off;
-                    return x.q_off < y.q_off;
-                });
+    if (ns == 0) return HB_OK;
+    hb_ctx *c = s->ctx;
+    HB_CHECK(hipSetDevice(c->device));
+    // the layout is built on the device (counts -> first indices, records
+    // grouped by subject and ordered by (s_off, q_off)), then copied out
+    constexpr uint32_t kBig = 4096;
+    const uint32_t nb = (ns + 4095) / 4096;
+    const size_t b_st = (size_t)ns * sizeof(hb_subject_stats);
+    const size_t b_rec = (size_t)std::max<uint64_t>(n, 1) * sizeof(hb_hsp);
+    const size_t b_scr = ((size_t)ns + nb + 1) * sizeof(uint32_t);
+    const size_t b_big = kBig * sizeof(hb_fin_big);
+    auto *d_st = static_cast<hb_subject_stats *>(hb_pool_get(c, b_st));
+    auto *d_rec = static_cast<hb_hsp *>(hb_pool_get(c, b_rec));
+    auto *d_scr = static_cast<uint32_t *>(hb_pool_get(c, b_scr));
+    auto *d_big = static_cast<hb_fin_big *>(hb_pool_get(c, b_big));
+    auto release = [&]() {
+        hb_pool_put(c, d_st, b_st);
+        hb_pool_put(c, d_rec, b_rec);
+        hb_pool_put(c, d_scr, b_scr);
+        hb_pool_put(c, d_big, b_big);
+    };
+    if (!d_st || !d_rec || !d_scr || !d_big) {
+        release();
+        return HB_ERR_NOMEM;
     }
-    if (stats) {
-        for (uint32_t i = 0; i < ns; ++i) {
-            stats[i].lookup_hits = raw[3 * (size_t)i];
-            stats[i].extensions = raw[3 * (size_t)i + 1];
-            stats[i].n_hsp = raw[3 * (size_t)i + 2];
-            stats[i].first = stats[i].n_hsp ? (int64_t)first[i] : -1;
+    uint32_t *d_nbig = d_scr + ns + nb;
+    hipStream_t st = c->stream;
+    uint32_t nbig = 0;
+    hipError_t e = hb_launch_fill_u32(d_nbig, 0u, 1, st);
+    if (e == hipSuccess)
+        e = hb_launch_finalize(s->d_stats, ns, s->d_out, (uint32_t)n, d_st, d_rec, d_scr, d_big,
+                               kBig, d_nbig, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&nbig, d_nbig, sizeof(nbig), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && stats && hb_copy_out(c, stats, d_st, b_st, st) != HB_OK)
+        e = hipErrorUnknown;
+    if (e == hipSuccess && hsps && n &&
+        hb_copy_out(c, hsps, d_rec, (size_t)n * sizeof(hb_hsp), st) != HB_OK)
+        e = hipErrorUnknown;
+    // subjects with many records are ordered here
+    std::vector<hb_fin_big> big;
+    if (e == hipSuccess && hsps && nbig) {
+        if (nbig <= kBig) {
+            big.resize(nbig);
+            e = hipMemcpy(big.data(), d_big, nbig * sizeof(hb_fin_big), hipMemcpyDeviceToHost);
+        } else {
+            std::vector<hb_subject_stats> all;
+            const hb_subject_stats *sp = stats;
+            if (!sp) {
+                all.resize(ns);
+                e = hipMemcpy(all.data(), d_st, b_st, hipMemcpyDeviceToHost);
+                sp = all.data();
+            }
+            for (uint32_t i = 0; e == hipSuccess && i < ns; ++i)
+                if (sp[i].n_hsp > HB_FIN_SORT_MAX)
+                    big.push_back(hb_fin_big{i, sp[i].n_hsp, (uint64_t)sp[i].first});
         }
     }
+    release();
+    if (e != hipSuccess) {
+        hb_note_hip_error(e, "hb_scan_results", __LINE__);
+        return HB_ERR_DEVICE;
+    }
+    for (const hb_fin_big &b : big)
+        std::sort(hsps + b.first, hsps + b.first + b.n, [](const hb_hsp &x, const hb_hsp &y) {
+            if (x.s_off != y.s_off) return x.s_off < y.s_off;
+            return x.q_off < y.q_off;
+        });
     return HB_OK;
 }
 
