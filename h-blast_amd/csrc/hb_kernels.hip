@@ -21,6 +21,9 @@
 
 #include "hb_internal.h"
 
+// Global (address space 1) pointee qualifier.
+#define HB_GLOBAL __attribute__((address_space(1)))
+
 namespace {
 
 __device__ __forceinline__ void wave_lds_sync()
@@ -322,8 +325,12 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m)
 struct WaveScan {
     const hb_scan_args *a;
     // group being scanned
-    const uint2 *cells;
-    const uint32_t *entries;
+    // Lookup-table pointers come from a descriptor in memory, so the
+    // compiler cannot tell their address space; as generic pointers their
+    // loads would be flat loads, which also count against the LDS wait
+    // counter.
+    const HB_GLOBAL uint64_t *cells;   // {start, count} as one 64-bit word
+    const HB_GLOBAL uint32_t *entries;
     uint32_t n_entries;
     int32_t qlen;
     int32_t n_ctx;
@@ -542,9 +549,9 @@ struct WaveScan {
             if (s + 3 <= slen) {
                 const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
                 if ((pv[idx >> 5] >> (idx & 31)) & 1u) {
-                    const uint2 c2 = cells[idx];
-                    cst = c2.x;
-                    cnt = c2.y;
+                    const uint64_t c2 = cells[idx];
+                    cst = (uint32_t)c2;
+                    cnt = (uint32_t)(c2 >> 32);
                 }
             }
             const uint32_t incl = wave_incl_add(cnt);
@@ -657,8 +664,8 @@ __device__ __forceinline__ void load_group(const hb_scan_args &a, const hb_group
 
 __device__ __forceinline__ void bind_group(WaveScan &w, const hb_group_desc &G)
 {
-    w.cells = G.cells;
-    w.entries = G.entries;
+    w.cells = (const HB_GLOBAL uint64_t *)G.cells;
+    w.entries = (const HB_GLOBAL uint32_t *)G.entries;
     w.n_entries = G.n_entries > 0 ? G.n_entries : 1u;
     w.qlen = G.qlen;
     w.n_ctx = G.n_ctx;
