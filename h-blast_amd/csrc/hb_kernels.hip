@@ -170,11 +170,14 @@ __global__ __launch_bounds__(1024) void lut_scan_kernel(const hb_lut_job *jobs)
 // (h_blast_kernel.cu:1329-1540): positions k = 0..n, n = min(s_off, q_off);
 // strict '>' keeps the first best; stops once max - score >= X.  Returns the
 // best score; *len = positions kept (0 if no prefix improved on maxscore).
-// The walk is branch-free per lane: eight residue pairs are fetched at once
-// (indices clamped into the sequences, so a lane past its stop reads valid
-// memory and ignores it), a lane that stopped keeps its values, and the
-// loop ends when no lane of the wavefront is still walking.
-template <typename SP>
+// The walk is branch-free per lane: eight residue pairs are fetched at once,
+// a lane that stopped keeps its best score and length (its running score
+// may change afterwards and is not used), and the loop ends when no lane of
+// the wavefront is still walking.  With the subject in LDS (SL) the fetches
+// past a sequence end read other LDS bytes (or 0 past the allocation) and
+// are ignored; with the subject in global memory the positions are clamped
+// into the sequences.
+template <bool SL, typename SP>
 __device__ __forceinline__ int32_t ext_left(const int8_t *m, SP S, const uint8_t *Q,
                                             int32_t s_off, int32_t q_off, int32_t X,
                                             int32_t maxscore, int32_t *len)
@@ -184,15 +187,23 @@ __device__ __forceinline__ int32_t ext_left(const int8_t *m, SP S, const uint8_t
     bool done = n < 0;
     for (int32_t k0 = 0;; k0 += 8) {
         int32_t mv[8];
+        if (SL) {
+            // qp[7 - j] = Q[q_off - k0 - j]
+            const uint8_t *qp = Q + (q_off - k0 - 7);
+            const SP sp = S + (s_off - k0 - 7);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int32_t k = min(k0 + j, n);
-            mv[j] = m[((uint32_t)Q[q_off - k] << 5) | S[s_off - k]];
+            for (int j = 0; j < 8; ++j) mv[j] = m[((uint32_t)qp[7 - j] << 5) | sp[7 - j]];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int32_t k = max(min(k0 + j, n), 0);
+                mv[j] = m[((uint32_t)Q[q_off - k] << 5) | S[s_off - k]];
+            }
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const bool live = !done && k0 + j <= n;
-            sc = live ? sc + mv[j] : sc;
+            sc += mv[j];
             const bool imp = live && sc > maxscore;
             maxscore = imp ? sc : maxscore;
             best = imp ? k0 + j : best;
@@ -208,7 +219,7 @@ __device__ __forceinline__ int32_t ext_left(const int8_t *m, SP S, const uint8_t
 // n = min(slen - s_off, qlen - q_off); also stops at score <= 0;
 // *s_last = s_off + stopping index (n if the walk ran out).  Branch-free
 // per lane like ext_left.
-template <typename SP>
+template <bool SL, typename SP>
 __device__ __forceinline__ int32_t ext_right(const int8_t *m, SP S, int32_t slen, const uint8_t *Q,
                                              int32_t qlen, int32_t s_off, int32_t q_off, int32_t X,
                                              int32_t maxscore, int32_t *len, int32_t *s_last)
@@ -218,15 +229,22 @@ __device__ __forceinline__ int32_t ext_right(const int8_t *m, SP S, int32_t slen
     bool done = n <= 0;
     for (int32_t i0 = 0;; i0 += 8) {
         int32_t mv[8];
+        if (SL) {
+            const uint8_t *qp = Q + (q_off + i0);
+            const SP sp = S + (s_off + i0);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int32_t i = max(min(i0 + j, n - 1), 0);
-            mv[j] = m[((uint32_t)Q[q_off + i] << 5) | S[s_off + i]];
+            for (int j = 0; j < 8; ++j) mv[j] = m[((uint32_t)qp[j] << 5) | sp[j]];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int32_t i = max(min(i0 + j, n - 1), 0);
+                mv[j] = m[((uint32_t)Q[q_off + i] << 5) | S[s_off + i]];
+            }
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const bool live = !done && i0 + j < n;
-            sc = live ? sc + mv[j] : sc;
+            sc += mv[j];
             const bool imp = live && sc > maxscore;
             maxscore = imp ? sc : maxscore;
             best = imp ? i0 + j : best;
@@ -249,7 +267,7 @@ struct ext_result {
 // best-scoring prefix of the second word, extend left from maxscore 0, and
 // extend right from the left score only if the left walk reached
 // s_left_off (the end of the first hit's word).
-template <typename SP>
+template <bool SL, typename SP>
 __device__ __forceinline__ ext_result ext_two_hit(const int8_t *m, SP S, int32_t slen,
                                                   const uint8_t *Q, int32_t qlen,
                                                   int32_t s_left_off, int32_t s_right_off,
@@ -270,10 +288,10 @@ __device__ __forceinline__ ext_result ext_two_hit(const int8_t *m, SP S, int32_t
     int32_t left_len = 0, right_len = 0, right_score = 0;
     r.right = 0;
     r.s_last = s_right_off;
-    left_score = ext_left(m, S, Q, s_right_off - 1, q_right_off - 1, X, 0, &left_len);
+    left_score = ext_left<SL>(m, S, Q, s_right_off - 1, q_right_off - 1, X, 0, &left_len);
     if (left_len >= s_right_off - s_left_off) {
         r.right = 1;
-        right_score = ext_right(m, S, slen, Q, qlen, s_right_off, q_right_off, X, left_score,
+        right_score = ext_right<SL>(m, S, slen, Q, qlen, s_right_off, q_right_off, X, left_score,
                                 &right_len, &r.s_last);
     }
     r.q_start = q_right_off - left_len;
@@ -438,7 +456,7 @@ struct WaveScan {
 
     // Runs every queued extension (one per lane), writes HSPs with score >=
     // cutoff and the diagonals' new states.
-    template <typename SP>
+    template <bool SL, typename SP>
     __device__ __forceinline__ void run_jobs(SP S, int32_t slen, uint32_t oid)
     {
         for (uint32_t j0 = 0; j0 < n_jobs; j0 += HB_WAVE) {
@@ -452,7 +470,7 @@ struct WaveScan {
                 so = (int32_t)jb.y;
                 const int32_t last_s = so - (int32_t)(jb.x >> 24);
                 k = context_of(ctx, n_ctx, q);
-                r = ext_two_hit(mat, S, slen, qry, qlen, last_s + 3, so, q, ctx[ctx_pad + k]);
+                r = ext_two_hit<SL>(mat, S, slen, qry, qlen, last_s + 3, so, q, ctx[ctx_pad + k]);
                 state[diag_of((uint32_t)q, so)] =
                     r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + base)) : so + base;
             }
@@ -500,11 +518,11 @@ struct WaveScan {
         }
     }
 
-    template <typename SP>
+    template <bool SL, typename SP>
     __device__ __forceinline__ void drain(SP S, int32_t slen, uint32_t oid)
     {
         while (n_jobs) {
-            run_jobs(S, slen, oid);
+            run_jobs<SL>(S, slen, oid);
             replay();
         }
     }
@@ -533,7 +551,7 @@ struct WaveScan {
     // the front lanes of the next round (ahead of every newer hit, so each
     // diagonal still sees its hits in subject order), and a round makes a
     // single decision pass.
-    template <typename SP>
+    template <bool SL, typename SP>
     __device__ __forceinline__ void subject(SP S, int32_t slen, uint32_t oid)
     {
         uint32_t hits = 0;
@@ -561,7 +579,7 @@ struct WaveScan {
             hits += H;
             uint32_t carry = 0;
             for (uint32_t r0 = 0; r0 < H;) {
-                if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain(S, slen, oid);
+                if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain<SL>(S, slen, oid);
                 // new hits r0 .. r0 + take - 1 go to lanes c .. c + take - 1
                 const uint32_t take = min((uint32_t)HB_WAVE - c, H - r0);
                 if (++rgen == (1u << 25)) {
@@ -597,12 +615,12 @@ struct WaveScan {
             if (s0m >= D) s0m -= D;
         }
         while (c) {   // hits still carried at the end of the subject
-            if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain(S, slen, oid);
+            if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain<SL>(S, slen, oid);
             const bool isc = (uint32_t)lane < c;
             const bool lost = decide_pass(isc, cq, cs, cd);
             c = carry_losers(lost, cq, cs, cd, cq, cs, cd);
         }
-        drain(S, slen, oid);
+        drain<SL>(S, slen, oid);
         const uint32_t e = wave_sum(n_ext), hs = wave_sum(n_hsp);
         if (a->stats && lane == 0) {
             a->stats[3 * (size_t)oid + 0] = hits;
@@ -630,10 +648,10 @@ struct WaveScan {
         if ((uint32_t)slen <= a->scap) {
             for (int i = lane; i < slen; i += HB_WAVE) sbuf[i] = S[i];
             wave_lds_sync();
-            subject(sbuf, slen, oid);
+            subject<true>(sbuf, slen, oid);
             wave_lds_sync();
         } else {
-            subject(S, slen, oid);
+            subject<false>(S, slen, oid);
         }
     }
 
