@@ -12,7 +12,9 @@
 #define HB_WAVE 64
 #define HB_JOBCAP 128       // queued two-hit extensions per wave
 #define HB_BLKCAP 128       // hits waiting on a pending diagonal per wave
+#ifndef HB_SCAP_MAX
 #define HB_SCAP_MAX 1024    // subjects up to the launch's staging size (<= this) are staged in LDS
+#endif
 #ifndef HB_MARK
 #define HB_MARK 256         // same-round ordering markers per wave (power of two)
 #endif
@@ -68,15 +70,22 @@ struct hb_scan_args {
 
 // per wave: diagonal state, ordering markers, 64 segment tags, job and
 // blocked queues (8 B entries), staged subject
+// The regions of a workgroup's wavefronts are padded so that their distance
+// modulo 512 bytes is at least 128: distances of 0 or 64 bytes (mod 512)
+// made the C1 scan 8% slower on gfx950 (measured), which points to LDS bank
+// or port conflicts between wavefronts.
 static inline size_t hb_scan_wave_lds(int32_t diag_len, int32_t scap)
 {
-    return 4 * (size_t)diag_len + 4 * HB_MARK + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + scap;
+    size_t b = 4 * (size_t)diag_len + 4 * HB_MARK + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + scap;
+    const size_t r = b % 512;
+    if (r < 128) b += 128 - r;
+    return b;
 }
 static inline int32_t hb_ctx_pad(int32_t n_ctx) { return (n_ctx + 3) & ~3; }
-// per workgroup: score matrix, presence vector, context tables, query
+// per workgroup: score matrix, context tables, query
 static inline size_t hb_scan_fixed_lds(int32_t qlen_pad, int32_t ctx_pad)
 {
-    return 1024 + 4 * HB_PV_WORDS + 12 * (size_t)ctx_pad + qlen_pad;
+    return 1024 + 12 * (size_t)ctx_pad + qlen_pad;
 }
 
 // Kernel launchers (hb_kernels.hip).

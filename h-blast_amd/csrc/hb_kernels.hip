@@ -3,20 +3,23 @@
 // Scan kernel (hb_scan_kernel): the BLASTP/BLASTX word scan, two-hit
 // diagonal tracking and X-drop ungapped extension of
 // h_blast_v1.1/H-BLAST/h_blast_kernel.cu:21-1540, restructured for CDNA4:
-//   * one wavefront owns one subject at a time; subjects come from a
-//     length-descending tile queue (one atomic per tile);
-//   * lane i owns subject position s0+i of a 64-position window; the 3-mer
-//     index is tested against the presence vector in LDS and the cell
-//     {start, count} fetched from the lookup table (L2 resident);
-//   * the window's hits are flattened in (s, q) order into an LDS list by a
-//     wavefront prefix sum, then processed 64 at a time;
-//   * the per-diagonal state {last_hit + offset | flag} lives in LDS (int32
-//     per diagonal, no 15-bit limit).  Hits of one round that fall on the
-//     same diagonal are ordered by an LDS atomic-max marker so that every
-//     diagonal sees its hits in subject order -- the exact sequential
-//     semantics of the CPU two-hit word finder (the reference kernel's
-//     deferral, h_blast_kernel.cu:873-942, is not reproduced);
-//   * X-drop extensions read the query and the score matrix from LDS.
+//   * one wavefront owns one subject at a time (staged into LDS when it
+//     fits); subjects come from a length-descending tile queue;
+//   * lane i owns subject position s0+i of a 64-position window and fetches
+//     the cell {start, count} of its 3-mer from the lookup table (L2
+//     resident); a wavefront prefix sum flattens the window's hits in
+//     (s, q) order;
+//   * hits are decided 64 per round in one pass: the per-diagonal state
+//     {last_hit + offset | flag | pending} lives in LDS (int32 per diagonal),
+//     hits of a round on one diagonal are ordered by an LDS atomic-max
+//     marker, and a hit that loses is carried to the next round ahead of
+//     newer hits, so every diagonal sees its hits in subject order -- the
+//     exact sequential semantics of the CPU two-hit word finder (the
+//     reference kernel's deferral, h_blast_kernel.cu:873-942, is not
+//     reproduced);
+//   * extensions are queued and run 64 at a time, one per lane, as
+//     branch-free X-drop walks over LDS data; hits on diagonals whose
+//     extension is pending wait in an ordered list and are replayed after.
 #include <hip/hip_runtime.h>
 
 #include "hb_internal.h"
@@ -326,7 +329,7 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 //   sbuf[scap]    residues of the current subject (if it fits; the host picks
 //                 scap <= HB_SCAP_MAX so that the LDS layout gives the most
 //                 wavefronts per CU)
-// Per-workgroup LDS: score matrix, presence vector, context tables (stride
+// Per-workgroup LDS: score matrix, context tables (stride
 // ctx_pad_max) and query of the group being scanned.
 constexpr int kJobCap = HB_JOBCAP;
 constexpr int kBlkCap = HB_BLKCAP;
@@ -359,7 +362,6 @@ struct WaveScan {
     const uint8_t *qry;
     const int32_t *ctx;   // ctx_off[ctx_pad], xdrop[ctx_pad], cutoff[ctx_pad]
     int32_t ctx_pad;
-    const uint32_t *pv;
     int32_t *state;
     uint32_t *marker;
     uint32_t *seg;
@@ -566,11 +568,12 @@ struct WaveScan {
             uint32_t cnt = 0, cst = 0;
             if (s + 3 <= slen) {
                 const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
-                if ((pv[idx >> 5] >> (idx & 31)) & 1u) {
-                    const uint64_t c2 = cells[idx];
-                    cst = (uint32_t)c2;
-                    cnt = (uint32_t)(c2 >> 32);
-                }
+                // every cell is present (count 0 for words without
+                // neighbours); a presence-bit test first saves no load for
+                // the queries this stage sees and costs an LDS read
+                const uint64_t c2 = cells[idx];
+                cst = (uint32_t)c2;
+                cnt = (uint32_t)(c2 >> 32);
             }
             const uint32_t incl = wave_incl_add(cnt);
             const uint32_t excl = incl - cnt;
@@ -668,9 +671,8 @@ struct WaveScan {
 
 // Copies group g's tables into the workgroup's LDS (all threads).
 __device__ __forceinline__ void load_group(const hb_scan_args &a, const hb_group_desc &G,
-                                           uint32_t *sm_pv, int32_t *sm_ctx, uint8_t *sm_q)
+                                           int32_t *sm_ctx, uint8_t *sm_q)
 {
-    for (int i = threadIdx.x; i < HB_PV_WORDS; i += blockDim.x) sm_pv[i] = G.pv[i];
     for (int i = threadIdx.x; i < G.qlen_pad / 4; i += blockDim.x)
         reinterpret_cast<uint32_t *>(sm_q)[i] = reinterpret_cast<const uint32_t *>(G.query)[i];
     for (int i = threadIdx.x; i < G.n_ctx; i += blockDim.x) {
@@ -706,8 +708,7 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     const int lane = threadIdx.x & (HB_WAVE - 1);
     const int wid = threadIdx.x / HB_WAVE;
     int8_t *sm_mat = reinterpret_cast<int8_t *>(lds);
-    uint32_t *sm_pv = reinterpret_cast<uint32_t *>(lds + 1024);
-    int32_t *sm_ctx = reinterpret_cast<int32_t *>(lds + 1024 + 4 * HB_PV_WORDS);
+    int32_t *sm_ctx = reinterpret_cast<int32_t *>(lds + 1024);
     uint8_t *sm_q = reinterpret_cast<uint8_t *>(sm_ctx + 3 * a.ctx_pad_max);
     unsigned char *wb = sm_q + a.qlen_pad_max + (size_t)wid * a.wave_lds;
     const int32_t Dm = a.diag_len_max;
@@ -719,7 +720,6 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     w.a = &a;
     w.mat = sm_mat;
     w.qry = sm_q;
-    w.pv = sm_pv;
     w.ctx = sm_ctx;
     w.ctx_pad = a.ctx_pad_max;
     w.state = reinterpret_cast<int32_t *>(wb);
@@ -739,7 +739,7 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
 
     if (!MULTI) {
         const hb_group_desc G = a.groups[0];
-        load_group(a, G, sm_pv, sm_ctx, sm_q);
+        load_group(a, G, sm_ctx, sm_q);
         bind_group(w, G);
         w.reset();
         __syncthreads();
@@ -769,7 +769,7 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
             if (g != cur) {
                 if (cur != 0xffffffffu) w.flush_totals(cur);
                 const hb_group_desc G = a.groups[g];
-                load_group(a, G, sm_pv, sm_ctx, sm_q);
+                load_group(a, G, sm_ctx, sm_q);
                 bind_group(w, G);
                 w.reset();
                 cur = g;

@@ -298,10 +298,12 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
 }
 
 // Picks wavefronts per workgroup and the staged-subject capacity: the most
-// wavefronts per CU first (the scan is latency-bound: LDS round trips and
-// dependent lane operations), then the largest staging buffer (subjects
-// above it read their residues from global memory), then more wavefronts
-// per workgroup (they share the query tables).
+// wavefronts per CU up to 16 first (measured on C1: 8 -> 12 -> 16 wavefronts
+// per CU gave 27.7 -> 20.6 -> 16.8 ms; beyond 16 the kernel is bound by
+// VALU issue and more wavefronts gained nothing), then the largest staging
+// buffer (subjects above it read their residues from global memory; 18
+// wavefronts with 320-byte staging ran 8% slower than 16 with 1024), then
+// more wavefronts per workgroup (they share the query tables).
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg)
 {
@@ -313,7 +315,7 @@ int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_
             if (lds > c->max_lds) continue;
             int per_cu = 0;
             HB_CHECK(hb_scan_occupancy(multi, lds, waves * HB_WAVE, &per_cu));
-            const int wpc = per_cu * waves;
+            const int wpc = std::min(per_cu * waves, 16);
             if (wpc > best_wpc || (wpc == best_wpc && scap > cfg->scap)) {
                 best_wpc = wpc;
                 cfg->waves = waves;
