@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--queries", type=int, default=2048, help="c3: number of queries")
     ap.add_argument("--plain-sample", type=int, default=48,
                     help="c3: queries timed one by one through the plain path")
-    ap.add_argument("--qps", type=float, default=2.5, help="c4: Poisson arrival rate")
+    ap.add_argument("--qps", type=float, default=4.0, help="c4: Poisson arrival rate")
     ap.add_argument("--duration", type=float, default=40.0,
                     help="c4: seconds of arrivals")
     ap.add_argument("--serve-subjects", type=int, default=20_000_000,
