@@ -9,7 +9,11 @@ resident in HBM).  Word 3, T 11, window 40, X-drop 16, cutoff 41.
 
 One step = one pass of the stage over the whole database for the query
 (hb_scan_start + hb_scan_wait: lookup, two-hit tracking, extensions, HSP
-records and per-subject counters in HBM).  Prints one JSON line.
+records and per-subject counters in HBM).  The same line carries a compact
+batched-vs-plain and served measurement on the same database ("forms";
+--no-forms skips it).  --config c3 / c4 run the full batched (2,048
+queries) and served (20,000,000 subjects) workloads instead.  Prints one
+JSON line.
 """
 import argparse
 import json
@@ -77,6 +81,143 @@ def cpu_baseline(q, res, offs, seconds_target=10.0):
                       f"{dt:.2f} s wall on {threads} threads"}
 
 
+def measure_batched(ctx, db, qs, steps, warmup, plain_sample, group_residues=0):
+    """All queries in one batched scan (groups of group_residues, 0 =
+    autotuned; hipGraph replay), beside the plain per-query loop (prepare +
+    scan + results per query) on the first plain_sample queries.  One step
+    = one batched pass over the database for all queries."""
+    b = hbseed.Batch(ctx, db, group_residues=group_residues)
+    for q in qs:
+        b.add(hbseed.blastp_query(q))
+    t0 = time.perf_counter()
+    b.run()    # builds groups + lookup tables (autotuned), captures the graph
+    first_ms = (time.perf_counter() - t0) * 1e3
+    log(f"batched first run {first_ms:.0f} ms: {b.info()}")
+    for _ in range(max(0, warmup - 1)):
+        b.run()
+    ctx.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+        kms.append(b.info()["kernel_ms"])
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    info = b.info()
+    b.close()
+    sample = qs[:plain_sample]
+    for q in sample[:2]:
+        hbseed.search(ctx, db, hbseed.blastp_query(q))
+    ctx.synchronize()
+    tp = time.perf_counter()
+    for q in sample:
+        hbseed.search(ctx, db, hbseed.blastp_query(q))
+    ctx.synchronize()
+    plain_qps = len(sample) / (time.perf_counter() - tp)
+    qps = len(qs) * steps / wall
+    return {"queries": len(qs), "query_residues": int(sum(len(q) for q in qs)),
+            "queries_per_s": qps, "plain_queries_per_s": plain_qps,
+            "plain_sample": len(sample), "speedup_vs_plain": qps / plain_qps,
+            "ms_per_step": wall / steps * 1e3, "kernel_ms": float(np.mean(kms)),
+            "first_run_ms": first_ms, "build_ms": info["build_ms"],
+            "group_residues": info["group_residues"], "groups": info["n_groups"],
+            "launches": info["n_classes"], "hsps": info["hsps"],
+            "lookup_hits": info["lookup_hits"], "extensions": info["extensions"]}
+
+
+def measure_served(ctx, db, specs, arrivals, max_batch_residues, max_inflight, check=3):
+    """Submits specs[i] at time arrivals[i] (seconds) from a client thread to
+    a server over the resident database, waits for every request, and
+    returns throughput and latency (submit -> results ready, from the
+    server's own clock).  `check` sampled results are compared with the
+    plain per-query path."""
+    import threading
+    n = len(specs)
+    srv = hbseed.Server(ctx, db, max_batch_residues=max_batch_residues,
+                        max_inflight=max_inflight)
+    # warm-up request (first launch, code-object load), not counted
+    t = srv.submit(hbseed.blastp_query(hbseed.random_protein(1499, 300)))
+    srv.wait(t)
+    srv.results(t)
+    st0 = srv.stats()
+    tickets = [None] * n
+    done_t = [0.0] * n
+    n_hsp = [0] * n
+    sample = set(np.linspace(0, n - 1, min(check, n)).astype(int).tolist())
+    kept = {}
+    t0 = time.perf_counter()
+
+    def client():
+        for i in range(n):
+            d = t0 + arrivals[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(specs[i][1])
+
+    th = threading.Thread(target=client)
+    th.start()
+    last_log = time.perf_counter()
+    for i in range(n):
+        while tickets[i] is None:
+            time.sleep(0.001)
+        srv.wait(tickets[i])
+        done_t[i] = time.perf_counter()
+        h = srv.results(tickets[i])
+        n_hsp[i] = len(h)
+        if i in sample:
+            kept[i] = h
+        if time.perf_counter() - last_log > 20:
+            log(f"served {i + 1}/{n}")
+            last_log = time.perf_counter()
+    th.join()
+    wall = max(done_t) - t0
+    st = srv.stats()
+    srv.stop()
+    ok = True
+    for i, h in kept.items():
+        _, ref = hbseed.search(ctx, db, specs[i][1])
+        ok = ok and np.array_equal(h.view(np.int32), ref.view(np.int32))
+    log(f"served results equal plain path on {len(kept)} sampled queries: {ok}")
+    n_done = st["completed"] - st0["completed"]
+    return {"arrivals": n, "blastx": sum(1 for k, _ in specs if k == "blastx"),
+            "completed": n_done, "queries_per_s": n_done / wall, "wall_s": wall,
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "max_ms": st["max_ms"],
+            "query_residues_per_s": (st["query_residues"] - st0["query_residues"]) / wall,
+            "batches": st["batches"] - st0["batches"], "peak_inflight": st["peak_inflight"],
+            "reruns": st["reruns"], "gpu_ms": st["gpu_ms"] - st0["gpu_ms"],
+            "hsps": int(sum(n_hsp)), "sample_equal_plain": ok}
+
+
+def forms_on_c1(ctx, db, n_batch=256, plain_sample=32, served_s=10.0):
+    """Batched vs plain on the first 256 queries of the batched workload
+    (768-residue groups, the size the autotuner picks for it), then a
+    served stream (Poisson, 80% BLASTP / 20% BLASTX) at 60% of the batched
+    rate in query residues for served_s seconds, against the
+    2,000,000-subject database."""
+    qs = workload_c3(n_batch)
+    bat = measure_batched(ctx, db, qs, steps=2, warmup=1, plain_sample=plain_sample,
+                          group_residues=768)
+    bat["workload"] = (f"batched BLASTP: first {n_batch} queries of the 2,048-query workload "
+                       "vs the same 2,000,000 subjects")
+    # offered load: 60% of the batched rate in query residues, for the served
+    # mix's mean query length (six-frame BLASTX queries are longer)
+    gaps, specs = workload_c4(4000)
+    mean_res = float(np.mean([len(sp.seq) for _, sp in specs]))
+    qps = 0.6 * bat["queries_per_s"] * (bat["query_residues"] / bat["queries"]) / mean_res
+    n_arr = min(len(specs), int(qps * served_s * 1.5) + 20)
+    gaps, specs = gaps[:n_arr], specs[:n_arr]
+    arrivals = np.cumsum(gaps / qps)
+    n_sub = max(1, int(np.searchsorted(arrivals, served_s)))
+    srv = measure_served(ctx, db, specs[:n_sub], arrivals[:n_sub], max_batch_residues=16384,
+                         max_inflight=2)
+    srv["workload"] = (f"served stream: Poisson arrivals at {qps:.1f} q/s (60% of the batched "
+                       f"rate in query residues) for {served_s:.0f} s, 80% BLASTP / 20% BLASTX "
+                       "(600-3000 nt) vs the same 2,000,000 subjects")
+    srv["offered_qps"] = qps
+    return {"batched": bat, "served": srv}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +241,8 @@ def main():
     ap.add_argument("--max-batch-residues", type=int, default=16384,
                     help="c4: query residues per served batch")
     ap.add_argument("--max-inflight", type=int, default=2, help="c4: batches in flight")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="c1: skip the compact batched and served measurements")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; --gpus must be 1 "
@@ -168,6 +311,14 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(q, res, offs)
 
+    # compact batched and served measurements on the same resident database
+    forms = None
+    if not args.no_forms:
+        try:
+            forms = forms_on_c1(ctx, db)
+        except Exception as e:   # reported in the line; the plain figure stands
+            forms = {"error": repr(e)}
+
     out = {
         "metric": "subject residues/s",
         "value": n_res * args.steps / wall,
@@ -199,6 +350,8 @@ def main():
     }
     if copy:
         out["copy_calibration"] = copy
+    if forms:
+        out["forms"] = forms
     print(json.dumps(out), flush=True)
     query.close()
     db.close()
@@ -208,8 +361,7 @@ def main():
 def bench_c3(args):
     """Batched BLASTP: all queries in one batched scan (autotuned groups,
     hipGraph replay) against the C1 database, beside the plain per-query
-    loop (prepare + scan + results per query) on a sample of the same
-    queries.  One step = one batched pass over the database for all queries."""
+    loop on a sample of the same queries."""
     log("generating workload")
     _, res, offs = workload_c1(args.subjects)
     qs = workload_c3(args.queries)
@@ -218,66 +370,33 @@ def bench_c3(args):
     db = hbseed.Database(ctx, res, offs)
     log(f"{len(offs) - 1} subjects resident; {len(qs)} queries, "
         f"{sum(len(q) for q in qs)} query residues")
-    b = hbseed.Batch(ctx, db, group_residues=0)
-    for q in qs:
-        b.add(hbseed.blastp_query(q))
-    t0 = time.perf_counter()
-    b.run()    # builds groups + lookup tables (autotuned), captures the graph
-    first_ms = (time.perf_counter() - t0) * 1e3
-    info = b.info()
-    log(f"first run {first_ms:.0f} ms: {info}")
-    for _ in range(max(0, args.warmup - 1)):
-        b.run()
-    ctx.synchronize()
-    kms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        b.run()
-        kms.append(b.info()["kernel_ms"])
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    info = b.info()
-    b.close()
-    # plain per-query loop on a sample (same queries, one at a time)
-    sample = qs[:args.plain_sample]
-    for q in sample[:2]:
-        hbseed.search(ctx, db, hbseed.blastp_query(q))
-    ctx.synchronize()
-    tp = time.perf_counter()
-    for q in sample:
-        hbseed.search(ctx, db, hbseed.blastp_query(q))
-    ctx.synchronize()
-    plain_wall = time.perf_counter() - tp
-    plain_qps = len(sample) / plain_wall
-    qps = len(qs) * args.steps / wall
-    kernel_ms = float(np.mean(kms))
-    abytes = n_res + 24 * (len(offs) - 1) + 32 * info["hsps"]
+    m = measure_batched(ctx, db, qs, args.steps, args.warmup, args.plain_sample)
+    abytes = n_res + 24 * (len(offs) - 1) + 32 * m["hsps"]
     out = {
         "metric": "queries/s (batched vs plain)",
-        "value": qps,
+        "value": m["queries_per_s"],
         "unit": "queries/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int32", "data": "synthetic",
         "config": {"workload": f"batched BLASTP: {len(qs)} queries (log-normal median 350, "
                                "sigma 0.7, clipped 50-3000) vs 2,000,000 subjects",
-                   "queries": len(qs), "query_residues": int(sum(len(q) for q in qs)),
+                   "queries": len(qs), "query_residues": m["query_residues"],
                    "subjects": len(offs) - 1, "residues": n_res,
-                   "group_residues": info["group_residues"], "groups": info["n_groups"],
-                   "launches": info["n_classes"], "seeds": {"db": 1100, "queries": 1300}},
-        "effective_subject_residues_per_s": qps * n_res,
-        "plain_queries_per_s": plain_qps,
-        "plain_sample": len(sample),
-        "speedup_vs_plain": qps / plain_qps,
-        "kernel_ms": kernel_ms,
-        "first_run_ms": first_ms,
-        "build_ms": info["build_ms"],
-        "hsps": info["hsps"], "lookup_hits": info["lookup_hits"],
-        "extensions": info["extensions"],
-        "roofline": {"bound": "hbm", "achieved": abytes / (kernel_ms * 1e-3) / 1e9,
+                   "group_residues": m["group_residues"], "groups": m["groups"],
+                   "launches": m["launches"], "seeds": {"db": 1100, "queries": 1300}},
+        "effective_subject_residues_per_s": m["queries_per_s"] * n_res,
+        "plain_queries_per_s": m["plain_queries_per_s"],
+        "plain_sample": m["plain_sample"],
+        "speedup_vs_plain": m["speedup_vs_plain"],
+        "kernel_ms": m["kernel_ms"],
+        "first_run_ms": m["first_run_ms"],
+        "build_ms": m["build_ms"],
+        "hsps": m["hsps"], "lookup_hits": m["lookup_hits"], "extensions": m["extensions"],
+        "roofline": {"bound": "hbm", "achieved": abytes / (m["kernel_ms"] * 1e-3) / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": abytes / (kernel_ms * 1e-3) / HBM_PEAK, "traffic": None},
+                     "frac": abytes / (m["kernel_ms"] * 1e-3) / HBM_PEAK, "traffic": None},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -302,7 +421,7 @@ def workload_c4(n, seed=1400):
             specs.append(("blastp", hbseed.blastp_query(
                 hbseed.random_protein(seed * 100003 + i, int(lens[i])))))
         else:
-            seg_len = int(min(100, nts[i] // 6 - 2))
+            seg_len = int(min(100, nts[i] // 7))
             a = int(rng.integers(0, 1000 - seg_len))
             seg = hbseed.homolog(seed * 7 + i, src[a:a + seg_len], 0.7, 1)
             nt = hbseed.blastx_nt(seed * 13 + i, int(nts[i]), seg, seg[::-1].copy())
@@ -317,7 +436,6 @@ def bench_c4(args):
     queries/s, subject residues/s (queries x database residues / wall) and
     p50/p99 latency from submit to results ready; a sample of the served
     results is checked against the plain per-query path."""
-    import threading
     q1 = hbseed.random_protein(1101, 1000)
     log(f"generating {args.serve_subjects} subjects")
     res, offs = hbseed.make_db(1400, args.serve_subjects, plant_query=q1, plant_frac=0.05)
@@ -325,90 +443,40 @@ def bench_c4(args):
     n_arr = max(1, int(args.qps * args.duration * 1.5) + 20)
     gaps, specs = workload_c4(n_arr)
     arrivals = np.cumsum(gaps / args.qps)
-    n_sub = int(np.searchsorted(arrivals, args.duration))
-    n_sub = max(n_sub, 1)
+    n_sub = max(1, int(np.searchsorted(arrivals, args.duration)))
     specs, arrivals = specs[:n_sub], arrivals[:n_sub]
     ctx = hbseed.Context(0)
     db = hbseed.Database(ctx, res, offs)
     del res
     log(f"{len(offs) - 1} subjects, {n_res} residues resident; {n_sub} arrivals over "
         f"{args.duration:.0f} s at {args.qps} q/s")
-    srv = hbseed.Server(ctx, db, max_batch_residues=args.max_batch_residues,
-                        max_inflight=args.max_inflight)
-    # warm-up request (first launch, code-object load), not counted
-    t = srv.submit(hbseed.blastp_query(hbseed.random_protein(1499, 300)))
-    srv.wait(t)
-    srv.results(t)
-    st0 = srv.stats()
-    tickets = [None] * n_sub
-    done_t = [0.0] * n_sub
-    n_hsp = [0] * n_sub
-    sample = {0, n_sub // 2, n_sub - 1}
-    kept = {}
-    t0 = time.perf_counter()
-
-    def client():
-        for i in range(n_sub):
-            d = t0 + arrivals[i] - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            tickets[i] = srv.submit(specs[i][1])
-
-    th = threading.Thread(target=client)
-    th.start()
-    last_log = time.perf_counter()
-    for i in range(n_sub):
-        while tickets[i] is None:
-            time.sleep(0.001)
-        srv.wait(tickets[i])
-        done_t[i] = time.perf_counter()
-        h = srv.results(tickets[i])
-        n_hsp[i] = len(h)
-        if i in sample:
-            kept[i] = h
-        if time.perf_counter() - last_log > 20:
-            log(f"{i + 1}/{n_sub} done")
-            last_log = time.perf_counter()
-    th.join()
-    wall = max(done_t) - t0
-    st = srv.stats()
-    srv.stop()
-    ok = True
-    for i, h in kept.items():
-        _, ref = hbseed.search(ctx, db, specs[i][1])
-        ok = ok and np.array_equal(h.view(np.int32), ref.view(np.int32))
-    log(f"served results equal plain path on {len(kept)} sampled queries: {ok}")
-    n_done = st["completed"] - st0["completed"]
-    qres = st["query_residues"] - st0["query_residues"]
-    gpu_ms = st["gpu_ms"] - st0["gpu_ms"]
-    batches = st["batches"] - st0["batches"]
-    abytes = batches * (n_res + 24 * (len(offs) - 1)) + 32 * sum(n_hsp)
-    n_bx = sum(1 for k, _ in specs if k == "blastx")
+    m = measure_served(ctx, db, specs, arrivals, args.max_batch_residues, args.max_inflight)
+    abytes = m["batches"] * (n_res + 24 * (len(offs) - 1)) + 32 * m["hsps"]
     out = {
         "metric": "queries/s served (p50/p99 latency)",
-        "value": n_done / wall,
+        "value": m["queries_per_s"],
         "unit": "queries/s",
         "n_gpus": 1, "steps": n_sub, "warmup": 1,
-        "ms_per_step": wall / max(n_done, 1) * 1e3,
+        "ms_per_step": m["wall_s"] / max(m["completed"], 1) * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int32", "data": "synthetic",
         "config": {"workload": f"served stream: Poisson arrivals at {args.qps} q/s for "
                                f"{args.duration:.0f} s, 80% BLASTP (log-normal median 350) / "
                                f"20% BLASTX (600-3000 nt) vs {len(offs) - 1} subjects",
                    "subjects": len(offs) - 1, "residues": n_res, "arrivals": n_sub,
-                   "blastx": n_bx, "max_batch_residues": args.max_batch_residues,
+                   "blastx": m["blastx"], "max_batch_residues": args.max_batch_residues,
                    "max_inflight": args.max_inflight, "seeds": {"db": 1400, "stream": 1400}},
-        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
-        "max_ms": st["max_ms"],
-        "effective_subject_residues_per_s": n_done * n_res / wall,
-        "query_residues_per_s": qres / wall,
-        "batches": batches, "peak_inflight": st["peak_inflight"], "reruns": st["reruns"],
-        "gpu_busy_frac": gpu_ms / (wall * 1e3),
-        "hsps": int(sum(n_hsp)),
-        "sample_equal_plain": ok,
-        "roofline": {"bound": "hbm", "achieved": abytes / (gpu_ms * 1e-3) / 1e9,
+        "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "mean_ms": m["mean_ms"],
+        "max_ms": m["max_ms"],
+        "effective_subject_residues_per_s": m["completed"] * n_res / m["wall_s"],
+        "query_residues_per_s": m["query_residues_per_s"],
+        "batches": m["batches"], "peak_inflight": m["peak_inflight"], "reruns": m["reruns"],
+        "gpu_busy_frac": m["gpu_ms"] / (m["wall_s"] * 1e3),
+        "hsps": m["hsps"],
+        "sample_equal_plain": m["sample_equal_plain"],
+        "roofline": {"bound": "hbm", "achieved": abytes / (m["gpu_ms"] * 1e-3) / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": abytes / (gpu_ms * 1e-3) / HBM_PEAK, "traffic": None},
+                     "frac": abytes / (m["gpu_ms"] * 1e-3) / HBM_PEAK, "traffic": None},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

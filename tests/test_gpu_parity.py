@@ -136,3 +136,26 @@ def test_long_concatenated_query(ctx):
     q = hbseed.random_protein(51, 12000)
     res, offs = hbseed.make_db(50, 800, plant_query=q, plant_frac=0.2)
     compare(ctx, hbseed.blastp_query(q), res, offs)
+
+
+def test_subject_with_many_hsps(ctx):
+    """Subjects holding hundreds of homologous copies of query segments:
+    more than 256 HSPs in one subject take the host ordering path of
+    hb_scan_results; the records must still come out in (s_off, q_off)
+    order and equal the oracle."""
+    q = hbseed.random_protein(7101, 400)
+    parts = []
+    for k in range(420):
+        a = (37 * k) % 340
+        seg = hbseed.homolog(7200 + k, q[a:a + 60], 0.9, 0)
+        parts.append(seg)
+        parts.append(hbseed.random_protein(7300 + k, 7))
+    big = np.concatenate(parts).astype(np.uint8)
+    small = hbseed.random_protein(7400, 300)
+    mid = np.concatenate(parts[:200]).astype(np.uint8)
+    res = np.concatenate([small, big, mid, small]).astype(np.uint8)
+    lens = [len(small), len(big), len(mid), len(small)]
+    offs = np.zeros(len(lens) + 1, np.uint64)
+    np.cumsum(lens, out=offs[1:])
+    stats, hsps = compare(ctx, hbseed.blastp_query(q), res, offs)
+    assert stats["n_hsp"][1] > 256 and stats["n_hsp"][2] > 0
