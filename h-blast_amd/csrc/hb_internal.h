@@ -69,14 +69,15 @@ struct hb_scan_args {
 };
 
 // per wave: diagonal state, ordering markers, 64 segment tags, job and
-// blocked queues (8 B entries), staged subject
+// blocked queues (8 B entries), two staged subjects
 // The regions of a workgroup's wavefronts are padded so that their distance
 // modulo 512 bytes is at least 128: distances of 0 or 64 bytes (mod 512)
 // made the C1 scan 8% slower on gfx950 (measured), which points to LDS bank
 // or port conflicts between wavefronts.
 static inline size_t hb_scan_wave_lds(int32_t diag_len, int32_t scap)
 {
-    size_t b = 4 * (size_t)diag_len + 4 * HB_MARK + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + scap;
+    size_t b = 4 * (size_t)diag_len + 4 * HB_MARK + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) +
+               2 * (size_t)scap;
     const size_t r = b % 512;
     if (r < 128) b += 128 - r;
     return b;

@@ -334,6 +334,7 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 constexpr int kJobCap = HB_JOBCAP;
 constexpr int kBlkCap = HB_BLKCAP;
 static_assert((HB_MARK & (HB_MARK - 1)) == 0 && HB_MARK >= HB_WAVE, "HB_MARK: power of two >= 64");
+constexpr uint32_t kDeferJobs = 40;   // a staged subject ending with fewer queued extensions defers them
 constexpr uint32_t kFlag = 0x80000000u;
 constexpr uint32_t kPend = 0x40000000u;
 constexpr uint32_t kVal = 0x3fffffffu;
@@ -367,7 +368,7 @@ struct WaveScan {
     uint32_t *seg;
     uint2 *jobs;
     uint2 *blk;
-    uint8_t *sbuf;
+    uint8_t *sbuf0, *sbuf1;  // two staging buffers (scap bytes each)
     int lane;
     int32_t window;
     int32_t base;      // diagonal offset: stale entries read as "no recent hit"
@@ -375,8 +376,19 @@ struct WaveScan {
     uint32_t rgen;     // segment-tag generation
     uint32_t n_jobs;   // wave-uniform
     uint32_t n_blk;    // wave-uniform
-    uint32_t n_ext;    // per lane
-    uint32_t n_hsp;    // per lane
+    // Subject slots.  The subject being scanned is in slot cs; a staged
+    // subject that ended with few extensions queued may leave them (and its
+    // blocked hits) to run with the next subject's: it then stays in slot
+    // ds until they ran.  Job and blocked-hit records carry their slot in
+    // bit 31 of the subject offset.
+    // (Per-slot values are separate scalars: an indexed member array would
+    // move the whole struct to scratch memory.)
+    int cs;            // slot of the subject being scanned (wave-uniform)
+    int ds;            // slot of the deferred subject, -1 if none
+    uint32_t s_oid0, s_oid1, s_hits0, s_hits1;
+    int32_t s_len0, s_len1, s_base0, s_base1;
+    uint32_t n_ext0, n_ext1;  // per lane
+    uint32_t n_hsp0, n_hsp1;  // per lane
     uint64_t tot_hits, tot_ext, tot_hsp;  // wave-uniform running totals
 
     __device__ __forceinline__ uint32_t diag_of(uint32_t q, int32_t so) const
@@ -409,7 +421,8 @@ struct WaveScan {
     //   diff >= window     -> last = s
     //   diff < 3           -> skip
     //   otherwise          -> extension (diagonal pending until it ran)
-    __device__ __forceinline__ bool decide_pass(bool active, uint32_t q, int32_t so, uint32_t d)
+    __device__ __forceinline__ bool decide_pass(bool active, uint32_t q, int32_t so, uint32_t d,
+                                                bool sl)
     {
         if (++gen == (1u << 26)) {
             for (int i = lane; i < HB_MARK; i += HB_WAVE) marker[i] = 0;
@@ -426,7 +439,7 @@ struct WaveScan {
         const uint32_t ust = (uint32_t)st;
         const bool pend = (ust & kPend) != 0;
         const bool flag = st < 0;
-        const int32_t sb = so + base;
+        const int32_t sb = so + (sl ? s_base1 : s_base0);
         const int32_t diff = (int32_t)((uint32_t)sb - ust);  // s - last if unflagged
         const bool job = win && !pend && !flag && diff >= 3 && diff < window;
         const bool wait = win && pend;
@@ -438,43 +451,56 @@ struct WaveScan {
         const bool lose = active && !win;
         const bool lwait = lose && ((uint32_t)state[d] & kPend) != 0;
         const uint64_t mj = __ballot(job), mb = __ballot(wait || lwait);
+        const uint32_t tso = (uint32_t)so | (sl ? 0x80000000u : 0u);
         if (job) {
             const uint32_t delta = (uint32_t)diff;  // 3..window-1
-            jobs[n_jobs + lane_rank(mj)] = make_uint2(q | (delta << 24), (uint32_t)so);
+            jobs[n_jobs + lane_rank(mj)] = make_uint2(q | (delta << 24), tso);
         }
-        if (wait || lwait) blk[n_blk + lane_rank(mb)] = make_uint2(q, (uint32_t)so);
+        if (wait || lwait) blk[n_blk + lane_rank(mb)] = make_uint2(q, tso);
         n_jobs += (uint32_t)__popcll(mj);
         n_blk += (uint32_t)__popcll(mb);
-        n_ext += job ? 1u : 0u;
+        n_ext0 += job && !sl ? 1u : 0u;
+        n_ext1 += job && sl ? 1u : 0u;
         wave_lds_sync();
         return lose && !lwait;
     }
 
     // Decides a set of hits completely (passes until no hit is left).
-    __device__ __forceinline__ void decide(bool active, uint32_t q, int32_t so, uint32_t d)
+    __device__ __forceinline__ void decide(bool active, uint32_t q, int32_t so, uint32_t d, bool sl)
     {
-        while (__ballot(active)) active = decide_pass(active, q, so, d);
+        while (__ballot(active)) active = decide_pass(active, q, so, d, sl);
     }
 
     // Runs every queued extension (one per lane), writes HSPs with score >=
     // cutoff and the diagonals' new states.
+    // With SL the jobs may belong to either slot (staged subjects); without,
+    // all of them belong to the subject S being scanned (slot cs).
     template <bool SL, typename SP>
-    __device__ __forceinline__ void run_jobs(SP S, int32_t slen, uint32_t oid)
+    __device__ __forceinline__ void run_jobs(SP S)
     {
         for (uint32_t j0 = 0; j0 < n_jobs; j0 += HB_WAVE) {
             const uint32_t j = j0 + (uint32_t)lane;
             const bool v = j < n_jobs;
             ext_result r = {0, 0, 0, 0, 0, 0};
             int32_t k = 0, q = 0, so = 0;
+            bool sl = false;
             if (v) {
                 const uint2 jb = jobs[j];
                 q = (int32_t)(jb.x & 0xffffffu);
-                so = (int32_t)jb.y;
+                so = (int32_t)(jb.y & 0x7fffffffu);
+                sl = (jb.y >> 31) != 0;
                 const int32_t last_s = so - (int32_t)(jb.x >> 24);
+                const int32_t slen = sl ? s_len1 : s_len0;
+                const int32_t b = sl ? s_base1 : s_base0;
                 k = context_of(ctx, n_ctx, q);
-                r = ext_two_hit<SL>(mat, S, slen, qry, qlen, last_s + 3, so, q, ctx[ctx_pad + k]);
+                if (SL)
+                    r = ext_two_hit<true>(mat, sl ? sbuf1 : sbuf0, slen, qry, qlen, last_s + 3,
+                                          so, q, ctx[ctx_pad + k]);
+                else
+                    r = ext_two_hit<false>(mat, S, slen, qry, qlen, last_s + 3, so, q,
+                                           ctx[ctx_pad + k]);
                 state[diag_of((uint32_t)q, so)] =
-                    r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + base)) : so + base;
+                    r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + b)) : so + b;
             }
             const bool emit = v && r.score >= ctx[2 * ctx_pad + k];
             const uint64_t m = __ballot(emit);
@@ -486,7 +512,7 @@ struct WaveScan {
                     const uint32_t slot = slot0 + lane_rank(m);
                     if (slot < a->out_cap) {
                         hb_hsp o;
-                        o.oid = oid;
+                        o.oid = sl ? s_oid1 : s_oid0;
                         o.context = ctx_base + k;
                         o.q_start = r.q_start;
                         o.s_start = r.s_start;
@@ -498,7 +524,8 @@ struct WaveScan {
                     }
                 }
             }
-            n_hsp += emit ? 1u : 0u;
+            n_hsp0 += emit && !sl ? 1u : 0u;
+            n_hsp1 += emit && sl ? 1u : 0u;
         }
         n_jobs = 0;
         wave_lds_sync();
@@ -516,17 +543,46 @@ struct WaveScan {
             const bool v = b < nb;
             const uint2 e = v ? blk[b] : make_uint2(0, 0);
             wave_lds_sync();
-            decide(v, e.x, (int32_t)e.y, diag_of(e.x, (int32_t)e.y));
+            const int32_t so = (int32_t)(e.y & 0x7fffffffu);
+            decide(v, e.x, so, diag_of(e.x, so), (e.y >> 31) != 0);
         }
     }
 
+    // Runs every queued extension and replays the blocked hits until none is
+    // left; a deferred subject is then complete.
     template <bool SL, typename SP>
-    __device__ __forceinline__ void drain(SP S, int32_t slen, uint32_t oid)
+    __device__ __forceinline__ void drain(SP S)
     {
         while (n_jobs) {
-            run_jobs<SL>(S, slen, oid);
+            run_jobs<SL>(S);
             replay();
         }
+        if (ds >= 0) {
+            retire(ds);
+            ds = -1;
+        }
+    }
+
+    // Writes a finished subject's counters.
+    __device__ __forceinline__ void retire(int sl)
+    {
+        const uint32_t e = wave_sum(sl ? n_ext1 : n_ext0), hs = wave_sum(sl ? n_hsp1 : n_hsp0);
+        const uint32_t oid = sl ? s_oid1 : s_oid0, hits = sl ? s_hits1 : s_hits0;
+        if (a->stats && lane == 0) {
+            a->stats[3 * (size_t)oid + 0] = hits;
+            a->stats[3 * (size_t)oid + 1] = e;
+            a->stats[3 * (size_t)oid + 2] = hs;
+        }
+        tot_hits += hits;
+        tot_ext += e;
+        tot_hsp += hs;
+    }
+
+    // Completes a deferred subject (before its slot, the diagonal state or
+    // the group tables change).
+    __device__ __forceinline__ void finish_deferred()
+    {
+        if (ds >= 0) drain<true>(sbuf0);
     }
 
     // Moves the hits that lost a pass (lost = true) to lanes 0..n-1 in lane
@@ -557,12 +613,24 @@ struct WaveScan {
     __device__ __forceinline__ void subject(SP S, int32_t slen, uint32_t oid)
     {
         uint32_t hits = 0;
-        n_ext = 0;
-        n_hsp = 0;
+        const bool csl = cs != 0;
+        if (csl) {
+            s_oid1 = oid;
+            s_len1 = slen;
+            s_base1 = base;
+            n_ext1 = 0;
+            n_hsp1 = 0;
+        } else {
+            s_oid0 = oid;
+            s_len0 = slen;
+            s_base0 = base;
+            n_ext0 = 0;
+            n_hsp0 = 0;
+        }
         int32_t s0m = 0;  // s0 mod D
         uint32_t c = 0;   // carried hits, in lanes 0..c-1
         uint32_t cq = 0, cd = 0;
-        int32_t cs = 0;
+        int32_t cs_ = 0;
         for (int32_t s0 = 0; s0 + 3 <= slen; s0 += HB_WAVE) {
             const int32_t s = s0 + lane;
             uint32_t cnt = 0, cst = 0;
@@ -582,7 +650,7 @@ struct WaveScan {
             hits += H;
             uint32_t carry = 0;
             for (uint32_t r0 = 0; r0 < H;) {
-                if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain<SL>(S, slen, oid);
+                if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain<SL>(S);
                 // new hits r0 .. r0 + take - 1 go to lanes c .. c + take - 1
                 const uint32_t take = min((uint32_t)HB_WAVE - c, H - r0);
                 if (++rgen == (1u << 25)) {
@@ -607,33 +675,34 @@ struct WaveScan {
                 if (dn < 0) dn += D;
                 const bool isc = (uint32_t)lane < c;
                 const uint32_t q = isc ? cq : qn;
-                const int32_t so = isc ? cs : s0 + (int32_t)p;
+                const int32_t so = isc ? cs_ : s0 + (int32_t)p;
                 const uint32_t d = isc ? cd : (uint32_t)dn;
                 wave_lds_sync();
-                const bool lost = decide_pass(isc || isnew, q, so, d);
-                c = carry_losers(lost, q, so, d, cq, cs, cd);
+                const bool lost = decide_pass(isc || isnew, q, so, d, csl);
+                c = carry_losers(lost, q, so, d, cq, cs_, cd);
                 r0 += take;
             }
             s0m += HB_WAVE;
             if (s0m >= D) s0m -= D;
         }
         while (c) {   // hits still carried at the end of the subject
-            if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain<SL>(S, slen, oid);
+            if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain<SL>(S);
             const bool isc = (uint32_t)lane < c;
-            const bool lost = decide_pass(isc, cq, cs, cd);
-            c = carry_losers(lost, cq, cs, cd, cq, cs, cd);
+            const bool lost = decide_pass(isc, cq, cs_, cd, csl);
+            c = carry_losers(lost, cq, cs_, cd, cq, cs_, cd);
         }
-        drain<SL>(S, slen, oid);
-        const uint32_t e = wave_sum(n_ext), hs = wave_sum(n_hsp);
-        if (a->stats && lane == 0) {
-            a->stats[3 * (size_t)oid + 0] = hits;
-            a->stats[3 * (size_t)oid + 1] = e;
-            a->stats[3 * (size_t)oid + 2] = hs;
+        if (csl) s_hits1 = hits;
+        else s_hits0 = hits;
+        const bool reset = base + slen + window >= (1 << 29);
+        if (SL && ds < 0 && n_jobs < kDeferJobs && !reset) {
+            // few extensions queued: they run with the next subject's
+            ds = cs;
+            cs ^= 1;
+        } else {
+            drain<SL>(S);   // also completes a deferred subject
+            retire(cs);
         }
-        tot_hits += hits;
-        tot_ext += e;
-        tot_hsp += hs;
-        if (base + slen + window >= (1 << 29)) {
+        if (reset) {
             for (int i = lane; i < D; i += HB_WAVE) state[i] = 0;
             wave_lds_sync();
             base = window;
@@ -649,11 +718,13 @@ struct WaveScan {
         const int32_t slen = (int32_t)(a->offs[oid + 1] - sb);
         const uint8_t *S = a->res + sb;
         if ((uint32_t)slen <= a->scap) {
-            for (int i = lane; i < slen; i += HB_WAVE) sbuf[i] = S[i];
+            uint8_t *B = cs ? sbuf1 : sbuf0;
+            for (int i = lane; i < slen; i += HB_WAVE) B[i] = S[i];
             wave_lds_sync();
-            subject<true>(sbuf, slen, oid);
+            subject<true>(B, slen, oid);
             wave_lds_sync();
         } else {
+            finish_deferred();
             subject<false>(S, slen, oid);
         }
     }
@@ -727,7 +798,10 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     w.seg = w.marker + HB_MARK;
     w.jobs = reinterpret_cast<uint2 *>(w.seg + HB_WAVE);
     w.blk = w.jobs + kJobCap;
-    w.sbuf = reinterpret_cast<uint8_t *>(w.blk + kBlkCap);
+    w.sbuf0 = reinterpret_cast<uint8_t *>(w.blk + kBlkCap);
+    w.sbuf1 = w.sbuf0 + a.scap;
+    w.cs = 0;
+    w.ds = -1;
     w.lane = lane;
     w.window = a.window;
     w.rgen = 0;
@@ -752,6 +826,7 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
             const uint32_t i_end = a.tiles[t + 1];
             for (uint32_t it = a.tiles[t]; it < i_end; ++it) w.scan_subject(it);
         }
+        w.finish_deferred();
         w.flush_totals(0);
     } else {
         uint32_t cur = 0xffffffffu;
@@ -767,7 +842,11 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
             const uint32_t t0 = a.tile_begin + (item % n_chunks) * tpi;
             const uint32_t t1 = min(t0 + tpi, a.tile_end);
             if (g != cur) {
-                if (cur != 0xffffffffu) w.flush_totals(cur);
+                if (cur != 0xffffffffu) {
+                    w.finish_deferred();   // still with the old group's tables
+                    w.flush_totals(cur);
+                }
+                __syncthreads();
                 const hb_group_desc G = a.groups[g];
                 load_group(a, G, sm_ctx, sm_q);
                 bind_group(w, G);
@@ -790,7 +869,10 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
             }
             __syncthreads();
         }
-        if (cur != 0xffffffffu) w.flush_totals(cur);
+        if (cur != 0xffffffffu) {
+            w.finish_deferred();
+            w.flush_totals(cur);
+        }
     }
 }
 
