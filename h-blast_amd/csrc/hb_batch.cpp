@@ -139,6 +139,18 @@ static void make_groups(hb_batch *b, int32_t target, std::vector<hb_concat> *gro
     }
 }
 
+// Sizes work items for a pass over n_tiles tiles: about 64 items per
+// workgroup for balance, few enough for the queue atomic.
+static void size_items(hb_batch *b, uint32_t n_tiles)
+{
+    for (Class &c : b->classes) {
+        const uint32_t n_groups = (uint32_t)(c.g_end - c.g_begin);
+        const uint64_t want = (uint64_t)c.cfg.blocks * 64;
+        const uint64_t per_group = std::max<uint64_t>(1, want / std::max<uint32_t>(n_groups, 1));
+        c.tiles_per_item = (uint32_t)std::max<uint64_t>(1, (n_tiles + per_group - 1) / per_group);
+    }
+}
+
 // Splits the (diagonal-sorted) groups into classes: a new class starts when
 // the diagonal array would grow past twice the class's first one, so that
 // one launch's LDS layout does not waste occupancy on smaller groups.
@@ -170,11 +182,8 @@ static int build(hb_batch *b, int32_t target)
         rc = hb_scan_config(b->ctx, true, c.qlen_pad_max, c.ctx_pad_max, c.diag_len_max,
                             n_groups * b->db->n_tiles, &c.cfg);
         if (rc) return rc;
-        // about 64 items per workgroup for balance, few enough for the queue atomic
-        const uint64_t want = (uint64_t)c.cfg.blocks * 64;
-        const uint64_t per_group = std::max<uint64_t>(1, want / std::max<uint32_t>(n_groups, 1));
-        c.tiles_per_item = (uint32_t)std::max<uint64_t>(1, (b->db->n_tiles + per_group - 1) / per_group);
     }
+    size_items(b, b->db->n_tiles);
     b->d_misc = static_cast<uint32_t *>(hb_pool_get(b->ctx, (1 + b->classes.size()) * sizeof(uint32_t)));
     b->d_totals = static_cast<unsigned long long *>(
         hb_pool_get(b->ctx, 3 * (size_t)ng * sizeof(unsigned long long)));
@@ -311,6 +320,8 @@ static int autotune(hb_batch *b, int32_t *best)
         int rc = build(b, cand);
         if (rc == HB_ERR_UNSUPPORTED) continue;
         if (rc) return rc;
+        // items sized for the slice, so that the timing is not a tail effect
+        size_items(b, span);
         uint32_t n = 0;
         rc = run_pass(b, t0, t0 + span, false, &n);   // warm
         if (!rc) rc = run_pass(b, t0, t0 + span, false, &n);
