@@ -345,7 +345,11 @@ int hb_serve_results(hb_server *s, uint64_t ticket, hb_hsp *hsps, uint64_t cap, 
     if (it == s->live.end()) return HB_ERR_ARG;
     Request &r = *it->second;
     if (r.state != 2) return HB_ERR_STATE;
-    if (r.rc != HB_OK) return r.rc;
+    if (r.rc != HB_OK) {   // a failed request is released with its error
+        const int rc = r.rc;
+        s->live.erase(it);
+        return rc;
+    }
     if (n_hsp) *n_hsp = r.hsps.size();
     if (!hsps) return HB_OK;
     if (cap < r.hsps.size()) return HB_ERR_CAPACITY;

@@ -270,7 +270,8 @@ int hb_serve_submit(hb_server *s, const uint8_t *seq, int32_t len, int32_t n_ctx
  * results are ready, HB_ERR_BUSY on time-out, the batch's error otherwise. */
 int hb_serve_wait(hb_server *s, uint64_t ticket, int32_t timeout_ms);
 /* Results of a finished request in (oid, s_off, q_off) order.  With hsps
- * NULL only *n_hsp is set; a successful copy releases the ticket. */
+ * NULL only *n_hsp is set; a successful copy releases the ticket, and so
+ * does returning the error of a request that failed. */
 int hb_serve_results(hb_server *s, uint64_t ticket, hb_hsp *hsps, uint64_t cap, uint64_t *n_hsp);
 int hb_serve_stats_get(hb_server *s, hb_serve_stats *st);
 /* Finishes every queued and running request, then stops the loop. */
