@@ -94,6 +94,52 @@ int64_t hbo_scan_db(const hbo_query *q, const hbo_lut *lut,
                     uint32_t *counters);
 void hbo_free(void *p);
 
+/* --- preliminary gapped extension (hbo_gap.c) ---------------------------- */
+typedef struct hbo_gap_params {
+    int32_t gap_open;   /* 11 for BLOSUM62 */
+    int32_t gap_extend; /* 1 */
+    int32_t x_dropoff;  /* raw gapped X-drop (38 = 15 bits with lambda 0.267) */
+} hbo_gap_params;
+
+typedef struct hbo_ghsp {
+    uint32_t oid;
+    int32_t ctx;
+    int32_t q_start, q_end;  /* context-relative query range, end exclusive */
+    int32_t s_start, s_end;  /* subject range, end exclusive */
+    int32_t q_gapped_start;  /* start point of the extension (context-relative) */
+    int32_t s_gapped_start;
+    int32_t score;           /* gapped raw score */
+    int32_t pad;
+} hbo_ghsp;
+
+/* Score-only X-drop DP from the origin of A[1..M] x B[1..N] (reverse = 1:
+ * A[M-1..0] x B[N-1..0]).  Returns the best score; *a_off / *b_off receive the
+ * letters of A and B consumed by the best-scoring alignment. */
+int32_t hbo_semigapped(const int8_t *m, const uint8_t *A, const uint8_t *B, int32_t M,
+                       int32_t N, int reverse, int32_t gap_open, int32_t gap_extend,
+                       int32_t x_dropoff, int32_t *a_off, int32_t *b_off);
+/* Start point of a gapped extension inside an ungapped HSP (query offset). */
+int32_t hbo_gapped_start(const int8_t *m, const uint8_t *q, const uint8_t *s, int32_t q_start,
+                         int32_t q_len, int32_t s_start, int32_t s_len);
+/* Two-sided gapped extension through (q_off, s_off).  coords receives
+ * q_start, q_end, s_start, s_end; returns the score. */
+int32_t hbo_prot_gapped(const int8_t *m, const uint8_t *q, int32_t q_len, const uint8_t *s,
+                        int32_t s_len, int32_t q_off, int32_t s_off,
+                        const hbo_gap_params *gp, int32_t *coords);
+/* One subject's ungapped HSPs (any order) -> its gapped HSPs in save order;
+ * *extensions counts the extensions performed (HSPs not contained). */
+int hbo_gapped_subject(const hbo_query *q, const uint8_t *subj, int32_t slen, const hbo_hsp *in,
+                       int32_t n_in, const hbo_gap_params *gp, const int32_t *gap_cutoff,
+                       hbo_ghsp *out, int32_t *n_out, int32_t *extensions);
+/* Whole database: in is grouped by ascending oid (as hbo_scan_db returns
+ * it); counts receives 2 ints per subject (gapped HSPs saved, extensions).
+ * Returns the number of gapped HSPs, ordered by (oid, save order); *out is
+ * released with hbo_free. */
+int64_t hbo_gapped_db(const hbo_query *q, const uint8_t *residues, const uint64_t *offsets,
+                      uint32_t n_subjects, const hbo_hsp *in, uint64_t n_in,
+                      const hbo_gap_params *gp, const int32_t *gap_cutoff, int n_threads,
+                      hbo_ghsp **out, int32_t *counts);
+
 #define HBO_ERR_ARG (-1)
 #define HBO_ERR_NOMEM (-2)
 #define HBO_ERR_RESIDUE (-3)

@@ -145,3 +145,87 @@ def extend_two_hit(m, s, q, s_left_off, s_right_off, q_right_off, x):
                                   C.byref(hs), C.byref(hl), C.byref(re), C.byref(sl))
     return dict(score=sc, q_start=hq.value, s_start=hs.value, len=hl.value,
                 right=re.value, s_last=sl.value)
+
+
+# ------------------------------------------------------------ gapped extension
+GHSP_DTYPE = np.dtype([("oid", "<u4"), ("context", "<i4"), ("q_start", "<i4"),
+                       ("q_end", "<i4"), ("s_start", "<i4"), ("s_end", "<i4"),
+                       ("q_gstart", "<i4"), ("s_gstart", "<i4"), ("score", "<i4"),
+                       ("pad", "<i4")])
+
+
+class _GapParams(C.Structure):
+    _fields_ = [("gap_open", C.c_int32), ("gap_extend", C.c_int32), ("x_dropoff", C.c_int32)]
+
+
+def _gap_lib():
+    L = lib()
+    if not getattr(L, "_gap_ready", False):
+        i32, pi, vp = C.c_int32, C.POINTER(C.c_int32), C.c_void_p
+        L.hbo_semigapped.restype = i32
+        L.hbo_semigapped.argtypes = [vp, vp, vp, i32, i32, C.c_int, i32, i32, i32, pi, pi]
+        L.hbo_gapped_start.restype = i32
+        L.hbo_gapped_start.argtypes = [vp, vp, vp, i32, i32, i32, i32]
+        L.hbo_prot_gapped.restype = i32
+        L.hbo_prot_gapped.argtypes = [vp, vp, i32, vp, i32, i32, i32, C.POINTER(_GapParams), vp]
+        L.hbo_gapped_db.restype = C.c_int64
+        L.hbo_gapped_db.argtypes = [C.POINTER(_Query), vp, vp, C.c_uint32, vp, C.c_uint64,
+                                    C.POINTER(_GapParams), vp, C.c_int, C.POINTER(vp), vp]
+        L._gap_ready = True
+    return L
+
+
+def semigapped(m, A, B, M, N, reverse, gap_open=11, gap_extend=1, x_dropoff=38):
+    """Score-only X-drop DP: returns (score, a_off, b_off)."""
+    A = np.ascontiguousarray(A, np.uint8)
+    B = np.ascontiguousarray(B, np.uint8)
+    m = np.ascontiguousarray(m, np.int8)
+    a, b = C.c_int32(), C.c_int32()
+    sc = _gap_lib().hbo_semigapped(m.ctypes.data, A.ctypes.data, B.ctypes.data, M, N,
+                                   int(reverse), gap_open, gap_extend, x_dropoff,
+                                   C.byref(a), C.byref(b))
+    return sc, a.value, b.value
+
+
+def gapped_start(m, q, s, q_start, q_len, s_start, s_len):
+    q = np.ascontiguousarray(q, np.uint8)
+    s = np.ascontiguousarray(s, np.uint8)
+    m = np.ascontiguousarray(m, np.int8)
+    return _gap_lib().hbo_gapped_start(m.ctypes.data, q.ctypes.data, s.ctypes.data, q_start,
+                                       q_len, s_start, s_len)
+
+
+def prot_gapped(m, q, s, q_off, s_off, gap_open=11, gap_extend=1, x_dropoff=38):
+    """Two-sided extension: returns (score, q_start, q_end, s_start, s_end)."""
+    q = np.ascontiguousarray(q, np.uint8)
+    s = np.ascontiguousarray(s, np.uint8)
+    m = np.ascontiguousarray(m, np.int8)
+    co = np.zeros(4, np.int32)
+    gp = _GapParams(gap_open, gap_extend, x_dropoff)
+    sc = _gap_lib().hbo_prot_gapped(m.ctypes.data, q.ctypes.data, len(q), s.ctypes.data,
+                                    len(s), q_off, s_off, C.byref(gp), co.ctypes.data)
+    return (sc, *map(int, co))
+
+
+def gapped_db(oq, residues, offsets, hsps, cutoffs, gap_open=11, gap_extend=1, x_dropoff=38,
+              threads=1):
+    """Preliminary gapped extension of every subject's ungapped HSPs (grouped by
+    ascending oid).  Returns (counts[n, 2] = saved, extensions; gapped records)."""
+    residues = np.ascontiguousarray(residues, np.uint8)
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    hsps = np.ascontiguousarray(hsps, HSP_DTYPE)
+    cut = np.ascontiguousarray(cutoffs, np.int32)
+    n = len(offsets) - 1
+    counts = np.zeros(2 * max(n, 1), np.int32)
+    out = C.c_void_p()
+    gp = _GapParams(gap_open, gap_extend, x_dropoff)
+    k = _gap_lib().hbo_gapped_db(C.byref(oq.q), residues.ctypes.data, offsets.ctypes.data, n,
+                                 hsps.ctypes.data, len(hsps), C.byref(gp), cut.ctypes.data,
+                                 threads, C.byref(out), counts.ctypes.data)
+    if k < 0:
+        raise RuntimeError(f"hbo_gapped_db failed: {k}")
+    recs = np.zeros(k, GHSP_DTYPE)
+    if k:
+        C.memmove(recs.ctypes.data, out.value, k * GHSP_DTYPE.itemsize)
+    lib().hbo_free(out)
+    return counts[:2 * n].reshape(n, 2), recs
