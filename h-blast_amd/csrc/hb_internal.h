@@ -130,3 +130,42 @@ struct hb_fin_big {
 hipError_t hb_launch_finalize(const uint32_t *st3, uint32_t ns, const hb_hsp *recs, uint32_t n,
                               hb_subject_stats *stats_out, hb_hsp *recs_out, uint32_t *scratch,
                               hb_fin_big *big, uint32_t big_cap, uint32_t *n_big, hipStream_t st);
+// First-record indices of per-subject counts st3[3i + 2] (hb_kernels.hip):
+// first[i] = sum of the counts of subjects < i.  scratch: ns / 4096 + 1.
+hipError_t hb_launch_first_index(const uint32_t *st3, uint32_t ns, uint32_t *first,
+                                 uint32_t *scratch, hipStream_t st);
+
+// ---------------------------------------------------------------- gapped stage
+// Kernel arguments of the preliminary gapped extension (hb_gapped.hip).
+#define HB_GAP_RING 256       // DP columns per wavefront kept in LDS (power of two)
+#define HB_GAP_ORDER 256      // ungapped HSPs per subject ordered in LDS
+#define HB_GAP_SAVED 64       // saved gapped HSPs per subject kept in LDS
+struct hb_gap_args {
+    const uint8_t *res;            // database residues
+    const uint64_t *offs;          // n + 1 subject offsets
+    const uint8_t *query;          // concatenated query
+    const int32_t *ctx;            // ctx_off[n_ctx] | ctx_len[n_ctx] | cutoff[n_ctx]
+    int32_t n_ctx;
+    const int8_t *mat32;           // 32x32 scores
+    int32_t gap_open, gap_extend, x_dropoff;
+    const hb_subject_stats *ust;   // ungapped per-subject stats (first, n_hsp)
+    const hb_hsp *urec;            // ungapped HSPs grouped by subject
+    uint32_t n_subjects;
+    uint32_t *counter;             // work counter (zeroed before the launch)
+    hb_gapped_hsp *out;            // subject k's gapped HSPs at ust[k].first + i
+    uint32_t *st3;                 // per subject {0, extensions, saved}
+    uint32_t *wide;                // [0] count, then subjects for the wide pass
+    // wide pass: subjects wide[1 .. n_wide], one per wavefront-iteration,
+    // each wavefront with its own global scratch of scratch_words int32:
+    // DP ring 2 * ring_wide, saved list 6 * n_max, order n_max
+    uint32_t n_wide;
+    int32_t *scratch;
+    uint32_t ring_wide;            // power of two > longest subject + 1
+    uint32_t n_max;                // most ungapped HSPs of a wide subject
+    uint32_t scratch_words;
+};
+hipError_t hb_launch_gapped(const hb_gap_args &a, bool wide, int n_blocks, hipStream_t st);
+// Dense layout: stats[k] and out_dense from the sparse output and first[].
+hipError_t hb_launch_gap_compact(const hb_gap_args &a, const uint32_t *first,
+                                 hb_gapped_stats *stats, hb_gapped_hsp *out_dense,
+                                 hipStream_t st);

@@ -967,6 +967,29 @@ __global__ __launch_bounds__(kFinThreads) void fin_write_kernel(const uint32_t *
     }
 }
 
+// First-record index of each subject from its count (no stats written).
+__global__ __launch_bounds__(kFinThreads) void fin_first_kernel(const uint32_t *st3, uint32_t ns,
+                                                                const uint32_t *part,
+                                                                uint32_t *first_out)
+{
+    __shared__ uint32_t sm[kFinThreads / HB_WAVE + 1];
+    const uint32_t i0 = blockIdx.x * kFinSpan + (uint32_t)threadIdx.x * kFinPer;
+    uint32_t cnt[kFinPer];
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kFinPer; ++k) {
+        cnt[k] = i0 + k < ns ? st3[3 * (size_t)(i0 + k) + 2] : 0u;
+        v += cnt[k];
+    }
+    uint32_t total = 0;
+    uint32_t first = part[blockIdx.x] + block_excl_scan(v, sm, &total);
+#pragma unroll
+    for (int k = 0; k < kFinPer; ++k) {
+        if (i0 + k < ns) first_out[i0 + k] = first;
+        first += cnt[k];
+    }
+}
+
 __global__ void fin_scatter_kernel(const hb_hsp *in, uint32_t n, uint32_t *cursor, hb_hsp *out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1050,6 +1073,18 @@ hipError_t hb_launch_finalize(const uint32_t *st3, uint32_t ns, const hb_hsp *re
         hipLaunchKernelGGL(fin_sort_kernel, dim3((ns + 255) / 256), dim3(256), 0, st, stats_out,
                            ns, recs_out, big, big_cap, n_big);
     }
+    return hipGetLastError();
+}
+
+hipError_t hb_launch_first_index(const uint32_t *st3, uint32_t ns, uint32_t *first,
+                                 uint32_t *scratch, hipStream_t st)
+{
+    if (ns == 0) return hipSuccess;
+    const uint32_t nb = (ns + kFinSpan - 1) / kFinSpan;
+    hipLaunchKernelGGL(fin_sum_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, scratch);
+    hipLaunchKernelGGL(fin_part_kernel, dim3(1), dim3(kFinThreads), 0, st, scratch, nb);
+    hipLaunchKernelGGL(fin_first_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, scratch,
+                       first);
     return hipGetLastError();
 }
 

@@ -339,6 +339,7 @@ struct hb_query {
     hb_resolved_params p;
     hb_group_set gs;
     uint32_t n_cells_used = 0;
+    std::vector<int32_t> ctx_off, ctx_len;   // host copies (gapped stage)
 };
 
 struct hb_scan {
@@ -352,6 +353,8 @@ struct hb_scan {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool waited = false;
     uint32_t n_out = 0;
+    float gap_ms = 0.f;        // device time of the last gapped stage
+    uint32_t gap_wide = 0;     // subjects it ran in the wide pass
 };
 
 extern "C" {
@@ -574,6 +577,8 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
         delete q;
         return rc;
     }
+    q->ctx_off.assign(ctx_off, ctx_off + n_ctx);
+    q->ctx_len.assign(ctx_len, ctx_len + n_ctx);
     std::vector<hb_concat> qs(1);
     qs[0].seq.assign(seq, seq + len);
     qs[0].ctx_off.assign(ctx_off, ctx_off + n_ctx);
@@ -838,6 +843,185 @@ int hb_scan_results(hb_scan *s, hb_subject_stats *stats, hb_hsp *hsps, uint64_t 
             if (x.s_off != y.s_off) return x.s_off < y.s_off;
             return x.q_off < y.q_off;
         });
+    return HB_OK;
+}
+
+void hb_gap_params_default(hb_gap_params *p)
+{
+    if (!p) return;
+    p->gap_open = 11;
+    p->gap_extend = 1;
+    p->x_dropoff = HB_DEFAULT_GAP_XDROP;
+    p->reserved = 0;
+}
+
+int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutoff,
+                   hb_gapped_stats *stats, hb_gapped_hsp *hsps, uint64_t cap, uint64_t *n_hsp)
+{
+    if (!s || !gp || !gap_cutoff) return HB_ERR_ARG;
+    if (gp->gap_open < 0 || gp->gap_extend < 0 || gp->x_dropoff < 0 || gp->gap_open > 1000 ||
+        gp->gap_extend > 1000 || gp->x_dropoff > 100000)
+        return HB_ERR_ARG;
+    int rc = hb_scan_wait(s);
+    if (rc) return rc;
+    if (n_hsp) *n_hsp = 0;
+    const uint64_t n = s->n_out;
+    const uint32_t ns = s->db->n;
+    s->gap_ms = 0.f;
+    s->gap_wide = 0;
+    if (ns == 0) return HB_OK;
+    hb_ctx *c = s->ctx;
+    const hb_query *q = s->q;
+    const int32_t n_ctx = (int32_t)q->ctx_off.size();
+    HB_CHECK(hipSetDevice(c->device));
+    // pool buffers, all returned at the end
+    std::vector<std::pair<void *, size_t>> held;
+    bool oom = false;
+    auto get = [&](size_t bytes) -> void * {
+        bytes = std::max<size_t>(bytes, 64);
+        void *p = hb_pool_get(c, bytes);
+        if (!p) oom = true;
+        else held.emplace_back(p, bytes);
+        return p;
+    };
+    constexpr uint32_t kBig = 4096;
+    const uint32_t nb = (ns + 4095) / 4096;
+    const uint64_t nr = std::max<uint64_t>(n, 1);
+    auto *d_ust = static_cast<hb_subject_stats *>(get((size_t)ns * sizeof(hb_subject_stats)));
+    auto *d_urec = static_cast<hb_hsp *>(get(nr * sizeof(hb_hsp)));
+    auto *d_scr = static_cast<uint32_t *>(get(((size_t)ns + nb + 1) * sizeof(uint32_t)));
+    auto *d_big = static_cast<hb_fin_big *>(get(kBig * sizeof(hb_fin_big)));
+    auto *d_st3 = static_cast<uint32_t *>(get((size_t)ns * 3 * sizeof(uint32_t)));
+    auto *d_gout = static_cast<hb_gapped_hsp *>(get(nr * sizeof(hb_gapped_hsp)));
+    auto *d_misc = static_cast<uint32_t *>(get(((size_t)ns + 2) * sizeof(uint32_t)));
+    auto *d_ctx = static_cast<int32_t *>(get((size_t)3 * n_ctx * sizeof(int32_t)));
+    auto *d_first = static_cast<uint32_t *>(get(((size_t)ns + nb + 1) * sizeof(uint32_t)));
+    auto *d_gst = static_cast<hb_gapped_stats *>(get((size_t)ns * sizeof(hb_gapped_stats)));
+    auto *d_gdense = static_cast<hb_gapped_hsp *>(get(nr * sizeof(hb_gapped_hsp)));
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    auto release = [&]() {
+        for (auto &h : held) hb_pool_put(c, h.first, h.second);
+        for (hipEvent_t e : ev)
+            if (e) hipEventDestroy(e);
+    };
+    if (oom) {
+        release();
+        return HB_ERR_NOMEM;
+    }
+    hipStream_t st = c->stream;
+    std::vector<int32_t> hctx(3 * (size_t)n_ctx);
+    for (int32_t i = 0; i < n_ctx; ++i) {
+        hctx[i] = q->ctx_off[i];
+        hctx[n_ctx + i] = q->ctx_len[i];
+        hctx[2 * n_ctx + i] = gap_cutoff[i];
+    }
+    hb_gap_args a{};
+    a.res = s->db->d_res;
+    a.offs = s->db->d_offs;
+    a.query = q->gs.desc[0].query;
+    a.ctx = d_ctx;
+    a.n_ctx = n_ctx;
+    a.mat32 = q->gs.d_mat;
+    a.gap_open = gp->gap_open;
+    a.gap_extend = gp->gap_extend;
+    a.x_dropoff = gp->x_dropoff;
+    a.ust = d_ust;
+    a.urec = d_urec;
+    a.n_subjects = ns;
+    a.counter = d_misc;
+    a.out = d_gout;
+    a.st3 = d_st3;
+    a.wide = d_misc + 1;
+    uint32_t n_wide = 0;
+    uint32_t *d_nbig = d_scr + ns + nb;
+    hipError_t e = hb_launch_fill_u32(d_nbig, 0u, 1, st);
+    if (e == hipSuccess)
+        e = hb_launch_finalize(s->d_stats, ns, s->d_out, (uint32_t)n, d_ust, d_urec, d_scr, d_big,
+                               kBig, d_nbig, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_ctx, hctx.data(), hctx.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hb_launch_fill_u32(d_misc, 0u, 2, st);
+    for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&ev[i]);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], st);
+    if (e == hipSuccess) e = hb_launch_gapped(a, false, c->n_cu * 8, st);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&n_wide, d_misc + 1, sizeof(n_wide),
+                                            hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && n_wide) {
+        // subjects that outgrew the LDS capacities: same code, global scratch
+        std::vector<uint32_t> wl(n_wide);
+        std::vector<hb_subject_stats> ust(ns);
+        e = hipMemcpy(wl.data(), d_misc + 2, n_wide * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(ust.data(), d_ust, (size_t)ns * sizeof(hb_subject_stats),
+                          hipMemcpyDeviceToHost);
+        uint64_t longest = 0;
+        uint32_t n_max = 1;
+        for (uint32_t k : wl) {
+            longest = std::max<uint64_t>(longest, s->db->h_offs[k + 1] - s->db->h_offs[k]);
+            n_max = std::max(n_max, ust[k].n_hsp);
+        }
+        uint32_t ring = 64;
+        while (ring < longest + 2) ring <<= 1;
+        a.ring_wide = ring;
+        a.n_max = n_max;
+        a.scratch_words = (uint32_t)((2 * (uint64_t)ring + 7 * (uint64_t)n_max + 63) & ~63ull);
+        const uint64_t per_wave = 4ull * a.scratch_words;
+        uint64_t waves = std::min<uint64_t>(n_wide, 1024);
+        waves = std::max<uint64_t>(1, std::min<uint64_t>(waves, ((uint64_t)1 << 30) / per_wave));
+        const int blocks = (int)((waves + 3) / 4);
+        a.scratch = static_cast<int32_t *>(get((size_t)blocks * 4 * per_wave));
+        a.n_wide = n_wide;
+        s->gap_wide = n_wide;
+        if (oom) e = hipErrorOutOfMemory;
+        if (e == hipSuccess) e = hb_launch_fill_u32(d_misc, 0u, 1, st);
+        if (e == hipSuccess) e = hipEventRecord(ev[2], st);
+        if (e == hipSuccess) e = hb_launch_gapped(a, true, blocks, st);
+        if (e == hipSuccess) e = hipEventRecord(ev[3], st);
+    }
+    if (e == hipSuccess) e = hb_launch_first_index(d_st3, ns, d_first, d_first + ns, st);
+    if (e == hipSuccess) e = hb_launch_gap_compact(a, d_first, d_gst, d_gdense, st);
+    uint32_t tail[2] = {0, 0};
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&tail[0], d_first + ns - 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&tail[1], d_st3 + 3 * (size_t)(ns - 1) + 2, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    float ms = 0.f, ms2 = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (e == hipSuccess && n_wide) e = hipEventElapsedTime(&ms2, ev[2], ev[3]);
+    s->gap_ms = ms + ms2;
+    const uint64_t total = (uint64_t)tail[0] + tail[1];
+    if (e == hipSuccess) {
+        if (n_hsp) *n_hsp = total;
+        if (hsps && cap < total) {
+            release();
+            return HB_ERR_CAPACITY;
+        }
+        if (stats && hb_copy_out(c, stats, d_gst, (size_t)ns * sizeof(hb_gapped_stats), st) !=
+                         HB_OK)
+            e = hipErrorUnknown;
+        if (e == hipSuccess && hsps && total &&
+            hb_copy_out(c, hsps, d_gdense, total * sizeof(hb_gapped_hsp), st) != HB_OK)
+            e = hipErrorUnknown;
+    }
+    release();
+    if (e != hipSuccess) {
+        hb_note_hip_error(e, "hb_scan_gapped", __LINE__);
+        return e == hipErrorOutOfMemory ? HB_ERR_NOMEM : HB_ERR_DEVICE;
+    }
+    return HB_OK;
+}
+
+int hb_scan_gapped_info(hb_scan *s, float *ms, uint32_t *n_wide)
+{
+    if (!s) return HB_ERR_ARG;
+    if (ms) *ms = s->gap_ms;
+    if (n_wide) *n_wide = s->gap_wide;
     return HB_OK;
 }
 

@@ -35,6 +35,14 @@ HSP_DTYPE = np.dtype([("oid", "<u4"), ("context", "<i4"), ("q_start", "<i4"),
                       ("len", "<i4"), ("score", "<i4")])
 STATS_DTYPE = np.dtype([("lookup_hits", "<u4"), ("extensions", "<u4"), ("n_hsp", "<u4"),
                         ("first", "<i8")], align=True)
+GHSP_DTYPE = np.dtype([("oid", "<u4"), ("context", "<i4"), ("q_start", "<i4"),
+                       ("q_end", "<i4"), ("s_start", "<i4"), ("s_end", "<i4"),
+                       ("q_gstart", "<i4"), ("s_gstart", "<i4"), ("score", "<i4"),
+                       ("pad", "<i4")])
+GSTATS_DTYPE = np.dtype([("extensions", "<u4"), ("n_hsp", "<u4"), ("first", "<i8")])
+DEFAULT_GAP_OPEN = 11
+DEFAULT_GAP_EXTEND = 1
+DEFAULT_GAP_XDROP = 38
 
 # Every symbol include/hb_seed.h declares (checked by the CPU tests).
 EXPORTS = [
@@ -44,6 +52,7 @@ EXPORTS = [
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
+    "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info",
     "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
     "hb_batch_results", "hb_batch_info_get", "hb_batch_free",
     "hb_serve_opts_default", "hb_serve_start", "hb_serve_submit", "hb_serve_wait",
@@ -134,6 +143,10 @@ def _lib():
         L.hb_scan_elapsed_ms.argtypes = [vp, C.POINTER(C.c_float)]
         L.hb_scan_free.argtypes = [vp]
         L.hb_diag_copy.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
+        L.hb_gap_params_default.argtypes = [vp]
+        L.hb_gap_params_default.restype = None
+        L.hb_scan_gapped.argtypes = [vp, vp, vp, vp, vp, u64, C.POINTER(u64)]
+        L.hb_scan_gapped_info.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(u32)]
         L.hb_batch_opts_default.argtypes = [vp]
         L.hb_batch_create.argtypes = [vp, vp, vp, vp, pp]
         L.hb_batch_add.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(i32)]
@@ -347,6 +360,29 @@ class Scan:
         _check(_lib().hb_scan_results(self.h, _ptr(stats), _ptr(hsps), n.value, C.byref(n)),
                "hb_scan_results")
         return stats, hsps
+
+    def gapped(self, cutoffs, gap_open=DEFAULT_GAP_OPEN, gap_extend=DEFAULT_GAP_EXTEND,
+               x_dropoff=DEFAULT_GAP_XDROP):
+        """Preliminary gapped extension of this scan's HSPs (hb_scan_gapped)
+        -> (stats[n_subjects], gapped hsps[n]) numpy structured arrays."""
+        gp = (C.c_int32 * 4)(gap_open, gap_extend, x_dropoff, 0)
+        cut = np.ascontiguousarray(cutoffs, np.int32)
+        n_ungapped = C.c_uint64()
+        _check(_lib().hb_scan_results(self.h, None, None, 0, C.byref(n_ungapped)),
+               "hb_scan_results")
+        stats = np.zeros(self.db.n, GSTATS_DTYPE)
+        out = np.zeros(max(n_ungapped.value, 1), GHSP_DTYPE)
+        n = C.c_uint64()
+        _check(_lib().hb_scan_gapped(self.h, C.byref(gp), _ptr(cut), _ptr(stats), _ptr(out),
+                                     len(out), C.byref(n)), "hb_scan_gapped")
+        return stats, out[:n.value].copy()
+
+    def gapped_info(self):
+        """-> (device ms of the last gapped stage, subjects run in the wide pass)"""
+        ms, nw = C.c_float(), C.c_uint32()
+        _check(_lib().hb_scan_gapped_info(self.h, C.byref(ms), C.byref(nw)),
+               "hb_scan_gapped_info")
+        return ms.value, nw.value
 
     def close(self):
         if self.h:

@@ -174,6 +174,64 @@ int hb_search(hb_ctx *ctx, hb_db *db, const uint8_t *seq, int32_t len,
               const int32_t *xdrop, const int32_t *cutoff, const hb_params *params,
               hb_subject_stats *stats, hb_hsp *hsps, uint64_t cap, uint64_t *n_hsp);
 
+/* --- preliminary gapped extension ---------------------------------------
+ * The consumer of the seed stage's HSPs, run on the device for every subject
+ * of a finished scan at once.  Replaces the per-subject calls
+ * aux_struct->GetGappedScore(...) = BLAST_GetGappedScore
+ * (ncbi_blast_files/blast_engine.c:617, blast_gapalign.c:3233-3560) made by
+ * the CPU threads on the HSPs of H_BLAST_get_data, for in-frame protein
+ * searches (BLASTP, BLASTX), score only, without restricted alignment:
+ * per subject the ungapped HSPs are taken by descending score
+ * (Blast_InitHitListSortByScore, blast_engine.c:584), an HSP held by a saved
+ * gapped alignment of at least its score is skipped (the interval-tree test,
+ * blast_gapalign.c:3402), the others are extended both ways from the middle
+ * of their best 11-residue window (blast_gapalign.c:3184-3229) by the
+ * X-drop DP of Blast_SemiGappedAlign (blast_gapalign.c:711-937), and saved
+ * when they reach the context's gapped cutoff (blast_gapalign.c:3518-3534). */
+typedef struct hb_gap_params {
+    int32_t gap_open;    /* default 11 (BLAST_GAP_OPEN_PROT, blast_options.h:81) */
+    int32_t gap_extend;  /* default 1 (BLAST_GAP_EXTN_PROT, blast_options.h:88) */
+    int32_t x_dropoff;   /* raw gapped X-drop; default 38 */
+    int32_t reserved;    /* 0 */
+} hb_gap_params;
+/* Defaults: 11, 1 and X = (int)(15 ln2 / 0.267) = 38, the 15-bit
+ * BLAST_GAP_X_DROPOFF_PROT (blast_options.h:118) with BLOSUM62 11/1 gapped
+ * lambda 0.267 (truncated, as BlastExtensionParametersNew converts it). */
+void hb_gap_params_default(hb_gap_params *p);
+#define HB_DEFAULT_GAP_XDROP 38
+
+/* One gapped HSP (the fields of Blast_HSPInit, blast_gapalign.c:3522-3528). */
+typedef struct hb_gapped_hsp {
+    uint32_t oid;
+    int32_t context;
+    int32_t q_start, q_end;   /* context-relative query range, end exclusive */
+    int32_t s_start, s_end;   /* subject range, end exclusive */
+    int32_t q_gapped_start;   /* start point of the extension (context-relative) */
+    int32_t s_gapped_start;
+    int32_t score;            /* gapped raw score */
+    int32_t pad;
+} hb_gapped_hsp;
+
+/* Per-subject counters of the gapped stage. */
+typedef struct hb_gapped_stats {
+    uint32_t extensions;  /* gapped extensions performed (gapped_stats->extensions) */
+    uint32_t n_hsp;       /* gapped HSPs saved (score >= the context's cutoff) */
+    int64_t first;        /* index of the subject's first gapped HSP, -1 if none */
+} hb_gapped_stats;
+
+/* Runs the gapped stage on the HSPs of scan s (waiting for the scan first).
+ * gap_cutoff holds one raw gapped cutoff per query context
+ * (hit_params->cutoffs[context].cutoff_score).  hsps receives the gapped
+ * HSPs grouped by ascending oid, each subject's in save order; a capacity
+ * equal to the scan's HSP count always suffices.  stats (hb_db_subjects()
+ * entries) may be NULL. */
+int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutoff,
+                   hb_gapped_stats *stats, hb_gapped_hsp *hsps, uint64_t cap,
+                   uint64_t *n_hsp);
+/* Device time of the last hb_scan_gapped's kernels (ms) and the number of
+ * subjects it handed to the wide (global-memory) pass. */
+int hb_scan_gapped_info(hb_scan *s, float *ms, uint32_t *n_wide);
+
 /* --- batched form --------------------------------------------------------
  * Many queries (BLASTP or six-frame BLASTX, mixed lengths) searched against
  * the resident database in one persistent launch per size class.  Queries
