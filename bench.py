@@ -9,9 +9,11 @@ resident in HBM).  Word 3, T 11, window 40, X-drop 16, cutoff 41.
 
 One step = one pass of the stage over the whole database for the query
 (hb_scan_start + hb_scan_wait: lookup, two-hit tracking, extensions, HSP
-records and per-subject counters in HBM).  The same line carries a compact
-batched-vs-plain and served measurement on the same database ("forms";
---no-forms skips it).  --config c3 / c4 run the full batched (2,048
+records and per-subject counters in HBM).  The same line carries the
+preliminary gapped extension of the last pass's HSPs ("gapped": device
+kernel time, extensions, saved HSPs, the oracle on a bounded sample;
+--no-gapped skips it) and a compact batched-vs-plain and served measurement
+on the same database ("forms"; --no-forms skips it).  --config c3 / c4 run the full batched (2,048
 queries) and served (20,000,000 subjects) workloads instead.  Prints one
 JSON line.
 """
@@ -79,6 +81,56 @@ def cpu_baseline(q, res, offs, seconds_target=10.0):
             "kind": "port",
             "sample": f"first {n} of {n_total} subjects ({residues} residues), "
                       f"{dt:.2f} s wall on {threads} threads"}
+
+
+def gapped_cutoff(query_len, db_residues, evalue=10.0, lam=0.267, K=0.041):
+    """Raw gapped cutoff for E-value `evalue`: ceil(ln(K m n / E) / lambda)
+    with BLOSUM62 11/1 gapped lambda 0.267 and K 0.041, over the search space
+    m x n without NCBI's length adjustment."""
+    return int(np.ceil(np.log(K * query_len * db_residues / evalue) / lam))
+
+
+def measure_gapped(scan, q, res, offs, hsps, cutoff, reps=3, cpu=True):
+    """The preliminary gapped extension (hb_scan_gapped) on one scan's HSPs:
+    device kernel time (hipEvents around the gapped kernels), the whole call
+    (layout on the device, kernels, copy out), and the oracle on a bounded
+    leading sample of the same HSPs."""
+    kernel, call = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        gst, g = scan.gapped([cutoff])
+        call.append((time.perf_counter() - t0) * 1e3)
+        ms, n_wide = scan.gapped_info()
+        kernel.append(ms)
+    ext = int(gst["extensions"].sum())
+    out = {"kernel_ms": float(np.mean(kernel)), "call_ms": float(np.mean(call)),
+           "ungapped_hsps": int(len(hsps)), "extensions": ext, "hsps": int(len(g)),
+           "wide_subjects": int(n_wide),
+           "extensions_per_s": ext / (np.mean(kernel) * 1e-3),
+           "params": {"gap_open": 11, "gap_extend": 1, "x_dropoff": 38, "cutoff": cutoff}}
+    if cpu:
+        import oracle_lib
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        threads = max(1, min(threads, 64))
+        oq = oracle_lib.OracleQuery.from_spec(hbseed.blastp_query(q))
+        n_total = len(offs) - 1
+        n = min(n_total, 100000)
+        while True:
+            k = int(np.searchsorted(hsps["oid"], n))
+            t0 = time.perf_counter()
+            cnt, _ = oracle_lib.gapped_db(oq, res, offs[:n + 1], hsps[:k], [cutoff],
+                                          threads=threads)
+            dt = time.perf_counter() - t0
+            if n >= n_total or dt * threads >= 10.0 or dt > 2.5:
+                break
+            n = min(n_total, int(n * max(2.0, 10.0 / max(dt * threads, 1e-3))))
+        sample_ext = int(cnt[:, 1].sum())
+        out["cpu_baseline"] = {
+            "value": sample_ext / dt, "unit": "gapped extensions/s", "cores": threads,
+            "kind": "port",
+            "sample": f"HSPs of the first {n} of {n_total} subjects ({k} ungapped HSPs, "
+                      f"{sample_ext} extensions), {dt:.2f} s wall on {threads} threads"}
+    return out
 
 
 def measure_batched(ctx, db, qs, steps, warmup, plain_sample, group_residues=0):
@@ -243,6 +295,8 @@ def main():
     ap.add_argument("--max-inflight", type=int, default=2, help="c4: batches in flight")
     ap.add_argument("--no-forms", action="store_true",
                     help="c1: skip the compact batched and served measurements")
+    ap.add_argument("--no-gapped", action="store_true",
+                    help="c1: skip the gapped-extension measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; --gpus must be 1 "
@@ -288,6 +342,11 @@ def main():
     tr0 = time.perf_counter()
     stats, hsps = last.results()
     results_ms = (time.perf_counter() - tr0) * 1e3
+    gapped = None
+    if not args.no_gapped:
+        log("gapped extension of the last pass's HSPs")
+        gapped = measure_gapped(last, q, res, offs, hsps, gapped_cutoff(len(q), n_res),
+                                cpu=not args.no_cpu)
     last.close()
 
     copy = None
@@ -350,6 +409,8 @@ def main():
     }
     if copy:
         out["copy_calibration"] = copy
+    if gapped:
+        out["gapped"] = gapped
     if forms:
         out["forms"] = forms
     print(json.dumps(out), flush=True)

@@ -57,23 +57,55 @@ __device__ __forceinline__ int32_t dppn(int32_t v, int32_t old)
     return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
 }
 
-// Inclusive max-scan over the wavefront (row_shr 1/2/4/8 within rows of 16
-// lanes, row_bcast 15/31 across rows); lanes with no source keep kNeg.
-__device__ __forceinline__ int32_t scan_max(int32_t v)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dppu(uint32_t v)
 {
-    v = max(v, dppn<0x111, 0xf>(v, kNeg));
-    v = max(v, dppn<0x112, 0xf>(v, kNeg));
-    v = max(v, dppn<0x114, 0xf>(v, kNeg));
-    v = max(v, dppn<0x118, 0xf>(v, kNeg));
-    v = max(v, dppn<0x142, 0xa>(v, kNeg));
-    v = max(v, dppn<0x143, 0xc>(v, kNeg));
-    return v;
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+// Scores are scanned as v + kBias: every value met (down to kNeg - 128) is
+// then positive, so 0 is the identity of an unsigned max and the lane moves
+// fuse into v_max_u32 with DPP operands.
+constexpr uint32_t kBias = (1u << 30) + (1u << 20);
+
+// Two inclusive max-scans over the wavefront, interleaved (row_shr 1/2/4/8
+// within rows of 16 lanes, row_bcast 15/31 across rows).
+__device__ __forceinline__ void scan_max2(uint32_t &a, uint32_t &b)
+{
+    a = max(a, dppu<0x111, 0xf>(a));
+    b = max(b, dppu<0x111, 0xf>(b));
+    a = max(a, dppu<0x112, 0xf>(a));
+    b = max(b, dppu<0x112, 0xf>(b));
+    a = max(a, dppu<0x114, 0xf>(a));
+    b = max(b, dppu<0x114, 0xf>(b));
+    a = max(a, dppu<0x118, 0xf>(a));
+    b = max(b, dppu<0x118, 0xf>(b));
+    a = max(a, dppu<0x142, 0xa>(a));
+    b = max(b, dppu<0x142, 0xa>(b));
+    a = max(a, dppu<0x143, 0xc>(a));
+    b = max(b, dppu<0x143, 0xc>(b));
+}
+
+__device__ __forceinline__ uint32_t scan_max1(uint32_t a)
+{
+    a = max(a, dppu<0x111, 0xf>(a));
+    a = max(a, dppu<0x112, 0xf>(a));
+    a = max(a, dppu<0x114, 0xf>(a));
+    a = max(a, dppu<0x118, 0xf>(a));
+    a = max(a, dppu<0x142, 0xa>(a));
+    a = max(a, dppu<0x143, 0xc>(a));
+    return a;
 }
 
 // Value of lane - 1 (wave_shr:1); lane 0 receives `first`.
 __device__ __forceinline__ int32_t from_prev_lane(int32_t v, int32_t first)
 {
     return dppn<0x138, 0xf>(v, first);
+}
+
+__device__ __forceinline__ uint32_t from_prev_lane_u(uint32_t v)
+{
+    return dppu<0x138, 0xf>(v);
 }
 
 __device__ __forceinline__ int32_t lane_read(int32_t v, int32_t lane)
@@ -104,19 +136,31 @@ template <bool WIDE>
 struct GapWave {
     const hb_gap_args &A;
     const int8_t *mat;     // LDS 32x32
-    int32_t *best;         // ring of DP columns: best score in the column
-    int32_t *vgap;         // best score ending in a gap in the subject (carried down)
+    int2 *ring;            // DP columns: {best score, best ending in a gap in the subject}
     uint32_t rmask;        // ring size - 1
-    uint32_t *order;       // ranked HSP indices of the current subject
+    uint16_t *order16;     // LDS: ranked HSP indices of the current subject
+    uint32_t *order32;     // wide pass: the same in global scratch
     int32_t *saved;        // 6 ints per saved HSP: ctx, score, q0, q1, s0, s1
+    uint8_t *sbuf;         // LDS copy of the current subject (HB_GAP_SUBJ bytes)
     int32_t lane;
     int32_t go, ge, goe, X;
 
-    // Ranks the subject's n ungapped HSPs; order[rank] = index.
+    __device__ __forceinline__ void put_order(uint32_t r, uint32_t i)
+    {
+        if (WIDE) order32[r] = i;
+        else order16[r] = (uint16_t)i;
+    }
+    __device__ __forceinline__ uint32_t get_order(uint32_t r) const
+    {
+        return WIDE ? order32[r] : (uint32_t)order16[r];
+    }
+
+    // Ranks the subject's n ungapped HSPs by (score desc, s_start asc, len
+    // desc, q_start asc, index asc); order[rank] = index.
     __device__ void rank_hsps(const hb_hsp *U, uint32_t n)
     {
         if (n == 1) {
-            if (lane == 0) order[0] = 0;
+            if (lane == 0) put_order(0, 0);
             sync_lanes<WIDE>();
             return;
         }
@@ -156,7 +200,7 @@ struct GapWave {
                     rank += before ? 1u : 0u;
                 }
             }
-            if (vi) order[rank] = i;
+            if (vi) put_order(rank, i);
         }
         sync_lanes<WIDE>();
     }
@@ -194,7 +238,7 @@ struct GapWave {
                 for (int32_t t = 0; t < kWin; ++t)
                     w += mat[32 * q[qs + e - (kWin - 1) + t] + s[ss + e - (kWin - 1) + t]];
             }
-            const int32_t cmax = lane_read(scan_max(w), 63);
+            const int32_t cmax = (int32_t)((uint32_t)lane_read((int32_t)scan_max1((uint32_t)w + kBias), 63) - kBias);
             if (cmax > top) {
                 top = cmax;
                 at = e0 + (int32_t)__builtin_ctzll(__ballot(e < len && w == cmax));
@@ -205,23 +249,23 @@ struct GapWave {
 
     // Score-only X-drop DP of Blast_SemiGappedAlign from the origin.  Row a
     // (1..M) uses query letter arow[a] (REV: arow[-a]); column b (1..N) uses
-    // subject letter brow[b] (REV: brow[-b]).  Sets *ovf when the band
-    // outgrows the ring.
-    template <bool REV>
-    __device__ int32_t xdrop(const uint8_t *arow, const uint8_t *brow, int32_t M, int32_t N,
+    // subject letter subj[so + b] (REV: subj[so + 1 - b]).  Sets *ovf when
+    // the band outgrows the ring.
+    template <bool REV, class SUBJ>
+    __device__ int32_t xdrop(const uint8_t *arow, SUBJ subj, int32_t so, int32_t M, int32_t N,
                              int32_t *a_off, int32_t *b_off, bool *ovf)
     {
         *a_off = 0;
         *b_off = 0;
         if (M <= 0 || N <= 0) return 0;
-        const int32_t ring = (int32_t)rmask + 1;
+        const int32_t ring_n = (int32_t)rmask + 1;
         // row 0: leading gaps within the drop-off
         int32_t end = N + 1;
         if (ge > 0) {
             const int32_t k = (X - goe) / ge + 2;
             if (k < end) end = k;
         }
-        if (end > ring) {
+        if (end > ring_n) {
             *ovf = true;
             return 0;
         }
@@ -229,11 +273,11 @@ struct GapWave {
             const int32_t c = c0 + lane;
             if (c < end) {
                 const int32_t v = c ? -(go + c * ge) : 0;
-                best[c & rmask] = v;
-                vgap[c & rmask] = v - goe;
+                ring[c & rmask] = make_int2(v, v - goe);
             }
         }
         sync_lanes<WIDE>();
+        const int32_t lane_ge = lane * ge;
         int32_t lo = 0, top = 0, abest = 0, bbest = 0;
         int32_t rows = 0;
         for (int32_t a = 1; a <= M; ++a) {
@@ -248,20 +292,27 @@ struct GapWave {
             for (int32_t c0 = lo; c0 < end; c0 += 64) {
                 const int32_t b = c0 + lane;
                 const bool valid = b < end;
-                const int32_t ob = valid ? best[b & rmask] : kNeg;
-                const int32_t ov = valid ? vgap[b & rmask] : kNeg;
-                // diagonal from column b - 1 of the previous row; none into
-                // the band's first column
+                const int32_t col = (valid && b >= 1) ? b : 1;
+                const int32_t bl = REV ? subj[so + 1 - col] : subj[so + col];
+                const int2 cell = ring[b & rmask];
+                const int32_t ob = valid ? cell.x : kNeg;
+                const int32_t ov = valid ? cell.y : kNeg;
                 const int32_t pb = from_prev_lane(ob, carry);
-                const int32_t bl = (valid && b >= 1) ? (int32_t)(REV ? brow[-b] : brow[b]) : 0;
-                const int32_t T = max(pb + mrow[bl], ov);
-                const int32_t P = from_prev_lane(scan_max(T + lane * ge), kNeg);
-                const int32_t R = max(rin - lane * ge, P - goe - (lane - 1) * ge);
-                const int32_t s = valid ? max(T, R) : kNeg;
-                const int32_t tb = max(topc, from_prev_lane(scan_max(s), kNeg));
+                const int32_t T = valid ? max(pb + mrow[bl], ov) : kNeg;
+                uint32_t uA = (uint32_t)(T + lane_ge) + kBias;
+                uint32_t uB = (uint32_t)T + kBias;
+                scan_max2(uA, uB);
+                const uint32_t eA = from_prev_lane_u(uA);
+                const uint32_t eB = from_prev_lane_u(uB);
+                const int32_t P = (int32_t)(eA - kBias);
+                const int32_t R = max(rin - lane_ge, P - goe - lane_ge + ge);
+                const int32_t s = max(T, R);
+                const int32_t before = lane ? max(rin, (int32_t)(eB - kBias)) : kNeg;
+                const int32_t tb = max(topc, before);
                 const bool live = valid && !(tb - s > X);
-                if (valid) best[b & rmask] = live ? s : kNeg;
-                if (live) vgap[b & rmask] = max(s - goe, ov - ge);
+                if (valid)
+                    ring[b & rmask] =
+                        live ? make_int2(s, max(s - goe, ov - ge)) : make_int2(kNeg, ov);
                 const uint64_t lm = __ballot(live);
                 const int32_t nvalid = end - c0 < 64 ? end - c0 : 64;
                 if (lead) {
@@ -273,12 +324,13 @@ struct GapWave {
                     }
                 }
                 if (lm) last = c0 + 63 - (int32_t)__builtin_clzll(lm);
-                const int32_t cmax = lane_read(scan_max(s), 63);
+                const int32_t cmax =
+                    max(rin, (int32_t)((uint32_t)lane_read((int32_t)uB, 63) - kBias));
                 if (cmax > topc) {
                     topc = cmax;
                     rowarg = c0 + (int32_t)__builtin_ctzll(__ballot(valid && s == cmax));
                 }
-                const int32_t rnext = max(T - goe, R - ge);   // R after column b
+                const int32_t rnext = max(T - goe, R - ge);
                 rin = lane_read(rnext, 63);
                 rafter = lane_read(rnext, nvalid - 1);
                 carry = lane_read(ob, 63);
@@ -305,7 +357,7 @@ struct GapWave {
                 new_end = end + n_ext;
             }
             const int32_t guard = new_end <= N ? 1 : 0;
-            if (new_end + guard - lo > ring) {
+            if (new_end + guard - lo > ring_n) {
                 *ovf = true;
                 return 0;
             }
@@ -313,20 +365,49 @@ struct GapWave {
                 const int32_t i = i0 + lane;
                 if (i < n_ext) {
                     const int32_t v = rafter - i * ge;
-                    best[(end + i) & rmask] = v;
-                    vgap[(end + i) & rmask] = v - goe;
+                    ring[(end + i) & rmask] = make_int2(v, v - goe);
                 }
             }
-            if (guard && lane == 0) {
-                best[new_end & rmask] = kNeg;
-                vgap[new_end & rmask] = kNeg;
-            }
+            if (guard && lane == 0) ring[new_end & rmask] = make_int2(kNeg, kNeg);
             end = new_end + guard;
             sync_lanes<WIDE>();
         }
         *a_off = abest;
         *b_off = bbest;
         return top;
+    }
+
+    // Both sides of one extension through (qo, so); returns the score and
+    // fills the ranges.
+    template <class SUBJ>
+    __device__ int32_t extend(const uint8_t *qc, int32_t clen, SUBJ subj, int32_t slen,
+                              int32_t qo, int32_t so, int32_t *co, bool *ovf)
+    {
+        // AdjustSubjectRange
+        int32_t ql = qo + 1, sl = so + 1, sub_len = slen, shift = 0;
+        if (slen >= kLongSubject) {
+            const int32_t reach_left = ql + kGapAllowance;
+            const int32_t reach_right = clen - ql + kGapAllowance;
+            const int32_t s_off1 = sl;
+            if (s_off1 > reach_left) {
+                shift = s_off1 - reach_left;
+                sl = reach_left;
+            }
+            sub_len = min(slen, s_off1 + reach_right) - shift;
+        }
+        int32_t a, b;
+        const int32_t left = xdrop<true>(qc + qo + 1, subj, so, ql, sl, &a, &b, ovf);
+        co[0] = ql - a;
+        co[2] = sl - b + shift;
+        int32_t right = 0;
+        co[1] = qo;
+        co[3] = so;
+        if (ql < clen && sl < sub_len) {
+            right = xdrop<false>(qc + qo, subj, so, clen - ql, sub_len - sl, &a, &b, ovf);
+            co[1] = a + qo + 1;
+            co[3] = b + so + 1;
+        }
+        return left + right;
     }
 
     // One subject; false when it outgrows the LDS capacities.
@@ -340,10 +421,21 @@ struct GapWave {
         const int32_t slen = (int32_t)(A.offs[k + 1] - sbase);
         const uint8_t *subj = A.res + sbase;
         const hb_hsp *U = A.urec + f;
+        const bool staged = !WIDE && slen <= HB_GAP_SUBJ;
+        if (staged) {
+            // whole aligned words around the subject, 4 bytes per lane per
+            // step (the residue buffer has 64 bytes of padding; bytes outside
+            // the subject are never used)
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(subj - (sbase & 3));
+            uint32_t *dst = reinterpret_cast<uint32_t *>(sbuf);
+            const int32_t words = (int32_t)((slen + (sbase & 3) + 3) >> 2);
+            for (int32_t w = lane; w < words; w += 64) dst[w] = src[w];
+        }
+        const int32_t sofs = staged ? (int32_t)(sbase & 3) : 0;
         rank_hsps(U, n);
         uint32_t n_saved = 0, ext = 0;
         for (uint32_t r = 0; r < n; ++r) {
-            const uint32_t i = (uint32_t)uniform((int32_t)order[r]);
+            const uint32_t i = (uint32_t)uniform((int32_t)get_order(r));
             const int32_t c = uniform(U[i].context);
             const int32_t base = A.ctx[c];
             const int32_t clen = A.ctx[A.n_ctx + c];
@@ -357,41 +449,24 @@ struct GapWave {
             ++ext;
             const int32_t qo = start_point(qc, subj, qs, len, ss);
             const int32_t so = uniform(U[i].s_off) + (qo - (uniform(U[i].q_off) - base));
-            // AdjustSubjectRange
-            int32_t ql = qo + 1, sl = so + 1, sub_len = slen, shift = 0;
-            if (slen >= kLongSubject) {
-                const int32_t reach_left = ql + kGapAllowance;
-                const int32_t reach_right = clen - ql + kGapAllowance;
-                const int32_t s_off1 = sl;
-                if (s_off1 > reach_left) {
-                    shift = s_off1 - reach_left;
-                    sl = reach_left;
-                }
-                sub_len = min(slen, s_off1 + reach_right) - shift;
-            }
             bool ovf = false;
-            int32_t a, b;
-            const int32_t left =
-                xdrop<true>(qc + qo + 1, subj + so + 1, ql, sl, &a, &b, &ovf);
-            const int32_t q_start = ql - a, s_start = sl - b + shift;
-            int32_t right = 0, q_end = qo, s_end = so;
-            if (ql < clen && sl < sub_len) {
-                right = xdrop<false>(qc + qo, subj + so, clen - ql, sub_len - sl, &a, &b, &ovf);
-                q_end = a + qo + 1;
-                s_end = b + so + 1;
-            }
+            int32_t co[4];
+            int32_t score;
+            if (staged)
+                score = extend(qc, clen, (const uint8_t *)sbuf + sofs, slen, qo, so, co, &ovf);
+            else
+                score = extend(qc, clen, subj, slen, qo, so, co, &ovf);
             if (ovf) return false;
-            const int32_t score = left + right;
             if (score >= cut) {
                 if (!WIDE && n_saved >= HB_GAP_SAVED) return false;
                 if (lane == 0) {
                     hb_gapped_hsp g;
                     g.oid = k;
                     g.context = c;
-                    g.q_start = q_start;
-                    g.q_end = q_end;
-                    g.s_start = s_start;
-                    g.s_end = s_end;
+                    g.q_start = co[0];
+                    g.q_end = co[1];
+                    g.s_start = co[2];
+                    g.s_end = co[3];
                     g.q_gapped_start = qo;
                     g.s_gapped_start = so;
                     g.score = score;
@@ -400,10 +475,10 @@ struct GapWave {
                     int32_t *e = saved + 6 * (size_t)n_saved;
                     e[0] = c;
                     e[1] = score;
-                    e[2] = q_start;
-                    e[3] = q_end;
-                    e[4] = s_start;
-                    e[5] = s_end;
+                    e[2] = co[0];
+                    e[3] = co[1];
+                    e[4] = co[2];
+                    e[5] = co[3];
                 }
                 sync_lanes<WIDE>();
                 ++n_saved;
@@ -418,63 +493,56 @@ struct GapWave {
     }
 };
 
+// Per-wavefront LDS of the main pass.
+struct GapLds {
+    int2 ring[HB_GAP_RING];
+    int32_t saved[6 * HB_GAP_SAVED];
+    uint16_t order[HB_GAP_ORDER];
+    uint32_t subj[HB_GAP_SUBJ / 4 + 2];   // whole words around the subject
+};
+
 template <bool WIDE>
 __global__ __launch_bounds__(kGapWaves * HB_WAVE) void gapped_kernel(hb_gap_args A)
 {
     __shared__ int8_t mat[32 * 32];
-    __shared__ int32_t ring_lds[WIDE ? 1 : kGapWaves * 2 * HB_GAP_RING];
-    __shared__ uint32_t order_lds[WIDE ? 1 : kGapWaves * HB_GAP_ORDER];
-    __shared__ int32_t saved_lds[WIDE ? 1 : kGapWaves * 6 * HB_GAP_SAVED];
+    __shared__ GapLds lds[WIDE ? 1 : kGapWaves];
     for (int i = threadIdx.x; i < 32 * 32 / 4; i += blockDim.x)
         reinterpret_cast<int32_t *>(mat)[i] = reinterpret_cast<const int32_t *>(A.mat32)[i];
     __syncthreads();
     const int wave = threadIdx.x / HB_WAVE;
     const int lane = threadIdx.x % HB_WAVE;
-    GapWave<WIDE> W{A, mat, nullptr, nullptr, 0, nullptr, nullptr, lane, 0, 0, 0, 0};
+    GapWave<WIDE> W{A, mat, nullptr, 0, nullptr, nullptr, nullptr, nullptr, lane, 0, 0, 0, 0};
     W.go = A.gap_open;
     W.ge = A.gap_extend;
     W.goe = A.gap_open + A.gap_extend;
     W.X = A.x_dropoff < W.goe ? W.goe : A.x_dropoff;
     if (!WIDE) {
-        W.best = ring_lds + wave * 2 * HB_GAP_RING;
-        W.vgap = W.best + HB_GAP_RING;
+        GapLds &L = lds[wave];
+        W.ring = L.ring;
         W.rmask = HB_GAP_RING - 1;
-        W.order = order_lds + wave * HB_GAP_ORDER;
-        W.saved = saved_lds + wave * 6 * HB_GAP_SAVED;
-        const uint32_t n_chunks = (A.n_subjects + 63) / 64;
+        W.order16 = L.order;
+        W.saved = L.saved;
+        W.sbuf = reinterpret_cast<uint8_t *>(L.subj);
+        // one subject per pop, subjects with the most HSPs first
+        const uint32_t n_list = (uint32_t)uniform((int32_t)*A.n_list);
         for (;;) {
             uint32_t item = 0;
             if (lane == 0) item = atomicAdd(A.counter, 1u);
             item = (uint32_t)lane_read((int32_t)item, 0);
-            if (item >= n_chunks) break;
-            const uint32_t k = item * 64 + (uint32_t)lane;
-            uint32_t nh = 0;
-            if (k < A.n_subjects) {
-                nh = A.ust[k].n_hsp;
-                if (nh == 0) {
-                    A.st3[3 * (size_t)k] = 0;
-                    A.st3[3 * (size_t)k + 1] = 0;
-                    A.st3[3 * (size_t)k + 2] = 0;
-                }
-            }
-            uint64_t todo = __ballot(nh > 0);
-            while (todo) {
-                const uint32_t kk = item * 64 + (uint32_t)__builtin_ctzll(todo);
-                todo &= todo - 1;
-                if (!W.subject(kk) && lane == 0) {
-                    const uint32_t p = atomicAdd(A.wide, 1u);
-                    A.wide[1 + p] = kk;
-                }
+            if (item >= n_list) break;
+            const uint32_t k = (uint32_t)uniform((int32_t)A.list[item]);
+            if (!W.subject(k) && lane == 0) {
+                const uint32_t p = atomicAdd(A.wide, 1u);
+                A.wide[1 + p] = k;
             }
         }
     } else {
         const size_t gw = (size_t)blockIdx.x * kGapWaves + wave;
         int32_t *scr = A.scratch + gw * A.scratch_words;
-        W.best = scr;
-        W.vgap = scr + A.ring_wide;
+        W.ring = reinterpret_cast<int2 *>(scr);
         W.rmask = A.ring_wide - 1;
         W.saved = scr + 2 * (size_t)A.ring_wide;
-        W.order = reinterpret_cast<uint32_t *>(W.saved + 6 * (size_t)A.n_max);
+        W.order32 = reinterpret_cast<uint32_t *>(W.saved + 6 * (size_t)A.n_max);
         for (;;) {
             uint32_t item = 0;
             if (lane == 0) item = atomicAdd(A.counter, 1u);
@@ -483,6 +551,69 @@ __global__ __launch_bounds__(kGapWaves * HB_WAVE) void gapped_kernel(hb_gap_args
             W.subject(A.wide[1 + item]);
         }
     }
+}
+
+// Work list: bucket b = 63 - min(n_hsp, 63) for subjects with HSPs, so that
+// ascending buckets are descending HSP counts.
+__device__ __forceinline__ uint32_t gap_bucket_of(uint32_t n) { return 63 - (n < 63 ? n : 63); }
+
+__global__ __launch_bounds__(256) void gap_count_kernel(hb_gap_args A, uint32_t *bucket)
+{
+    __shared__ uint32_t h[64];
+    if (threadIdx.x < 64) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < A.n_subjects) {
+        const uint32_t n = A.ust[k].n_hsp;
+        if (n == 0) {
+            A.st3[3 * (size_t)k] = 0;
+            A.st3[3 * (size_t)k + 1] = 0;
+            A.st3[3 * (size_t)k + 2] = 0;
+        } else {
+            atomicAdd(&h[gap_bucket_of(n)], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&bucket[threadIdx.x], h[threadIdx.x]);
+}
+
+// One wavefront: exclusive prefix of the 64 bucket counts into the cursors
+// (bucket[64 ..]) and the total.
+__global__ void gap_bucket_kernel(uint32_t *bucket, uint32_t *n_list)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t v = bucket[lane];
+    uint32_t incl = v;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    bucket[64 + lane] = incl - v;
+    if (lane == 63) *n_list = incl;
+}
+
+// Places each subject with HSPs in its bucket: a workgroup reserves one range
+// per bucket with a single global atomic, its threads take slots in LDS.
+__global__ __launch_bounds__(256) void gap_place_kernel(hb_gap_args A, uint32_t *bucket,
+                                                        uint32_t *list)
+{
+    __shared__ uint32_t h[64], base[64];
+    if (threadIdx.x < 64) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t bk = 64, slot = 0;
+    if (k < A.n_subjects) {
+        const uint32_t n = A.ust[k].n_hsp;
+        if (n) {
+            bk = gap_bucket_of(n);
+            slot = atomicAdd(&h[bk], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x])
+        base[threadIdx.x] = atomicAdd(&bucket[64 + threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (bk < 64) list[base[bk] + slot] = k;
 }
 
 // One thread per subject: dense layout of the gapped HSPs and the stats.
@@ -504,6 +635,18 @@ __global__ void gap_compact_kernel(hb_gap_args A, const uint32_t *first, hb_gapp
 }
 
 }  // namespace
+
+hipError_t hb_launch_gap_list(const hb_gap_args &a, uint32_t *list, uint32_t *n_list,
+                              uint32_t *scratch, hipStream_t st)
+{
+    hipError_t e = hb_launch_fill_u32(scratch, 0u, 128, st);
+    if (e != hipSuccess || a.n_subjects == 0) return e;
+    const dim3 grid((a.n_subjects + 255) / 256);
+    hipLaunchKernelGGL(gap_count_kernel, grid, dim3(256), 0, st, a, scratch);
+    hipLaunchKernelGGL(gap_bucket_kernel, dim3(1), dim3(64), 0, st, scratch, n_list);
+    hipLaunchKernelGGL(gap_place_kernel, grid, dim3(256), 0, st, a, scratch, list);
+    return hipGetLastError();
+}
 
 hipError_t hb_launch_gapped(const hb_gap_args &a, bool wide, int n_blocks, hipStream_t st)
 {

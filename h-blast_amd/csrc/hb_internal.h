@@ -139,7 +139,8 @@ hipError_t hb_launch_first_index(const uint32_t *st3, uint32_t ns, uint32_t *fir
 // Kernel arguments of the preliminary gapped extension (hb_gapped.hip).
 #define HB_GAP_RING 256       // DP columns per wavefront kept in LDS (power of two)
 #define HB_GAP_ORDER 256      // ungapped HSPs per subject ordered in LDS
-#define HB_GAP_SAVED 64       // saved gapped HSPs per subject kept in LDS
+#define HB_GAP_SAVED 32       // saved gapped HSPs per subject kept in LDS
+#define HB_GAP_SUBJ 2048      // subjects up to this length are staged in LDS
 struct hb_gap_args {
     const uint8_t *res;            // database residues
     const uint64_t *offs;          // n + 1 subject offsets
@@ -152,6 +153,8 @@ struct hb_gap_args {
     const hb_hsp *urec;            // ungapped HSPs grouped by subject
     uint32_t n_subjects;
     uint32_t *counter;             // work counter (zeroed before the launch)
+    const uint32_t *list;          // subjects with HSPs, most HSPs first
+    const uint32_t *n_list;        // their number (device)
     hb_gapped_hsp *out;            // subject k's gapped HSPs at ust[k].first + i
     uint32_t *st3;                 // per subject {0, extensions, saved}
     uint32_t *wide;                // [0] count, then subjects for the wide pass
@@ -164,6 +167,11 @@ struct hb_gap_args {
     uint32_t n_max;                // most ungapped HSPs of a wide subject
     uint32_t scratch_words;
 };
+// Work list of the main pass: list[] = subjects with HSPs ordered by
+// descending HSP count (capped at 63), *n_list their number; subjects without
+// HSPs get zero stats.  scratch: 128 uint32 (zeroed here).
+hipError_t hb_launch_gap_list(const hb_gap_args &a, uint32_t *list, uint32_t *n_list,
+                              uint32_t *scratch, hipStream_t st);
 hipError_t hb_launch_gapped(const hb_gap_args &a, bool wide, int n_blocks, hipStream_t st);
 // Dense layout: stats[k] and out_dense from the sparse output and first[].
 hipError_t hb_launch_gap_compact(const hb_gap_args &a, const uint32_t *first,

@@ -898,6 +898,7 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
     auto *d_first = static_cast<uint32_t *>(get(((size_t)ns + nb + 1) * sizeof(uint32_t)));
     auto *d_gst = static_cast<hb_gapped_stats *>(get((size_t)ns * sizeof(hb_gapped_stats)));
     auto *d_gdense = static_cast<hb_gapped_hsp *>(get(nr * sizeof(hb_gapped_hsp)));
+    auto *d_list = static_cast<uint32_t *>(get(((size_t)ns + 129) * sizeof(uint32_t)));
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     auto release = [&]() {
         for (auto &h : held) hb_pool_put(c, h.first, h.second);
@@ -932,6 +933,8 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
     a.out = d_gout;
     a.st3 = d_st3;
     a.wide = d_misc + 1;
+    a.list = d_list + 129;
+    a.n_list = d_list + 128;
     uint32_t n_wide = 0;
     uint32_t *d_nbig = d_scr + ns + nb;
     hipError_t e = hb_launch_fill_u32(d_nbig, 0u, 1, st);
@@ -942,6 +945,7 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
         e = hipMemcpyAsync(d_ctx, hctx.data(), hctx.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hb_launch_fill_u32(d_misc, 0u, 2, st);
+    if (e == hipSuccess) e = hb_launch_gap_list(a, d_list + 129, d_list + 128, d_list, st);
     for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&ev[i]);
     if (e == hipSuccess) e = hipEventRecord(ev[0], st);
     if (e == hipSuccess) e = hb_launch_gapped(a, false, c->n_cu * 8, st);
