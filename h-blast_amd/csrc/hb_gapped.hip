@@ -286,61 +286,100 @@ struct GapWave {
                 rows = ra <= M ? (int32_t)(REV ? arow[-ra] : arow[ra]) : 0;
             }
             const int8_t *mrow = mat + 32 * lane_read(rows, (a - 1) & 63);
-            int32_t carry = kNeg, rin = kNeg, topc = top, rowarg = 0, last = -1, adv = 0;
-            int32_t rafter = kNeg;
-            bool lead = true;
-            for (int32_t c0 = lo; c0 < end; c0 += 64) {
-                const int32_t b = c0 + lane;
+            int32_t last, adv, rafter;
+            if (end - lo <= 64) {
+                // the whole band in one step (most rows)
+                const int32_t b = lo + lane;
                 const bool valid = b < end;
                 const int32_t col = (valid && b >= 1) ? b : 1;
                 const int32_t bl = REV ? subj[so + 1 - col] : subj[so + col];
                 const int2 cell = ring[b & rmask];
                 const int32_t ob = valid ? cell.x : kNeg;
                 const int32_t ov = valid ? cell.y : kNeg;
-                const int32_t pb = from_prev_lane(ob, carry);
+                const int32_t pb = from_prev_lane(ob, kNeg);
                 const int32_t T = valid ? max(pb + mrow[bl], ov) : kNeg;
                 uint32_t uA = (uint32_t)(T + lane_ge) + kBias;
                 uint32_t uB = (uint32_t)T + kBias;
                 scan_max2(uA, uB);
                 const uint32_t eA = from_prev_lane_u(uA);
                 const uint32_t eB = from_prev_lane_u(uB);
-                const int32_t P = (int32_t)(eA - kBias);
-                const int32_t R = max(rin - lane_ge, P - goe - lane_ge + ge);
+                const int32_t R = (int32_t)(eA - kBias) - goe - lane_ge + ge;
                 const int32_t s = max(T, R);
-                const int32_t before = lane ? max(rin, (int32_t)(eB - kBias)) : kNeg;
-                const int32_t tb = max(topc, before);
+                const int32_t tb = lane ? max(top, (int32_t)(eB - kBias)) : top;
                 const bool live = valid && !(tb - s > X);
                 if (valid)
                     ring[b & rmask] =
                         live ? make_int2(s, max(s - goe, ov - ge)) : make_int2(kNeg, ov);
                 const uint64_t lm = __ballot(live);
-                const int32_t nvalid = end - c0 < 64 ? end - c0 : 64;
-                if (lead) {
-                    if (lm) {
-                        adv += (int32_t)__builtin_ctzll(lm);
-                        lead = false;
-                    } else {
-                        adv += nvalid;
+                if (!lm) break;   // every column dropped
+                adv = (int32_t)__builtin_ctzll(lm);
+                last = lo + 63 - (int32_t)__builtin_clzll(lm);
+                const int32_t cmax = (int32_t)((uint32_t)lane_read((int32_t)uB, 63) - kBias);
+                if (cmax > top) {
+                    top = cmax;
+                    abest = a;
+                    bbest = lo + (int32_t)__builtin_ctzll(__ballot(valid && s == cmax));
+                }
+                rafter = lane_read(max(T - goe, R - ge), end - lo - 1);
+            } else {
+                int32_t carry = kNeg, rin = kNeg, topc = top, rowarg = 0;
+                last = -1;
+                adv = 0;
+                rafter = kNeg;
+                bool lead = true;
+                for (int32_t c0 = lo; c0 < end; c0 += 64) {
+                    const int32_t b = c0 + lane;
+                    const bool valid = b < end;
+                    const int32_t col = (valid && b >= 1) ? b : 1;
+                    const int32_t bl = REV ? subj[so + 1 - col] : subj[so + col];
+                    const int2 cell = ring[b & rmask];
+                    const int32_t ob = valid ? cell.x : kNeg;
+                    const int32_t ov = valid ? cell.y : kNeg;
+                    const int32_t pb = from_prev_lane(ob, carry);
+                    const int32_t T = valid ? max(pb + mrow[bl], ov) : kNeg;
+                    uint32_t uA = (uint32_t)(T + lane_ge) + kBias;
+                    uint32_t uB = (uint32_t)T + kBias;
+                    scan_max2(uA, uB);
+                    const uint32_t eA = from_prev_lane_u(uA);
+                    const uint32_t eB = from_prev_lane_u(uB);
+                    const int32_t P = (int32_t)(eA - kBias);
+                    const int32_t R = max(rin - lane_ge, P - goe - lane_ge + ge);
+                    const int32_t s = max(T, R);
+                    const int32_t before = lane ? max(rin, (int32_t)(eB - kBias)) : kNeg;
+                    const int32_t tb = max(topc, before);
+                    const bool live = valid && !(tb - s > X);
+                    if (valid)
+                        ring[b & rmask] =
+                            live ? make_int2(s, max(s - goe, ov - ge)) : make_int2(kNeg, ov);
+                    const uint64_t lm = __ballot(live);
+                    const int32_t nvalid = end - c0 < 64 ? end - c0 : 64;
+                    if (lead) {
+                        if (lm) {
+                            adv += (int32_t)__builtin_ctzll(lm);
+                            lead = false;
+                        } else {
+                            adv += nvalid;
+                        }
                     }
+                    if (lm) last = c0 + 63 - (int32_t)__builtin_clzll(lm);
+                    const int32_t cmax =
+                        max(rin, (int32_t)((uint32_t)lane_read((int32_t)uB, 63) - kBias));
+                    if (cmax > topc) {
+                        topc = cmax;
+                        rowarg = c0 + (int32_t)__builtin_ctzll(__ballot(valid && s == cmax));
+                    }
+                    const int32_t rnext = max(T - goe, R - ge);   // R after column b
+                    rin = lane_read(rnext, 63);
+                    rafter = lane_read(rnext, nvalid - 1);
+                    carry = lane_read(ob, 63);
                 }
-                if (lm) last = c0 + 63 - (int32_t)__builtin_clzll(lm);
-                const int32_t cmax =
-                    max(rin, (int32_t)((uint32_t)lane_read((int32_t)uB, 63) - kBias));
-                if (cmax > topc) {
-                    topc = cmax;
-                    rowarg = c0 + (int32_t)__builtin_ctzll(__ballot(valid && s == cmax));
+                if (topc > top) {
+                    top = topc;
+                    abest = a;
+                    bbest = rowarg;
                 }
-                const int32_t rnext = max(T - goe, R - ge);
-                rin = lane_read(rnext, 63);
-                rafter = lane_read(rnext, nvalid - 1);
-                carry = lane_read(ob, 63);
+                if (lo + adv == end) break;   // every column dropped
             }
-            if (topc > top) {
-                top = topc;
-                abest = a;
-                bbest = rowarg;
-            }
-            if (lo + adv == end) break;   // every column dropped
             lo += adv;
             int32_t new_end, n_ext = 0;
             if (last < end - 1) {
@@ -350,7 +389,9 @@ struct GapWave {
                 if (rafter >= thr) {
                     n_ext = N + 1 - end;
                     if (ge > 0) {
-                        const int32_t k = (rafter - thr) / ge + 1;
+                        // no division for the usual gap extension of 1
+                        const uint32_t d = (uint32_t)(rafter - thr);
+                        const int32_t k = (int32_t)(ge == 1 ? d : d / (uint32_t)ge) + 1;
                         if (k < n_ext) n_ext = k;
                     }
                 }
