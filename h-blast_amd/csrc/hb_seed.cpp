@@ -865,6 +865,7 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
     int rc = hb_scan_wait(s);
     if (rc) return rc;
     if (n_hsp) *n_hsp = 0;
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
     const uint64_t n = s->n_out;
     const uint32_t ns = s->db->n;
     s->gap_ms = 0.f;
@@ -1094,6 +1095,74 @@ int hb_encode_protein(const char *text, int32_t len, uint8_t *out)
         out[i] = (uint8_t)(p - kStdaa);
     }
     return HB_OK;
+}
+
+int hb_fasta_parse(const char *text, uint64_t len, uint8_t *residues, uint64_t res_cap,
+                   uint64_t *offsets, uint64_t *titles, uint32_t *titles_len,
+                   uint32_t seq_cap, uint64_t *n_res, uint32_t *n_seq)
+{
+    if (!n_res || !n_seq || (len && !text)) return HB_ERR_ARG;
+    *n_res = 0;
+    *n_seq = 0;
+    // letter -> code + 1 (0: not a residue)
+    uint8_t code[256] = {0};
+    for (int i = 0; kStdaa[i]; ++i) {
+        const unsigned char ch = (unsigned char)kStdaa[i];
+        code[ch] = (uint8_t)(i + 1);
+        if (ch >= 'A' && ch <= 'Z') code[ch - 'A' + 'a'] = (uint8_t)(i + 1);
+    }
+    const bool fill = residues || offsets;
+    uint64_t nr = 0;
+    uint64_t ns = 0;
+    bool overflow = false;
+    uint64_t i = 0;
+    while (i < len) {
+        uint64_t e = i;
+        while (e < len && text[e] != '\n') ++e;   // line [i, e)
+        const char c0 = text[i];
+        if (c0 == '>') {
+            if (ns >= 0xffffffffull) return HB_ERR_CAPACITY;
+            if (fill) {
+                if (ns < seq_cap) {
+                    if (offsets) offsets[ns] = nr;
+                    uint64_t te = e;
+                    if (te > i + 1 && text[te - 1] == '\r') --te;
+                    if (titles) titles[ns] = i + 1;
+                    if (titles_len) titles_len[ns] = (uint32_t)std::min<uint64_t>(te - i - 1, 0xffffffffu);
+                } else {
+                    overflow = true;
+                }
+            }
+            ++ns;
+        } else if (c0 != ';') {
+            for (uint64_t k = i; k < e; ++k) {
+                const unsigned char ch = (unsigned char)text[k];
+                const uint8_t cd = code[ch];
+                if (cd) {
+                    if (ns == 0) {   // residues before the first '>' line
+                        *n_res = k;
+                        return HB_ERR_RESIDUE;
+                    }
+                    if (fill) {
+                        if (residues && nr < res_cap) residues[nr] = (uint8_t)(cd - 1);
+                        else overflow = true;
+                    }
+                    ++nr;
+                } else if (!(ch == ' ' || ch == '\t' || ch == '\r' || (ch >= '0' && ch <= '9'))) {
+                    *n_res = k;
+                    return HB_ERR_RESIDUE;
+                }
+            }
+        }
+        i = e + 1;
+    }
+    *n_res = nr;
+    *n_seq = (uint32_t)ns;
+    if (fill && offsets) {
+        if (ns < seq_cap + 1ull) offsets[ns] = nr;
+        else overflow = true;
+    }
+    return overflow ? HB_ERR_CAPACITY : HB_OK;
 }
 
 // Standard genetic code (NCBI translation table 1), codons indexed

@@ -52,7 +52,7 @@ EXPORTS = [
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
-    "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info",
+    "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_fasta_parse",
     "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
     "hb_batch_results", "hb_batch_info_get", "hb_batch_free",
     "hb_serve_opts_default", "hb_serve_start", "hb_serve_submit", "hb_serve_wait",
@@ -147,6 +147,8 @@ def _lib():
         L.hb_gap_params_default.restype = None
         L.hb_scan_gapped.argtypes = [vp, vp, vp, vp, vp, u64, C.POINTER(u64)]
         L.hb_scan_gapped_info.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(u32)]
+        L.hb_fasta_parse.argtypes = [C.c_char_p, u64, vp, u64, vp, vp, vp, u32,
+                                     C.POINTER(u64), C.POINTER(u32)]
         L.hb_batch_opts_default.argtypes = [vp]
         L.hb_batch_create.argtypes = [vp, vp, vp, vp, pp]
         L.hb_batch_add.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(i32)]
@@ -208,6 +210,28 @@ def encode_protein(text: str) -> np.ndarray:
     out = np.zeros(len(text), np.uint8)
     _check(_lib().hb_encode_protein(text.encode(), len(text), _ptr(out)), "encode_protein")
     return out
+
+
+def parse_fasta(text):
+    """Protein FASTA (str or bytes) -> (residues uint8, offsets uint64[n+1],
+    titles list of str), through hb_fasta_parse."""
+    if isinstance(text, str):
+        text = text.encode()
+    nr, ns = C.c_uint64(), C.c_uint32()
+    rc = _lib().hb_fasta_parse(text, len(text), None, 0, None, None, None, 0, C.byref(nr),
+                               C.byref(ns))
+    if rc:
+        raise HBError(rc, f"hb_fasta_parse (text offset {nr.value})")
+    res = np.zeros(max(nr.value, 1), np.uint8)
+    offs = np.zeros(ns.value + 1, np.uint64)
+    tpos = np.zeros(max(ns.value, 1), np.uint64)
+    tlen = np.zeros(max(ns.value, 1), np.uint32)
+    _check(_lib().hb_fasta_parse(text, len(text), _ptr(res), len(res), _ptr(offs), _ptr(tpos),
+                                 _ptr(tlen), ns.value, C.byref(nr), C.byref(ns)),
+           "hb_fasta_parse")
+    titles = [text[int(p):int(p) + int(n)].decode(errors="replace")
+              for p, n in zip(tpos[:ns.value], tlen[:ns.value])]
+    return res[:nr.value].copy(), offs, titles
 
 
 def decode_protein(codes) -> str:

@@ -148,6 +148,21 @@ int hb_blastx_translate(const char *nt, int32_t nt_len, uint8_t *out_seq,
                         int32_t *ctx_off, int32_t *ctx_len);
 /* IUPAC protein letters <-> ncbistdaa codes. */
 int hb_encode_protein(const char *text, int32_t len, uint8_t *out);
+/* Protein FASTA text -> packed database arrays for hb_db_upload, the input
+ * H-BLAST takes from NCBI's database (makeblastdb's FASTA reader, then
+ * CreateH_BLAST_database, ncbi_blast_files/functions.cpp:785-1165).  Each
+ * '>' line starts a subject; residue lines are encoded to ncbistdaa
+ * (letters case-insensitive, '*' and '-' allowed; spaces, tabs, digits and
+ * CR ignored); other characters give HB_ERR_RESIDUE with *n_res set to the
+ * text offset of the first one.  Call with residues/offsets NULL to size:
+ * *n_res and *n_seq receive the totals.  With buffers, residues holds
+ * res_cap bytes, offsets seq_cap + 1 entries (subject i is
+ * residues[offsets[i] .. offsets[i+1])), and the optional titles /
+ * titles_len seq_cap entries (each '>' line's text offset after the '>' and
+ * its length); HB_ERR_CAPACITY if they are too small (totals still set). */
+int hb_fasta_parse(const char *text, uint64_t len, uint8_t *residues, uint64_t res_cap,
+                   uint64_t *offsets, uint64_t *titles, uint32_t *titles_len,
+                   uint32_t seq_cap, uint64_t *n_res, uint32_t *n_seq);
 
 /* --- plain per-query scan ----------------------------------------------
  * hb_scan_start launches the stage on the context's stream and returns
