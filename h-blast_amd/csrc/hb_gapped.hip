@@ -657,6 +657,12 @@ __global__ __launch_bounds__(256) void gap_place_kernel(hb_gap_args A, uint32_t 
     if (bk < 64) list[base[bk] + slot] = k;
 }
 
+__global__ void count_oids_kernel(const hb_hsp *rec, uint32_t n, uint32_t *st3)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&st3[3 * (size_t)rec[i].oid + 2], 1u);
+}
+
 // One thread per subject: dense layout of the gapped HSPs and the stats.
 __global__ void gap_compact_kernel(hb_gap_args A, const uint32_t *first, hb_gapped_stats *stats,
                                    hb_gapped_hsp *dense)
@@ -686,6 +692,13 @@ hipError_t hb_launch_gap_list(const hb_gap_args &a, uint32_t *list, uint32_t *n_
     hipLaunchKernelGGL(gap_count_kernel, grid, dim3(256), 0, st, a, scratch);
     hipLaunchKernelGGL(gap_bucket_kernel, dim3(1), dim3(64), 0, st, scratch, n_list);
     hipLaunchKernelGGL(gap_place_kernel, grid, dim3(256), 0, st, a, scratch, list);
+    return hipGetLastError();
+}
+
+hipError_t hb_launch_count_oids(const hb_hsp *rec, uint32_t n, uint32_t *st3, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(count_oids_kernel, dim3((n + 255) / 256), dim3(256), 0, st, rec, n, st3);
     return hipGetLastError();
 }
 

@@ -173,6 +173,8 @@ struct hb_gap_args {
 hipError_t hb_launch_gap_list(const hb_gap_args &a, uint32_t *list, uint32_t *n_list,
                               uint32_t *scratch, hipStream_t st);
 hipError_t hb_launch_gapped(const hb_gap_args &a, bool wide, int n_blocks, hipStream_t st);
+// st3[3 * oid + 2] += 1 for each of the n records (st3 zeroed by the caller).
+hipError_t hb_launch_count_oids(const hb_hsp *rec, uint32_t n, uint32_t *st3, hipStream_t st);
 // Dense layout: stats[k] and out_dense from the sparse output and first[].
 hipError_t hb_launch_gap_compact(const hb_gap_args &a, const uint32_t *first,
                                  hb_gapped_stats *stats, hb_gapped_hsp *out_dense,

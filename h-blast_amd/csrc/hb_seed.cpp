@@ -855,26 +855,26 @@ void hb_gap_params_default(hb_gap_params *p)
     p->reserved = 0;
 }
 
-int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutoff,
-                   hb_gapped_stats *stats, hb_gapped_hsp *hsps, uint64_t cap, uint64_t *n_hsp)
+static bool gap_params_ok(const hb_gap_params *gp)
 {
-    if (!s || !gp || !gap_cutoff) return HB_ERR_ARG;
-    if (gp->gap_open < 0 || gp->gap_extend < 0 || gp->x_dropoff < 0 || gp->gap_open > 1000 ||
-        gp->gap_extend > 1000 || gp->x_dropoff > 100000)
-        return HB_ERR_ARG;
-    int rc = hb_scan_wait(s);
-    if (rc) return rc;
-    if (n_hsp) *n_hsp = 0;
-    std::lock_guard<std::mutex> lk(s->ctx->mu);
-    const uint64_t n = s->n_out;
-    const uint32_t ns = s->db->n;
-    s->gap_ms = 0.f;
-    s->gap_wide = 0;
-    if (ns == 0) return HB_OK;
-    hb_ctx *c = s->ctx;
-    const hb_query *q = s->q;
-    const int32_t n_ctx = (int32_t)q->ctx_off.size();
-    HB_CHECK(hipSetDevice(c->device));
+    return gp && gp->gap_open >= 0 && gp->gap_extend >= 0 && gp->x_dropoff >= 0 &&
+           gp->gap_open <= 1000 && gp->gap_extend <= 1000 && gp->x_dropoff <= 100000;
+}
+
+// The gapped stage on ungapped records already on the device: st3_in holds
+// 3 counters per subject (the HSP count third), rec_in the n records in any
+// order.  Lays the records out by subject, runs the main and wide passes,
+// compacts and copies out.  Runs on the context's stream; the caller holds
+// the context lock.
+static int gapped_core(hb_ctx *c, const hb_db *db, const uint8_t *d_query, const int8_t *d_mat,
+                       const std::vector<int32_t> &ctx_off, const std::vector<int32_t> &ctx_len,
+                       const int32_t *gap_cutoff, const hb_gap_params *gp, const uint32_t *st3_in,
+                       const hb_hsp *rec_in, uint64_t n, hb_gapped_stats *stats,
+                       hb_gapped_hsp *hsps, uint64_t cap, uint64_t *n_hsp, float *ms_out,
+                       uint32_t *wide_out)
+{
+    const uint32_t ns = db->n;
+    const int32_t n_ctx = (int32_t)ctx_off.size();
     // pool buffers, all returned at the end
     std::vector<std::pair<void *, size_t>> held;
     bool oom = false;
@@ -913,17 +913,17 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
     hipStream_t st = c->stream;
     std::vector<int32_t> hctx(3 * (size_t)n_ctx);
     for (int32_t i = 0; i < n_ctx; ++i) {
-        hctx[i] = q->ctx_off[i];
-        hctx[n_ctx + i] = q->ctx_len[i];
+        hctx[i] = ctx_off[i];
+        hctx[n_ctx + i] = ctx_len[i];
         hctx[2 * n_ctx + i] = gap_cutoff[i];
     }
     hb_gap_args a{};
-    a.res = s->db->d_res;
-    a.offs = s->db->d_offs;
-    a.query = q->gs.desc[0].query;
+    a.res = db->d_res;
+    a.offs = db->d_offs;
+    a.query = d_query;
     a.ctx = d_ctx;
     a.n_ctx = n_ctx;
-    a.mat32 = q->gs.d_mat;
+    a.mat32 = d_mat;
     a.gap_open = gp->gap_open;
     a.gap_extend = gp->gap_extend;
     a.x_dropoff = gp->x_dropoff;
@@ -940,7 +940,7 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
     uint32_t *d_nbig = d_scr + ns + nb;
     hipError_t e = hb_launch_fill_u32(d_nbig, 0u, 1, st);
     if (e == hipSuccess)
-        e = hb_launch_finalize(s->d_stats, ns, s->d_out, (uint32_t)n, d_ust, d_urec, d_scr, d_big,
+        e = hb_launch_finalize(st3_in, ns, rec_in, (uint32_t)n, d_ust, d_urec, d_scr, d_big,
                                kBig, d_nbig, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_ctx, hctx.data(), hctx.size() * sizeof(int32_t),
@@ -965,7 +965,7 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
         uint64_t longest = 0;
         uint32_t n_max = 1;
         for (uint32_t k : wl) {
-            longest = std::max<uint64_t>(longest, s->db->h_offs[k + 1] - s->db->h_offs[k]);
+            longest = std::max<uint64_t>(longest, db->h_offs[k + 1] - db->h_offs[k]);
             n_max = std::max(n_max, ust[k].n_hsp);
         }
         uint32_t ring = 64;
@@ -979,13 +979,13 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
         const int blocks = (int)((waves + 3) / 4);
         a.scratch = static_cast<int32_t *>(get((size_t)blocks * 4 * per_wave));
         a.n_wide = n_wide;
-        s->gap_wide = n_wide;
         if (oom) e = hipErrorOutOfMemory;
         if (e == hipSuccess) e = hb_launch_fill_u32(d_misc, 0u, 1, st);
         if (e == hipSuccess) e = hipEventRecord(ev[2], st);
         if (e == hipSuccess) e = hb_launch_gapped(a, true, blocks, st);
         if (e == hipSuccess) e = hipEventRecord(ev[3], st);
     }
+    if (wide_out) *wide_out = n_wide;
     if (e == hipSuccess) e = hb_launch_first_index(d_st3, ns, d_first, d_first + ns, st);
     if (e == hipSuccess) e = hb_launch_gap_compact(a, d_first, d_gst, d_gdense, st);
     uint32_t tail[2] = {0, 0};
@@ -999,7 +999,7 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
     float ms = 0.f, ms2 = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
     if (e == hipSuccess && n_wide) e = hipEventElapsedTime(&ms2, ev[2], ev[3]);
-    s->gap_ms = ms + ms2;
+    if (ms_out) *ms_out = ms + ms2;
     const uint64_t total = (uint64_t)tail[0] + tail[1];
     if (e == hipSuccess) {
         if (n_hsp) *n_hsp = total;
@@ -1016,10 +1016,97 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
     }
     release();
     if (e != hipSuccess) {
-        hb_note_hip_error(e, "hb_scan_gapped", __LINE__);
+        hb_note_hip_error(e, "gapped stage", __LINE__);
         return e == hipErrorOutOfMemory ? HB_ERR_NOMEM : HB_ERR_DEVICE;
     }
     return HB_OK;
+}
+
+int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutoff,
+                   hb_gapped_stats *stats, hb_gapped_hsp *hsps, uint64_t cap, uint64_t *n_hsp)
+{
+    if (!s || !gap_params_ok(gp) || !gap_cutoff) return HB_ERR_ARG;
+    int rc = hb_scan_wait(s);
+    if (rc) return rc;
+    if (n_hsp) *n_hsp = 0;
+    hb_ctx *c = s->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    s->gap_ms = 0.f;
+    s->gap_wide = 0;
+    if (s->db->n == 0) return HB_OK;
+    HB_CHECK(hipSetDevice(c->device));
+    return gapped_core(c, s->db, s->q->gs.desc[0].query, s->q->gs.d_mat, s->q->ctx_off,
+                       s->q->ctx_len, gap_cutoff, gp, s->d_stats, s->d_out, s->n_out, stats, hsps,
+                       cap, n_hsp, &s->gap_ms, &s->gap_wide);
+}
+
+int hb_gapped_search(hb_ctx *c, hb_db *db, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                     const int32_t *ctx_off, const int32_t *ctx_len, const int8_t *matrix,
+                     const hb_hsp *in, uint64_t n_in, const hb_gap_params *gp,
+                     const int32_t *gap_cutoff, hb_gapped_stats *stats, hb_gapped_hsp *hsps,
+                     uint64_t cap, uint64_t *n_hsp)
+{
+    if (!c || !db || db->ctx != c || !gap_params_ok(gp) || !gap_cutoff || (n_in && !in))
+        return HB_ERR_ARG;
+    if (n_hsp) *n_hsp = 0;
+    std::vector<int32_t> xd(n_ctx > 0 ? n_ctx : 1, 1);
+    int rc = hb_check_query(seq, len, n_ctx, ctx_off, ctx_len, xd.data());
+    if (rc) return rc;
+    if (n_in >= 0xffffffffull) return HB_ERR_UNSUPPORTED;
+    // every record must lie inside its context and subject, on its diagonal
+    for (uint64_t i = 0; i < n_in; ++i) {
+        const hb_hsp &h = in[i];
+        if (h.oid >= db->n || h.context < 0 || h.context >= n_ctx || h.len < 1) return HB_ERR_ARG;
+        const int64_t c0 = ctx_off[h.context], c1 = c0 + ctx_len[h.context];
+        const int64_t slen = (int64_t)(db->h_offs[h.oid + 1] - db->h_offs[h.oid]);
+        if (h.q_start < c0 || (int64_t)h.q_start + h.len > c1 || h.s_start < 0 ||
+            (int64_t)h.s_start + h.len > slen || h.q_off - h.s_off != h.q_start - h.s_start)
+            return HB_ERR_ARG;
+    }
+    hb_resolved_params rp;
+    hb_params p;
+    hb_params_default(&p);
+    p.matrix = matrix;
+    rc = hb_resolve_params(&p, &rp);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HB_CHECK(hipSetDevice(c->device));
+    if (db->n == 0) return HB_OK;
+    const uint32_t ns = db->n;
+    const size_t b_q = (size_t)len + 64, b_rec = std::max<size_t>(n_in, 1) * sizeof(hb_hsp);
+    const size_t b_st3 = (size_t)ns * 3 * sizeof(uint32_t);
+    auto *d_q = static_cast<uint8_t *>(hb_pool_get(c, b_q));
+    auto *d_mat = static_cast<int8_t *>(hb_pool_get(c, 1024));
+    auto *d_rec = static_cast<hb_hsp *>(hb_pool_get(c, b_rec));
+    auto *d_st3 = static_cast<uint32_t *>(hb_pool_get(c, b_st3));
+    auto release = [&]() {
+        hb_pool_put(c, d_q, b_q);
+        hb_pool_put(c, d_mat, 1024);
+        hb_pool_put(c, d_rec, b_rec);
+        hb_pool_put(c, d_st3, b_st3);
+    };
+    if (!d_q || !d_mat || !d_rec || !d_st3) {
+        release();
+        return HB_ERR_NOMEM;
+    }
+    hipStream_t st = c->stream;
+    hipError_t e = hipMemcpyAsync(d_q, seq, (size_t)len, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_mat, rp.mat32, 1024, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n_in)
+        e = hipMemcpyAsync(d_rec, in, n_in * sizeof(hb_hsp), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hb_launch_fill_u32(d_st3, 0u, 3 * ns, st);
+    if (e == hipSuccess) e = hb_launch_count_oids(d_rec, (uint32_t)n_in, d_st3, st);
+    if (e != hipSuccess) {
+        release();
+        hb_note_hip_error(e, "hb_gapped_search", __LINE__);
+        return HB_ERR_DEVICE;
+    }
+    std::vector<int32_t> co(ctx_off, ctx_off + n_ctx), cl(ctx_len, ctx_len + n_ctx);
+    rc = gapped_core(c, db, d_q, d_mat, co, cl, gap_cutoff, gp, d_st3, d_rec, n_in, stats, hsps,
+                     cap, n_hsp, nullptr, nullptr);
+    release();
+    return rc;
 }
 
 int hb_scan_gapped_info(hb_scan *s, float *ms, uint32_t *n_wide)

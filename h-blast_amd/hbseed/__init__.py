@@ -52,7 +52,8 @@ EXPORTS = [
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
-    "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_fasta_parse",
+    "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
+    "hb_fasta_parse",
     "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
     "hb_batch_results", "hb_batch_info_get", "hb_batch_free",
     "hb_serve_opts_default", "hb_serve_start", "hb_serve_submit", "hb_serve_wait",
@@ -147,6 +148,8 @@ def _lib():
         L.hb_gap_params_default.restype = None
         L.hb_scan_gapped.argtypes = [vp, vp, vp, vp, vp, u64, C.POINTER(u64)]
         L.hb_scan_gapped_info.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(u32)]
+        L.hb_gapped_search.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, vp, u64, vp, vp, vp, vp,
+                                       u64, C.POINTER(u64)]
         L.hb_fasta_parse.argtypes = [C.c_char_p, u64, vp, u64, vp, vp, vp, u32,
                                      C.POINTER(u64), C.POINTER(u32)]
         L.hb_batch_opts_default.argtypes = [vp]
@@ -534,6 +537,25 @@ def search(ctx, db, spec: QuerySpec, **kw):
             s.close()
     finally:
         q.close()
+
+
+def gapped_search(ctx, db, spec: QuerySpec, hsps, cutoffs, gap_open=DEFAULT_GAP_OPEN,
+                  gap_extend=DEFAULT_GAP_EXTEND, x_dropoff=DEFAULT_GAP_XDROP, matrix=None):
+    """Gapped stage on caller-held ungapped HSPs of one query (hb_gapped_search)
+    -> (stats[n_subjects], gapped hsps[n])."""
+    hsps = np.ascontiguousarray(hsps, HSP_DTYPE)
+    cut = np.ascontiguousarray(cutoffs, np.int32)
+    gp = (C.c_int32 * 4)(gap_open, gap_extend, x_dropoff, 0)
+    mat = None if matrix is None else np.ascontiguousarray(matrix, np.int8)
+    stats = np.zeros(db.n, GSTATS_DTYPE)
+    out = np.zeros(max(len(hsps), 1), GHSP_DTYPE)
+    n = C.c_uint64()
+    _check(_lib().hb_gapped_search(ctx.h, db.h, _ptr(spec.seq), len(spec.seq), len(spec.ctx_off),
+                                   _ptr(spec.ctx_off), _ptr(spec.ctx_len),
+                                   None if mat is None else _ptr(mat), _ptr(hsps), len(hsps),
+                                   C.byref(gp), _ptr(cut), _ptr(stats), _ptr(out), len(out),
+                                   C.byref(n)), "hb_gapped_search")
+    return stats, out[:n.value].copy()
 
 
 # ------------------------------------------------------------------ synthetic data

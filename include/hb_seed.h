@@ -246,6 +246,18 @@ int hb_scan_gapped(hb_scan *s, const hb_gap_params *gp, const int32_t *gap_cutof
 /* Device time of the last hb_scan_gapped's kernels (ms) and the number of
  * subjects it handed to the wide (global-memory) pass. */
 int hb_scan_gapped_info(hb_scan *s, float *ms, uint32_t *n_wide);
+/* The same stage on ungapped HSPs the caller holds (any order): those of
+ * hb_batch_results / hb_serve_results for one query, or NCBI's own initial
+ * hit lists.  The query is given as to hb_query_prepare (matrix NULL =
+ * BLOSUM62); in[i].context indexes its contexts and q_start / q_off are
+ * concatenated-query offsets.  Each record must lie inside its context and
+ * subject with q_off - s_off == q_start - s_start (else HB_ERR_ARG).
+ * Outputs as hb_scan_gapped; a capacity of n_in always suffices. */
+int hb_gapped_search(hb_ctx *ctx, hb_db *db, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                     const int32_t *ctx_off, const int32_t *ctx_len, const int8_t *matrix,
+                     const hb_hsp *in, uint64_t n_in, const hb_gap_params *gp,
+                     const int32_t *gap_cutoff, hb_gapped_stats *stats, hb_gapped_hsp *hsps,
+                     uint64_t cap, uint64_t *n_hsp);
 
 /* --- batched form --------------------------------------------------------
  * Many queries (BLASTP or six-frame BLASTX, mixed lengths) searched against

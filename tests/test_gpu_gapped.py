@@ -130,3 +130,75 @@ def test_gapped_no_hsps(ctx):
     res, offs = hbseed.make_db(6600, 50)
     gst, g, _, _ = run_both(ctx, hbseed.blastp_query(q), res, offs, [60])
     assert len(g) == 0 and np.all(gst["first"] == -1)
+
+
+def test_gapped_search_on_held_hsps_equals_scan_gapped(ctx):
+    """hb_gapped_search on the scan's HSPs (shuffled) equals hb_scan_gapped."""
+    q = hbseed.random_protein(8801, 700)
+    res, offs = hbseed.make_db(8800, 4000, plant_query=q, plant_frac=0.15)
+    spec = hbseed.blastp_query(q)
+    db = hbseed.Database(ctx, res, offs)
+    qq = hbseed.Query(ctx, spec)
+    try:
+        s = hbseed.Scan(ctx, db, qq)
+        _, hsps = s.results()
+        gst, g = s.gapped([60])
+        s.close()
+        perm = np.random.default_rng(1).permutation(len(hsps))
+        gst2, g2 = hbseed.gapped_search(ctx, db, spec, hsps[perm], [60])
+    finally:
+        qq.close()
+        db.close()
+    assert len(g) > 0
+    assert np.array_equal(g.view(np.int32), g2.view(np.int32))
+    assert np.array_equal(gst.view(np.int64), gst2.view(np.int64))
+
+
+def test_gapped_search_on_batched_results(ctx):
+    """The gapped stage on each query's results from the batched form equals
+    the oracle's gapped stage on that query."""
+    pq = hbseed.random_protein(8901, 900)
+    res, offs = hbseed.make_db(8900, 3000, plant_query=pq, plant_frac=0.15)
+    specs = [hbseed.blastp_query(pq[a:a + L]) for a, L in [(0, 300), (200, 500), (600, 250)]]
+    specs.append(hbseed.blastx_query(hbseed.blastx_nt(8902, 900, pq[100:180], pq[400:480])))
+    db = hbseed.Database(ctx, res, offs)
+    try:
+        b = hbseed.Batch(ctx, db, group_residues=1024)
+        ids = [b.add(sp) for sp in specs]
+        b.run()
+        got = []
+        for sp, qid in zip(specs, ids):
+            h = b.results(qid)
+            cut = np.full(sp.n_ctx, 45, np.int32)
+            got.append((h, cut) + hbseed.gapped_search(ctx, db, sp, h, cut))
+        b.close()
+    finally:
+        db.close()
+    for sp, (h, cut, gst, g) in zip(specs, got):
+        oq = oracle_lib.OracleQuery.from_spec(sp)
+        _, ref_h = oq.scan(res, offs, threads=8)
+        assert np.array_equal(h.view(np.int32), ref_h.view(np.int32))
+        cnt, ref = oracle_lib.gapped_db(oq, res, offs, ref_h, cut, threads=8)
+        assert np.array_equal(g.view(np.int32), ref.view(np.int32))
+        assert np.array_equal(gst["extensions"], cnt[:, 1])
+        assert len(ref) > 0
+
+
+def test_gapped_search_rejects_bad_records(ctx):
+    q = hbseed.random_protein(9001, 100)
+    res, offs = hbseed.make_db(9000, 20)
+    spec = hbseed.blastp_query(q)
+    db = hbseed.Database(ctx, res, offs)
+    try:
+        good = np.zeros(1, hbseed.HSP_DTYPE)
+        good[0] = (3, 0, 10, 5, 12, 7, 8, 30)
+        hbseed.gapped_search(ctx, db, spec, good, [40])
+        for field, value in [("oid", 20), ("context", 1), ("q_start", 95), ("s_off", 9),
+                             ("len", 0), ("s_start", -1)]:
+            bad = good.copy()
+            bad[field] = value
+            with pytest.raises(hbseed.HBError) as e:
+                hbseed.gapped_search(ctx, db, spec, bad, [40])
+            assert e.value.code == -1, field
+    finally:
+        db.close()
