@@ -118,6 +118,8 @@ static void make_groups(hb_batch *b, int32_t target, std::vector<hb_concat> *gro
     for (int32_t gi : gorder) {
         hb_concat G;
         ctx_base->push_back((int32_t)b->ctx_query.size());
+        bool masked = false;   // the group carries a mask if any member does
+        for (int32_t qi : members[gi]) masked |= !b->queries[qi].mask.empty();
         for (size_t k = 0; k < members[gi].size(); ++k) {
             const int32_t qi = members[gi][k];
             const hb_concat &Q = b->queries[qi];
@@ -125,6 +127,11 @@ static void make_groups(hb_batch *b, int32_t target, std::vector<hb_concat> *gro
             const int32_t base = (int32_t)G.seq.size();
             b->q_base[qi] = base;
             G.seq.insert(G.seq.end(), Q.seq.begin(), Q.seq.end());
+            if (masked) {
+                G.mask.resize((size_t)base, 0);
+                if (Q.mask.empty()) G.mask.insert(G.mask.end(), Q.seq.size(), (uint8_t)0);
+                else G.mask.insert(G.mask.end(), Q.mask.begin(), Q.mask.end());
+            }
             for (size_t c = 0; c < Q.ctx_off.size(); ++c) {
                 G.ctx_off.push_back(base + Q.ctx_off[c]);
                 G.ctx_len.push_back(Q.ctx_len[c]);
@@ -134,6 +141,7 @@ static void make_groups(hb_batch *b, int32_t target, std::vector<hb_concat> *gro
                 b->ctx_local.push_back((int32_t)c);
             }
         }
+        if (masked) G.mask.resize(G.seq.size(), 0);
         diag->push_back(((int32_t)G.seq.size() + b->p.window + 63) & ~63);
         groups->push_back(std::move(G));
     }
@@ -510,6 +518,14 @@ int hb_batch_add(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_ctx,
                  const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
                  const int32_t *cutoff, int32_t *query_id)
 {
+    return hb_batch_add_masked(b, seq, len, n_ctx, ctx_off, ctx_len, xdrop, cutoff, nullptr,
+                               query_id);
+}
+
+int hb_batch_add_masked(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                        const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
+                        const int32_t *cutoff, const uint8_t *lut_mask, int32_t *query_id)
+{
     if (!b || !cutoff) return HB_ERR_ARG;
     if (b->built) return HB_ERR_STATE;
     int rc = hb_check_query(seq, len, n_ctx, ctx_off, ctx_len, xdrop);
@@ -520,6 +536,7 @@ int hb_batch_add(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_ctx,
     q.ctx_len.assign(ctx_len, ctx_len + n_ctx);
     q.xdrop.assign(xdrop, xdrop + n_ctx);
     q.cutoff.assign(cutoff, cutoff + n_ctx);
+    if (lut_mask) q.mask.assign(lut_mask, lut_mask + len);
     b->queries.push_back(std::move(q));
     if (query_id) *query_id = (int32_t)b->queries.size() - 1;
     return HB_OK;

@@ -71,6 +71,7 @@ int hb_resolve_params(const hb_params *p, hb_resolved_params *out);
 struct hb_concat {
     std::vector<uint8_t> seq;
     std::vector<int32_t> ctx_off, ctx_len, xdrop, cutoff;
+    std::vector<uint8_t> mask;   // lookup-table mask per residue (empty: none)
 };
 // Validates contexts and residues of one query.
 int hb_check_query(const uint8_t *seq, int32_t len, int32_t n_ctx, const int32_t *ctx_off,

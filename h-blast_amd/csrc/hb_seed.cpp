@@ -224,6 +224,11 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
         for (int32_t i = 0; i < qlen; ++i) qf[i] = Q.seq[i];
         for (int32_t k = 0; k < nctx; ++k)
             for (int32_t w = Q.ctx_off[k]; w + 3 <= Q.ctx_off[k] + Q.ctx_len[k]; ++w) qf[w] |= 0x80;
+        // masked residues seed no word (NCBI builds its table over the
+        // unmasked segments only)
+        if (!Q.mask.empty())
+            for (int32_t w = 0; w + 3 <= qlen; ++w)
+                if (Q.mask[w] | Q.mask[w + 1] | Q.mask[w + 2]) qf[w] &= 0x7f;
         o += align256(qlen);
         uint32_t *total = reinterpret_cast<uint32_t *>(base + o);
         hb_group_desc &D = out->desc[g];
@@ -565,6 +570,15 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
                      const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
                      const int32_t *cutoff, const hb_params *params, hb_query **out)
 {
+    return hb_query_prepare_masked(c, seq, len, n_ctx, ctx_off, ctx_len, xdrop, cutoff, params,
+                                   nullptr, out);
+}
+
+int hb_query_prepare_masked(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                            const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
+                            const int32_t *cutoff, const hb_params *params,
+                            const uint8_t *lut_mask, hb_query **out)
+{
     if (!c || !out || !cutoff) return HB_ERR_ARG;
     *out = nullptr;
     int rc = hb_check_query(seq, len, n_ctx, ctx_off, ctx_len, xdrop);
@@ -585,6 +599,7 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
     qs[0].ctx_len.assign(ctx_len, ctx_len + n_ctx);
     qs[0].xdrop.assign(xdrop, xdrop + n_ctx);
     qs[0].cutoff.assign(cutoff, cutoff + n_ctx);
+    if (lut_mask) qs[0].mask.assign(lut_mask, lut_mask + len);
     const int32_t D = (len + q->p.window + 63) & ~63;
     if (hb_scan_fixed_lds((len + 15) & ~15, hb_ctx_pad(n_ctx)) + hb_scan_wave_lds(D, 256) > c->max_lds) {
         delete q;

@@ -50,7 +50,7 @@ EXPORTS = [
     "hb_mem_info", "hb_synchronize", "hb_db_upload", "hb_db_free", "hb_db_subjects",
     "hb_db_residues", "hb_query_prepare", "hb_query_free", "hb_query_lut_info",
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
-    "hb_encode_protein", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
+    "hb_encode_protein", "hb_query_prepare_masked", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
     "hb_fasta_parse",
@@ -131,6 +131,7 @@ def _lib():
         L.hb_db_residues.restype = u64
         L.hb_db_residues.argtypes = [vp]
         L.hb_query_prepare.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, vp, pp]
+        L.hb_query_prepare_masked.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, pp]
         L.hb_query_free.argtypes = [vp]
         L.hb_query_lut_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
         L.hb_query_lut_fetch.argtypes = [vp, vp, vp, u32]
@@ -155,6 +156,7 @@ def _lib():
         L.hb_batch_opts_default.argtypes = [vp]
         L.hb_batch_create.argtypes = [vp, vp, vp, vp, pp]
         L.hb_batch_add.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(i32)]
+        L.hb_batch_add_masked.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, vp, C.POINTER(i32)]
         L.hb_batch_run.argtypes = [vp]
         L.hb_batch_results.argtypes = [vp, i32, vp, u64, C.POINTER(u64)]
         L.hb_batch_info_get.argtypes = [vp, vp]
@@ -247,8 +249,10 @@ class QuerySpec:
     per-context X-drop/cutoff (what NCBI's BLAST_SequenceBlk, BlastQueryInfo
     and BlastInitialWordParameters carry to H_BLAST_pre)."""
 
-    def __init__(self, seq, ctx_off, ctx_len, xdrop=None, cutoff=None):
+    def __init__(self, seq, ctx_off, ctx_len, xdrop=None, cutoff=None, lut_mask=None):
         self.seq = np.ascontiguousarray(seq, dtype=np.uint8)
+        # residues that seed no lookup-table word (NCBI's masked segments)
+        self.lut_mask = None if lut_mask is None else np.ascontiguousarray(lut_mask, np.uint8)
         self.ctx_off = np.ascontiguousarray(ctx_off, dtype=np.int32)
         self.ctx_len = np.ascontiguousarray(ctx_len, dtype=np.int32)
         n = len(self.ctx_off)
@@ -342,9 +346,10 @@ class Query:
         self.spec = spec
         p, self._keep = _params_struct(threshold, window, word_size, matrix)
         h = C.c_void_p()
-        _check(_lib().hb_query_prepare(ctx.h, _ptr(spec.seq), len(spec.seq), spec.n_ctx,
-                                       _ptr(spec.ctx_off), _ptr(spec.ctx_len), _ptr(spec.xdrop),
-                                       _ptr(spec.cutoff), C.byref(p), C.byref(h)),
+        _check(_lib().hb_query_prepare_masked(ctx.h, _ptr(spec.seq), len(spec.seq), spec.n_ctx,
+                                              _ptr(spec.ctx_off), _ptr(spec.ctx_len),
+                                              _ptr(spec.xdrop), _ptr(spec.cutoff), C.byref(p),
+                                              _ptr(spec.lut_mask), C.byref(h)),
                "hb_query_prepare")
         self.h = h
 
@@ -433,9 +438,10 @@ class Batch:
 
     def add(self, spec: QuerySpec):
         qid = C.c_int32()
-        _check(_lib().hb_batch_add(self.h, _ptr(spec.seq), len(spec.seq), spec.n_ctx,
-                                   _ptr(spec.ctx_off), _ptr(spec.ctx_len), _ptr(spec.xdrop),
-                                   _ptr(spec.cutoff), C.byref(qid)), "hb_batch_add")
+        _check(_lib().hb_batch_add_masked(self.h, _ptr(spec.seq), len(spec.seq), spec.n_ctx,
+                                          _ptr(spec.ctx_off), _ptr(spec.ctx_len),
+                                          _ptr(spec.xdrop), _ptr(spec.cutoff),
+                                          _ptr(spec.lut_mask), C.byref(qid)), "hb_batch_add")
         self.specs.append(spec)
         return qid.value
 

@@ -130,6 +130,18 @@ int hb_query_prepare(hb_ctx *ctx, const uint8_t *seq, int32_t len, int32_t n_ctx
                      const int32_t *ctx_off, const int32_t *ctx_len,
                      const int32_t *xdrop, const int32_t *cutoff,
                      const hb_params *params, hb_query **out);
+/* As hb_query_prepare, with a lookup-table mask: lut_mask[i] != 0 keeps the
+ * words that cover residue i out of the lookup table (NULL: no mask).  NCBI
+ * builds its table over the unmasked segments of each context only
+ * (lookup_segments: SEG / low-complexity filtering, the default for BLASTX,
+ * or user masks); the reference copies that table (h_blast.cu:940-958).
+ * Extensions still read the residues as given (soft masking); for hard
+ * masking the caller replaces masked residues by X (21) as NCBI does. */
+int hb_query_prepare_masked(hb_ctx *ctx, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                            const int32_t *ctx_off, const int32_t *ctx_len,
+                            const int32_t *xdrop, const int32_t *cutoff,
+                            const hb_params *params, const uint8_t *lut_mask,
+                            hb_query **out);
 int hb_query_free(hb_query *q);
 /* Number of (query offset, cell) entries of the lookup table. */
 int hb_query_lut_info(const hb_query *q, uint32_t *n_entries, uint32_t *n_cells_used);
@@ -299,6 +311,10 @@ int hb_batch_create(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_ba
 int hb_batch_add(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_ctx,
                  const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
                  const int32_t *cutoff, int32_t *query_id);
+/* hb_batch_add with a lookup-table mask (as hb_query_prepare_masked). */
+int hb_batch_add_masked(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                        const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
+                        const int32_t *cutoff, const uint8_t *lut_mask, int32_t *query_id);
 /* Builds groups and lookup tables on the first call (autotuning the group
  * size if asked), then runs the scan; blocking. */
 int hb_batch_run(hb_batch *b);

@@ -133,6 +133,9 @@ int hbo_lut_build(const hbo_query *q, hbo_lut *out)
         for (int32_t k = 0; k < q->n_ctx; ++k) {
             int32_t lo = q->ctx_off[k], hi = q->ctx_off[k] + q->ctx_len[k] - 3;
             for (int32_t o = lo; o <= hi; ++o) {
+                /* NCBI indexes words of the unmasked segments only */
+                if (q->lut_mask && (q->lut_mask[o] | q->lut_mask[o + 1] | q->lut_mask[o + 2]))
+                    continue;
                 const uint8_t q0 = q->seq[o], q1 = q->seq[o + 1], q2 = q->seq[o + 2];
                 const uint32_t self = ((uint32_t)q0 << 10) | ((uint32_t)q1 << 5) | q2;
                 const int8_t *r0 = m + q0 * HBO_ALPHA, *r1 = m + q1 * HBO_ALPHA, *r2 = m + q2 * HBO_ALPHA;

@@ -52,6 +52,7 @@ typedef struct hbo_query {
     int32_t threshold;      /* neighbourhood threshold T */
     int32_t window;         /* two-hit window */
     const int8_t *matrix;   /* 28x28 row-major, or NULL for BLOSUM62 */
+    const uint8_t *lut_mask;/* len bytes, nonzero = residue seeds no word; or NULL */
 } hbo_query;
 
 typedef struct hbo_lut {

@@ -20,7 +20,7 @@ class _Query(C.Structure):
     _fields_ = [("seq", C.c_void_p), ("len", C.c_int32), ("n_ctx", C.c_int32),
                 ("ctx_off", C.c_void_p), ("ctx_len", C.c_void_p), ("xdrop", C.c_void_p),
                 ("cutoff", C.c_void_p), ("word_size", C.c_int32), ("threshold", C.c_int32),
-                ("window", C.c_int32), ("matrix", C.c_void_p)]
+                ("window", C.c_int32), ("matrix", C.c_void_p), ("lut_mask", C.c_void_p)]
 
 
 class _Lut(C.Structure):
@@ -65,17 +65,19 @@ def blosum62():
 
 class OracleQuery:
     def __init__(self, seq, ctx_off, ctx_len, xdrop, cutoff, threshold=11, window=40,
-                 matrix=None):
+                 matrix=None, lut_mask=None):
         self.arrays = [np.ascontiguousarray(seq, np.uint8),
                        np.ascontiguousarray(ctx_off, np.int32),
                        np.ascontiguousarray(ctx_len, np.int32),
                        np.ascontiguousarray(xdrop, np.int32),
                        np.ascontiguousarray(cutoff, np.int32)]
         self.matrix = None if matrix is None else np.ascontiguousarray(matrix, np.int8)
+        self.lut_mask = None if lut_mask is None else np.ascontiguousarray(lut_mask, np.uint8)
         s, o, l, x, c = self.arrays
         self.q = _Query(s.ctypes.data, len(s), len(o), o.ctypes.data, l.ctypes.data,
                         x.ctypes.data, c.ctypes.data, 3, threshold, window,
-                        None if self.matrix is None else self.matrix.ctypes.data)
+                        None if self.matrix is None else self.matrix.ctypes.data,
+                        None if self.lut_mask is None else self.lut_mask.ctypes.data)
         self.lut = _Lut()
         rc = lib().hbo_lut_build(C.byref(self.q), C.byref(self.lut))
         if rc:
@@ -83,6 +85,8 @@ class OracleQuery:
 
     @classmethod
     def from_spec(cls, spec, **kw):
+        if getattr(spec, "lut_mask", None) is not None:
+            kw.setdefault("lut_mask", spec.lut_mask)
         return cls(spec.seq, spec.ctx_off, spec.ctx_len, spec.xdrop, spec.cutoff, **kw)
 
     def lut_arrays(self):

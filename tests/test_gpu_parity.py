@@ -106,6 +106,54 @@ def test_device_lut_matches_oracle(ctx):
     assert np.array_equal(ent, oent)
 
 
+def masked_spec(spec, seed, runs=6, longest=15):
+    rng = np.random.default_rng(seed)
+    mask = np.zeros(len(spec.seq), np.uint8)
+    for a in rng.integers(0, len(spec.seq) - 3, runs):
+        mask[a:a + int(rng.integers(3, longest))] = 1
+    return hbseed.QuerySpec(spec.seq, spec.ctx_off, spec.ctx_len, spec.xdrop, spec.cutoff,
+                            lut_mask=mask)
+
+
+def test_masked_lut_and_scan_match_oracle(ctx):
+    """A lookup-table mask (NCBI's masked segments) on the device equals the
+    oracle: table entries, HSPs and counters."""
+    q = hbseed.random_protein(4101, 600)
+    spec = masked_spec(hbseed.blastp_query(q), 4102, runs=12, longest=40)
+    dq = hbseed.Query(ctx, spec)
+    try:
+        start, ent = dq.lut()
+    finally:
+        dq.close()
+    ostart, oent = oracle_lib.OracleQuery.from_spec(spec).lut_arrays()
+    assert np.array_equal(start, ostart) and np.array_equal(ent, oent)
+    res, offs = hbseed.make_db(4100, 5000, plant_query=q, plant_frac=0.1)
+    stats, hsps = compare(ctx, spec, res, offs)
+    _, unmasked = compare(ctx, hbseed.blastp_query(q), res, offs)
+    assert 0 < len(hsps) < len(unmasked)
+
+
+def test_masked_blastx_and_batch_match_oracle(ctx):
+    pq = hbseed.random_protein(4201, 800)
+    res, offs = hbseed.make_db(4200, 3000, plant_query=pq, plant_frac=0.1)
+    specs = [masked_spec(hbseed.blastx_query(hbseed.blastx_nt(4202, 1200, pq[100:200],
+                                                              pq[400:500])), 4203),
+             masked_spec(hbseed.blastp_query(pq[:500]), 4204), hbseed.blastp_query(pq[300:700])]
+    compare(ctx, specs[0], res, offs)
+    db = hbseed.Database(ctx, res, offs)
+    try:
+        b = hbseed.Batch(ctx, db, group_residues=1024)
+        ids = [b.add(sp) for sp in specs]
+        b.run()
+        got = [b.results(i) for i in ids]
+        b.close()
+    finally:
+        db.close()
+    for sp, h in zip(specs, got):
+        _, ref = oracle_lib.OracleQuery.from_spec(sp).scan(res, offs, threads=8)
+        assert np.array_equal(h.view(np.int32), ref.view(np.int32))
+
+
 def test_long_subjects_global_path(ctx):
     # subjects longer than the LDS staging buffer take the global-memory path
     q = hbseed.random_protein(31, 400)

@@ -79,12 +79,15 @@ def test_matrix_matches_reference_table():
 
 
 # ---------------------------------------------------------------- lookup table
-def brute_lut(seq, ctx_off, ctx_len, m, T):
-    """All 28^3 words scored at once against each query word (numpy)."""
+def brute_lut(seq, ctx_off, ctx_len, m, T, mask=None):
+    """All 28^3 words scored at once against each query word (numpy); a word
+    covering a masked residue is left out."""
     m = m.astype(np.int64)
     cells = {}
     for o, l in zip(ctx_off, ctx_len):
         for q in range(o, o + l - 2):
+            if mask is not None and mask[q:q + 3].any():
+                continue
             a, b, c = seq[q], seq[q + 1], seq[q + 2]
             sc = m[a][:, None, None] + m[b][None, :, None] + m[c][None, None, :]
             hit = sc >= T
@@ -106,6 +109,27 @@ def test_lut_matches_bruteforce(seed):
     exp = brute_lut(spec.seq, spec.ctx_off, spec.ctx_len, oracle_lib.blosum62(), 11)
     got = {c: list(ent[start[c]:start[c + 1]]) for c in range(32768) if start[c + 1] > start[c]}
     assert got == exp
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_masked_lut_matches_bruteforce(seed):
+    rng = np.random.default_rng(seed)
+    q = hbseed.random_protein(seed, 80)
+    mask = np.zeros(80, np.uint8)
+    for a in rng.integers(0, 75, 4):   # a few masked runs, e.g. SEG segments
+        mask[a:a + int(rng.integers(1, 9))] = 1
+    spec = hbseed.QuerySpec(q, [0, 35], [30, 45], lut_mask=mask)
+    spec.seq[30:35] = 0
+    oq = oracle_lib.OracleQuery.from_spec(spec)
+    start, ent = oq.lut_arrays()
+    exp = brute_lut(spec.seq, spec.ctx_off, spec.ctx_len, oracle_lib.blosum62(), 11, mask)
+    got = {c: list(ent[start[c]:start[c + 1]]) for c in range(32768) if start[c + 1] > start[c]}
+    assert got == exp
+    # an all-masked query seeds nothing; masking nothing changes nothing
+    assert oracle_lib.OracleQuery(q, [0], [80], [16], [41], lut_mask=np.ones(80)).lut.n_entries == 0
+    a = oracle_lib.OracleQuery(q, [0], [80], [16], [41], lut_mask=np.zeros(80)).lut_arrays()
+    b = oracle_lib.OracleQuery(q, [0], [80], [16], [41]).lut_arrays()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
 def test_self_word_below_threshold_still_seeds():
