@@ -1,0 +1,162 @@
+// Sanitizer driver (test infrastructure): random and malformed inputs through
+// the host-only C-ABI helpers (h-blast_amd/csrc/hb_seq.cpp) and the CPU
+// oracle (oracle/hbo.c, oracle/hbo_gap.c).  tests/test_sanitizers.py builds it
+// with -fsanitize=address,undefined (and the threaded oracle parts with
+// -fsanitize=thread) and fails on any report.  Exit status 0 = clean.
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "hb_seed.h"
+extern "C" {
+#include "hbo.h"
+}
+
+static int fails = 0;
+#define CHECK(c)                                                         \
+    do {                                                                 \
+        if (!(c)) {                                                      \
+            std::fprintf(stderr, "check failed: %s (line %d)\n", #c, __LINE__); \
+            ++fails;                                                     \
+        }                                                                \
+    } while (0)
+
+static void fasta(std::mt19937_64 &rng, int rounds)
+{
+    const char alphabet[] = "ACDEFGHIKLMNPQRSTVWYXBZUOJ*-acdefg \t\r0123456789>>\n\n\n;#.";
+    for (int r = 0; r < rounds; ++r) {
+        const size_t len = rng() % 600;
+        std::string t;
+        if (rng() % 2) t = ">";
+        for (size_t i = 0; i < len; ++i) t += alphabet[rng() % (sizeof(alphabet) - 1)];
+        uint64_t nr = 0;
+        uint32_t ns = 0;
+        const int rc = hb_fasta_parse(t.data(), t.size(), nullptr, 0, nullptr, nullptr, nullptr, 0,
+                                      &nr, &ns);
+        if (rc != HB_OK) {
+            CHECK(rc == HB_ERR_RESIDUE && nr < t.size());
+            continue;
+        }
+        for (int shrink = 0; shrink < 3; ++shrink) {
+            const uint64_t rc_cap = nr > (uint64_t)shrink ? nr - shrink : 0;
+            const uint32_t sc = ns > (uint32_t)shrink ? ns - shrink : 0;
+            std::vector<uint8_t> res(rc_cap);
+            std::vector<uint64_t> offs(sc + 1), tpos(sc);
+            std::vector<uint32_t> tlen(sc);
+            uint64_t nr2 = 0;
+            uint32_t ns2 = 0;
+            const int rc2 = hb_fasta_parse(t.data(), t.size(), res.data(), rc_cap, offs.data(),
+                                           tpos.data(), tlen.data(), sc, &nr2, &ns2);
+            CHECK(nr2 == nr && ns2 == ns);
+            CHECK(shrink == 0 || nr == 0 ? rc2 == HB_OK || rc2 == HB_ERR_CAPACITY
+                                         : rc2 == HB_ERR_CAPACITY);
+            if (shrink == 0) {
+                CHECK(rc2 == HB_OK);
+                for (uint32_t k = 0; k < ns; ++k)
+                    CHECK(offs[k] <= offs[k + 1] && tpos[k] + tlen[k] <= t.size());
+                for (uint64_t k = 0; k < nr; ++k) CHECK(res[k] < 28);
+            }
+        }
+    }
+}
+
+static void translate(std::mt19937_64 &rng, int rounds)
+{
+    const char nts[] = "ACGTacgtUNRY-";
+    for (int r = 0; r < rounds; ++r) {
+        const int32_t n = (int32_t)(rng() % 2000);
+        std::string nt;
+        for (int32_t i = 0; i < n; ++i) nt += nts[rng() % (sizeof(nts) - 1)];
+        std::vector<uint8_t> out((size_t)hb_blastx_concat_len(n));
+        int32_t off[6], len[6];
+        CHECK(hb_blastx_translate(nt.data(), n, out.data(), off, len) == HB_OK);
+        for (int f = 0; f < 6; ++f) CHECK(off[f] >= 0 && off[f] + len[f] <= (int32_t)out.size());
+        std::string prot;
+        for (int i = 0; i < 50; ++i) prot += "ACDXZ*-J#"[rng() % 9];
+        std::vector<uint8_t> codes(prot.size());
+        const int rc = hb_encode_protein(prot.data(), (int32_t)prot.size(), codes.data());
+        CHECK(rc == HB_OK || rc == HB_ERR_RESIDUE);
+    }
+}
+
+static void oracle(std::mt19937_64 &rng, int rounds, int threads)
+{
+    for (int r = 0; r < rounds; ++r) {
+        const int32_t qlen = 20 + (int32_t)(rng() % 300);
+        std::vector<uint8_t> q(qlen);
+        for (auto &x : q) x = (uint8_t)(1 + rng() % 27);
+        int32_t ctx_off[2] = {0, qlen / 2 + 1}, ctx_len[2] = {qlen / 2, qlen - qlen / 2 - 1};
+        q[qlen / 2] = 0;
+        int32_t xd[2] = {16, 16}, cut[2] = {30, 35};
+        std::vector<uint8_t> mask(qlen, 0);
+        for (int i = 0; i < 3; ++i) mask[rng() % qlen] = 1;
+        hbo_query Q = {q.data(), qlen, 2, ctx_off, ctx_len, xd, cut, 3, 11, 40, nullptr,
+                       (rng() % 2) ? mask.data() : nullptr};
+        hbo_lut lut;
+        CHECK(hbo_lut_build(&Q, &lut) == 0);
+        const uint32_t ns = 1 + (uint32_t)(rng() % 60);
+        std::vector<uint64_t> offs(ns + 1, 0);
+        std::vector<uint8_t> res;
+        for (uint32_t k = 0; k < ns; ++k) {
+            const size_t L = rng() % 400;
+            for (size_t i = 0; i < L; ++i) {
+                // homologous stretches from the query make HSPs
+                const bool copy = (i % 97) < 40 && (rng() % 10) < 8;
+                res.push_back(copy ? q[(i * 7 + k) % qlen] : (uint8_t)(1 + rng() % 22));
+            }
+            offs[k + 1] = res.size();
+        }
+        res.resize(res.size() + 64, 0);
+        hbo_hsp *hsps = nullptr;
+        std::vector<uint32_t> cnt(3 * ns);
+        const int64_t nh =
+            hbo_scan_db(&Q, &lut, res.data(), offs.data(), ns, threads, &hsps, cnt.data());
+        CHECK(nh >= 0);
+        if (nh >= 0) {
+            hbo_gap_params gp = {11, 1, (int32_t)(rng() % 80)};
+            hbo_ghsp *g = nullptr;
+            std::vector<int32_t> gc(2 * ns);
+            int32_t gcut[2] = {25, 30};
+            const int64_t ng =
+                hbo_gapped_db(&Q, res.data(), offs.data(), ns, hsps, (uint64_t)nh, &gp, gcut,
+                              threads, &g, gc.data());
+            CHECK(ng >= 0 && ng <= nh);
+            hbo_free(g);
+        }
+        hbo_free(hsps);
+        hbo_lut_free(&lut);
+        // the DP on random operands, both directions, clamped and huge X
+        const int32_t M = (int32_t)(rng() % 40), N = (int32_t)(rng() % 40);
+        std::vector<uint8_t> A(M + 2), B(N + 2);
+        for (auto &x : A) x = (uint8_t)(1 + rng() % 22);
+        for (auto &x : B) x = (uint8_t)(1 + rng() % 22);
+        int32_t ao, bo;
+        for (int32_t X : {0, 12, 38, 100000}) {
+            const int32_t s0 = hbo_semigapped(hbo_blosum62(), A.data(), B.data(), M, N, 0, 11, 1,
+                                              X, &ao, &bo);
+            CHECK(s0 >= 0 && ao <= M && bo <= N);
+            const int32_t s1 = hbo_semigapped(hbo_blosum62(), A.data(), B.data(), M, N, 1, 11, 1,
+                                              X, &ao, &bo);
+            CHECK(s1 >= 0 && ao <= M && bo <= N);
+        }
+    }
+}
+
+int main(int argc, char **argv)
+{
+    const bool threads_only = argc > 1 && std::strcmp(argv[1], "threads") == 0;
+    std::mt19937_64 rng(20261202);
+    if (threads_only) {
+        oracle(rng, 30, 4);
+    } else {
+        fasta(rng, 3000);
+        translate(rng, 300);
+        oracle(rng, 60, 1);
+    }
+    std::fprintf(stderr, "%s: %d failed checks\n", threads_only ? "threads" : "all", fails);
+    return fails ? 1 : 0;
+}
