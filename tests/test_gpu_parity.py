@@ -207,3 +207,20 @@ def test_subject_with_many_hsps(ctx):
     np.cumsum(lens, out=offs[1:])
     stats, hsps = compare(ctx, hbseed.blastp_query(q), res, offs)
     assert stats["n_hsp"][1] > 256 and stats["n_hsp"][2] > 0
+
+
+def test_plain_output_overflow_rerun(ctx):
+    """More HSPs than the plain path's first reservation (2^18 records for a
+    small database): the pass is rerun with the counted size and nothing is
+    dropped (H-BLAST instead flags the subject for a CPU recompute,
+    h_blast_kernel.cu:468-471)."""
+    q = hbseed.random_protein(6101, 400)
+    segs = [np.concatenate([hbseed.homolog(6200 + k, q[(29 * k) % 330:(29 * k) % 330 + 60], 0.9, 0),
+                            hbseed.random_protein(6300 + k, 9)]) for k in range(97)]
+    rng = np.random.default_rng(6102)
+    parts = [np.concatenate([segs[j] for j in rng.integers(0, 97, 24)]) for _ in range(14000)]
+    res = np.concatenate(parts).astype(np.uint8)
+    offs = np.zeros(len(parts) + 1, np.uint64)
+    np.cumsum([len(p) for p in parts], out=offs[1:])
+    stats, hsps = compare(ctx, hbseed.blastp_query(q), res, offs, threads=8)
+    assert len(hsps) > (1 << 18)
