@@ -150,9 +150,10 @@ def measure_batched(ctx, db, qs, steps, warmup, plain_sample, group_residues=0):
     ctx.synchronize()
     kms = []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         b.run()
         kms.append(b.info()["kernel_ms"])
+        log(f"batched step {k + 1}/{steps}: kernel {kms[-1]:.0f} ms")
     ctx.synchronize()
     wall = time.perf_counter() - t0
     info = b.info()
