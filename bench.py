@@ -284,6 +284,8 @@ def main():
                     help="c1: plain 1000-aa query (default); c3: batched 2,048 queries; "
                          "c4: served Poisson stream over 20M subjects")
     ap.add_argument("--queries", type=int, default=2048, help="c3: number of queries")
+    ap.add_argument("--group-residues", type=int, default=0,
+                    help="c3: query residues per group (0: autotuned)")
     ap.add_argument("--plain-sample", type=int, default=48,
                     help="c3: queries timed one by one through the plain path")
     ap.add_argument("--qps", type=float, default=4.0, help="c4: Poisson arrival rate")
@@ -438,7 +440,8 @@ def bench_c3(args):
     db = hbseed.Database(ctx, res, offs)
     log(f"{len(offs) - 1} subjects resident; {len(qs)} queries, "
         f"{sum(len(q) for q in qs)} query residues")
-    m = measure_batched(ctx, db, qs, args.steps, args.warmup, args.plain_sample)
+    m = measure_batched(ctx, db, qs, args.steps, args.warmup, args.plain_sample,
+                        group_residues=args.group_residues)
     abytes = n_res + 24 * (len(offs) - 1) + 32 * m["hsps"]
     out = {
         "metric": "queries/s (batched vs plain)",
