@@ -41,6 +41,17 @@ def test_argument_checks_need_no_device():
     assert L.hb_query_prepare(None, None, 0, 0, None, None, None, None, None, C.byref(h)) == -1
     assert L.hb_scan_start(None, None, None, C.byref(h)) == -1
     assert L.hb_scan_results(None, None, None, 0, None) == -1
+    assert L.hb_query_prepare_masked(None, None, 0, 0, None, None, None, None, None, None,
+                                     C.byref(h)) == -1
+    gp = (C.c_int32 * 4)(11, 1, 38, 0)
+    cut = (C.c_int32 * 1)(50)
+    assert L.hb_scan_gapped(None, C.byref(gp), cut, None, None, 0, None) == -1
+    assert L.hb_gapped_search(None, None, None, 0, 0, None, None, None, None, 0, C.byref(gp),
+                              cut, None, None, 0, None) == -1
+    bad = (C.c_int32 * 4)(-1, 1, 38, 0)
+    assert L.hb_gapped_search(None, None, None, 0, 0, None, None, None, None, 0, C.byref(bad),
+                              cut, None, None, 0, None) == -1
+    assert L.hb_batch_add_masked(None, None, 0, 0, None, None, None, None, None, None) == -1
     rc = L.hb_create(0, C.byref(h))
     if rc == 0:      # a GPU is present: release it
         L.hb_destroy(h)
