@@ -224,3 +224,41 @@ def test_plain_output_overflow_rerun(ctx):
     np.cumsum([len(p) for p in parts], out=offs[1:])
     stats, hsps = compare(ctx, hbseed.blastp_query(q), res, offs, threads=8)
     assert len(hsps) > (1 << 18)
+
+
+def perturbed_blosum(seed):
+    """BLOSUM62 with a random symmetric +-1 perturbation on the 27 letters
+    (the gap letter keeps -128, so sentinels still stop extensions)."""
+    m = oracle_lib.blosum62().astype(np.int16)
+    rng = np.random.default_rng(seed)
+    d = rng.integers(-1, 2, size=(28, 28))
+    d = np.triu(d) + np.triu(d, 1).T
+    d[0, :] = 0
+    d[:, 0] = 0
+    return np.clip(m + d, -128, 127).astype(np.int8)
+
+
+@pytest.mark.parametrize("threshold,window,matrix_seed", [(13, 40, None), (9, 40, None),
+                                                          (11, 16, None), (11, 120, None),
+                                                          (11, 255, None), (12, 60, 77)])
+def test_search_parameters_match_oracle(ctx, threshold, window, matrix_seed):
+    """Non-default neighbourhood threshold, two-hit window and score matrix
+    (each reaches the device: lookup build, diagonal sizing, two-hit rule,
+    extensions)."""
+    q = hbseed.random_protein(9101, 500)
+    res, offs = hbseed.make_db(9100, 3000, plant_query=q, plant_frac=0.15)
+    m = None if matrix_seed is None else perturbed_blosum(matrix_seed)
+    stats, hsps = compare(ctx, hbseed.blastp_query(q), res, offs, threshold=threshold,
+                          window=window, matrix=m)
+    assert len(hsps) > 0
+
+
+def test_per_context_xdrop_and_cutoff(ctx):
+    """BLASTX frames with their own X-drop and cutoff values."""
+    pq = hbseed.random_protein(9201, 900)
+    res, offs = hbseed.make_db(9200, 3000, plant_query=pq, plant_frac=0.15)
+    base = hbseed.blastx_query(hbseed.blastx_nt(9202, 1500, pq[100:200], pq[500:600]))
+    spec = hbseed.QuerySpec(base.seq, base.ctx_off, base.ctx_len,
+                            xdrop=[10, 16, 22, 30, 7, 16], cutoff=[35, 41, 50, 30, 41, 60])
+    stats, hsps = compare(ctx, spec, res, offs)
+    assert len(hsps) > 0
