@@ -13,8 +13,10 @@ records and per-subject counters in HBM).  The same line carries the
 preliminary gapped extension of the last pass's HSPs ("gapped": device
 kernel time, extensions, saved HSPs, the oracle on a bounded sample;
 --no-gapped skips it) and a compact batched-vs-plain and served measurement
-on the same database ("forms"; --no-forms skips it).  --config c3 / c4 run the full batched (2,048
-queries) and served (20,000,000 subjects) workloads instead.  Prints one
+on the same database ("forms"; --no-forms skips it), and the database
+loaded from an H-BLAST .hDb volume ("hdb_load"; --no-hdb skips it).
+--config c3 / c4 run the full batched (2,048 queries) and served
+(20,000,000 subjects) workloads instead.  Prints one
 JSON line.
 """
 import argparse
@@ -130,6 +132,54 @@ def measure_gapped(scan, q, res, offs, hsps, cutoff, reps=3, cpu=True):
             "kind": "port",
             "sample": f"HSPs of the first {n} of {n_total} subjects ({k} ungapped HSPs, "
                       f"{sample_ext} extensions), {dt:.2f} s wall on {threads} threads"}
+    return out
+
+
+def measure_hdb(ctx, res, offs, reps=3, cpu=True, blocksx=512, threadsx=192):
+    """The same database loaded as an H-BLAST .hDb volume (hb_db_upload_hdb):
+    the sequences sorted by length, the volume's GPU share (99.9%, a multiple
+    of 32) packed on the host in the reference's layout (default grid
+    512 x 192), then uploaded and unpacked on the device.  Reports the two
+    unpacking kernels (hipEvents), the whole call (PCIe upload included) and
+    the oracle's reader on the same file."""
+    s_res, s_offs, _ = hbseed.sort_by_length(res, offs)
+    n = hbseed.hdb_gpu_seqs(len(s_offs) - 1)
+    data, gl = hbseed.hdb_pack(s_res, s_offs, n, blocksx, threadsx)
+    n_res = int(s_offs[n])
+    # file read once, residues written, lengths written and read, offsets read
+    abytes = data.size + n_res + 16 * n
+    call, k_len, k_copy = [], [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        db = hbseed.Database.from_hdb(ctx, data, blocksx, threadsx, n, gl)
+        call.append((time.perf_counter() - t0) * 1e3)
+        a, b = db.hdb_kernel_ms()
+        k_len.append(a)
+        k_copy.append(b)
+        ok = db.n == n and db.residues == n_res
+        db.close()
+        if not ok:
+            raise RuntimeError("hb_db_upload_hdb: loaded volume has the wrong size")
+    kms = float(np.mean(k_len) + np.mean(k_copy))
+    achieved = abytes / (kms * 1e-3)
+    out = {"volume": f"{n} shortest of {len(s_offs) - 1} subjects, grid {blocksx}x{threadsx}, "
+                     f"{gl.size} groups",
+           "sequences": n, "file_bytes": int(data.size), "residues": n_res,
+           "lengths_kernel_ms": float(np.mean(k_len)), "copy_kernel_ms": float(np.mean(k_copy)),
+           "call_ms": float(np.mean(call)),
+           "call_gb_per_s": data.size / (np.mean(call) * 1e-3) / 1e9,
+           "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None}}
+    if cpu:
+        import oracle_lib
+        t0 = time.perf_counter()
+        r2, o2 = oracle_lib.hdb_unpack(data, n, blocksx, threadsx, gl)
+        dt = time.perf_counter() - t0
+        if int(o2[-1]) != n_res:
+            raise RuntimeError("oracle reader disagrees on the volume size")
+        out["cpu_baseline"] = {
+            "value": abytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"the whole {data.size}-byte volume, {dt:.2f} s on one thread"}
     return out
 
 
@@ -300,6 +350,8 @@ def main():
                     help="c1: skip the compact batched and served measurements")
     ap.add_argument("--no-gapped", action="store_true",
                     help="c1: skip the gapped-extension measurement")
+    ap.add_argument("--no-hdb", action="store_true",
+                    help="c1: skip loading the database as an .hDb volume")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; --gpus must be 1 "
@@ -378,6 +430,11 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(q, res, offs)
 
+    hdb = None
+    if not args.no_hdb:
+        log("database as an .hDb volume")
+        hdb = measure_hdb(ctx, res, offs, cpu=not args.no_cpu)
+
     # compact batched and served measurements on the same resident database
     forms = None
     if not args.no_forms:
@@ -420,6 +477,8 @@ def main():
         out["copy_calibration"] = copy
     if gapped:
         out["gapped"] = gapped
+    if hdb:
+        out["hdb_load"] = hdb
     if forms:
         out["forms"] = forms
     print(json.dumps(out), flush=True)

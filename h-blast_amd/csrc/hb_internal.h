@@ -179,3 +179,19 @@ hipError_t hb_launch_count_oids(const hb_hsp *rec, uint32_t n, uint32_t *st3, hi
 hipError_t hb_launch_gap_compact(const hb_gap_args &a, const uint32_t *first,
                                  hb_gapped_stats *stats, hb_gapped_hsp *out_dense,
                                  hipStream_t st);
+
+// Unpacking of a .hDb volume on the device (hb_hdb.hip).  Sequence c sits
+// in block B = c / threadsx (global over the volume), group g = B / blocksx;
+// its 4-byte stripe i is word (c % threadsx) + i * threadsx of the block,
+// which starts at group_base[g] + (B % blocksx) * threadsx * group_len[g].
+struct hb_hdb_args {
+    const uint8_t *hdb;            // the volume's bytes
+    const uint64_t *group_base;    // byte offset of each group
+    const int32_t *group_len;      // each group's padded length (multiple of 16)
+    uint32_t n_seqs, blocksx, threadsx;
+    uint32_t *lens;                // out: sequence lengths (last non-zero byte + 1)
+    const uint64_t *offs;          // packed offsets (copy pass)
+    uint8_t *res;                  // out: packed residues (copy pass)
+};
+hipError_t hb_launch_hdb_lengths(const hb_hdb_args &a, hipStream_t st);
+hipError_t hb_launch_hdb_copy(const hb_hdb_args &a, hipStream_t st);

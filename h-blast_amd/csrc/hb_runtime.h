@@ -56,6 +56,7 @@ struct hb_db {
     uint32_t *d_tiles = nullptr;
     uint32_t n_tiles = 0;
     std::vector<uint64_t> h_offs;
+    float hdb_len_ms = 0.f, hdb_copy_ms = 0.f;   // hb_db_upload_hdb's kernels
 };
 
 // Search parameters resolved from hb_params.

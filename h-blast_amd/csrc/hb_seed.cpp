@@ -376,6 +376,7 @@ const char *hb_strerror(int err)
     case HB_ERR_STATE: return "handle in wrong state";
     case HB_ERR_UNSUPPORTED: return "unsupported parameters";
     case HB_ERR_BUSY: return "busy";
+    case HB_ERR_FORMAT: return "malformed .hDb volume or .hDbI index";
     default: return "unknown error";
     }
 }
@@ -469,27 +470,11 @@ int hb_db_free(hb_db *db)
     return HB_OK;
 }
 
-int hb_db_upload(hb_ctx *c, const uint8_t *residues, const uint64_t *offsets, uint32_t n,
-                 hb_db **out)
+// Processing order and tiles of a database whose host offsets are set, and
+// the device copies of offsets, order and tiles.
+static int db_meta(hb_db *db)
 {
-    if (!c || !offsets || !out || (n > 0 && !residues)) return HB_ERR_ARG;
-    *out = nullptr;
-    // subject lengths are int32 on the device and diagonal state keeps
-    // position + offset below 2^31: cap a subject at 2^28 residues
-    for (uint32_t i = 0; i < n; ++i)
-        if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > (1ull << 28))
-            return HB_ERR_ARG;
-    std::lock_guard<std::mutex> lk(c->mu);
-    HB_CHECK(hipSetDevice(c->device));
-    hb_db *db = new (std::nothrow) hb_db;
-    if (!db) return HB_ERR_NOMEM;
-    db->ctx = c;
-    db->n = n;
-    const uint64_t first = n ? offsets[0] : 0;
-    db->total = n ? offsets[n] - first : 0;
-    db->h_offs.resize((size_t)n + 1);
-    for (uint32_t i = 0; i <= n; ++i) db->h_offs[i] = n ? offsets[i] - first : 0;
-
+    const uint32_t n = db->n;
     // Length-descending processing order, then tiles of <= 64 subjects and
     // ~16K residues: one queue pop per tile keeps atomics off the hot path
     // and the longest subjects start first.
@@ -513,41 +498,193 @@ int hb_db_upload(hb_ctx *c, const uint8_t *residues, const uint64_t *offsets, ui
     }
     if (tiles.back() != n) tiles.push_back(n);
     db->n_tiles = (uint32_t)tiles.size() - 1;
-
-    int rc = HB_OK;
-    uint32_t *d_flag = nullptr;
-    auto fail = [&](int r) {
-        hipFree(d_flag);
-        hb_db_free(db);
-        return r;
-    };
-    if (hipMalloc(&db->d_res, db->total + 64) != hipSuccess ||
-        hipMalloc(&db->d_offs, ((size_t)n + 1) * sizeof(uint64_t)) != hipSuccess ||
+    if (hipMalloc(&db->d_offs, ((size_t)n + 1) * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc(&db->d_order, (size_t)std::max<uint32_t>(n, 1) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&db->d_tiles, tiles.size() * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&d_flag, sizeof(uint32_t)) != hipSuccess)
-        return fail(HB_ERR_NOMEM);
-    if (hipMemcpy(db->d_res, residues + first, db->total, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(db->d_res + db->total, 0, 64) != hipSuccess ||
-        hipMemcpy(db->d_offs, db->h_offs.data(), ((size_t)n + 1) * sizeof(uint64_t),
+        hipMalloc(&db->d_tiles, tiles.size() * sizeof(uint32_t)) != hipSuccess)
+        return HB_ERR_NOMEM;
+    if (hipMemcpy(db->d_offs, db->h_offs.data(), ((size_t)n + 1) * sizeof(uint64_t),
                   hipMemcpyHostToDevice) != hipSuccess ||
         (n && hipMemcpy(db->d_order, order.data(), (size_t)n * sizeof(uint32_t),
                         hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(db->d_tiles, tiles.data(), tiles.size() * sizeof(uint32_t),
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(d_flag, 0, sizeof(uint32_t)) != hipSuccess)
-        return fail(HB_ERR_DEVICE);
-    if (hb_launch_check_residues(db->d_res, db->total, d_flag, c->stream) != hipSuccess)
-        return fail(HB_ERR_DEVICE);
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return HB_ERR_DEVICE;
+    return HB_OK;
+}
+
+// Every residue code of the uploaded database is < 28.
+static int db_check_residues(hb_db *db)
+{
+    hb_ctx *c = db->ctx;
+    uint32_t *d_flag = nullptr;
+    if (hipMalloc(&d_flag, sizeof(uint32_t)) != hipSuccess) return HB_ERR_NOMEM;
     uint32_t flag = 0;
-    if (hipMemcpyAsync(&flag, d_flag, sizeof(flag), hipMemcpyDeviceToHost, c->stream) !=
-            hipSuccess ||
+    const bool ok = hipMemsetAsync(d_flag, 0, sizeof(uint32_t), c->stream) == hipSuccess &&
+                    hb_launch_check_residues(db->d_res, db->total, d_flag, c->stream) ==
+                        hipSuccess &&
+                    hipMemcpyAsync(&flag, d_flag, sizeof(flag), hipMemcpyDeviceToHost,
+                                   c->stream) == hipSuccess &&
+                    hipStreamSynchronize(c->stream) == hipSuccess;
+    hipFree(d_flag);
+    if (!ok) return HB_ERR_DEVICE;
+    return flag ? HB_ERR_RESIDUE : HB_OK;
+}
+
+int hb_db_upload(hb_ctx *c, const uint8_t *residues, const uint64_t *offsets, uint32_t n,
+                 hb_db **out)
+{
+    if (!c || !offsets || !out || (n > 0 && !residues)) return HB_ERR_ARG;
+    *out = nullptr;
+    // subject lengths are int32 on the device and diagonal state keeps
+    // position + offset below 2^31: cap a subject at 2^28 residues
+    for (uint32_t i = 0; i < n; ++i)
+        if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > (1ull << 28))
+            return HB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HB_CHECK(hipSetDevice(c->device));
+    hb_db *db = new (std::nothrow) hb_db;
+    if (!db) return HB_ERR_NOMEM;
+    db->ctx = c;
+    db->n = n;
+    const uint64_t first = n ? offsets[0] : 0;
+    db->total = n ? offsets[n] - first : 0;
+    db->h_offs.resize((size_t)n + 1);
+    for (uint32_t i = 0; i <= n; ++i) db->h_offs[i] = n ? offsets[i] - first : 0;
+    auto fail = [&](int r) {
+        hb_db_free(db);
+        return r;
+    };
+    if (hipMalloc(&db->d_res, db->total + 64) != hipSuccess) return fail(HB_ERR_NOMEM);
+    if (hipMemcpy(db->d_res, residues + first, db->total, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(db->d_res + db->total, 0, 64) != hipSuccess)
+        return fail(HB_ERR_DEVICE);
+    int rc = db_meta(db);
+    if (rc == HB_OK) rc = db_check_residues(db);
+    if (rc != HB_OK) return fail(rc);
+    *out = db;
+    return HB_OK;
+}
+
+int hb_db_upload_hdb(hb_ctx *c, const uint8_t *hdb, uint64_t bytes, int32_t blocksx,
+                     int32_t threadsx, int32_t n_seqs, const int32_t *group_len,
+                     int32_t n_groups, hb_db **out)
+{
+    if (!c || !out || n_seqs < 0 || n_groups < 0 || blocksx < 1 || blocksx > 65536 ||
+        threadsx < 1 || threadsx > 65536 || (n_groups > 0 && !group_len) || (bytes > 0 && !hdb))
+        return HB_ERR_ARG;
+    *out = nullptr;
+    // the layout the volume must have (ncbi_blast_files/functions.cpp:844-911)
+    const uint64_t T = (uint64_t)threadsx, G = (uint64_t)blocksx * T;
+    if ((uint64_t)n_groups != ((uint64_t)n_seqs + G - 1) / G) return HB_ERR_FORMAT;
+    std::vector<uint64_t> gbase((size_t)n_groups + 1, 0);
+    uint64_t acc = 0;
+    for (int32_t g = 0; g < n_groups; ++g) {
+        const int32_t L = group_len[g];
+        if (L < 0 || L % 16 || L > (1 << 28)) return HB_ERR_FORMAT;
+        const uint64_t members = std::min<uint64_t>(G, (uint64_t)n_seqs - (uint64_t)g * G);
+        gbase[g] = acc;
+        acc += (members + T - 1) / T * T * (uint64_t)L;
+    }
+    if (acc != bytes) return HB_ERR_FORMAT;
+
+    std::lock_guard<std::mutex> lk(c->mu);
+    HB_CHECK(hipSetDevice(c->device));
+    hb_db *db = new (std::nothrow) hb_db;
+    if (!db) return HB_ERR_NOMEM;
+    db->ctx = c;
+    db->n = (uint32_t)n_seqs;
+    uint8_t *d_hdb = nullptr;
+    uint64_t *d_gbase = nullptr;
+    int32_t *d_glen = nullptr;
+    uint32_t *d_lens = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    auto release = [&]() {
+        hipFree(d_hdb);
+        hipFree(d_gbase);
+        hipFree(d_glen);
+        hipFree(d_lens);
+        for (hipEvent_t &e : ev)
+            if (e) hipEventDestroy(e);
+    };
+    auto fail = [&](int r) {
+        hipStreamSynchronize(c->stream);
+        release();
+        hb_db_free(db);
+        return r;
+    };
+    for (hipEvent_t &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(HB_ERR_DEVICE);
+    const size_t ng1 = (size_t)std::max(n_groups, 1);
+    if (hipMalloc(&d_hdb, std::max<uint64_t>(bytes, 4)) != hipSuccess ||
+        hipMalloc(&d_gbase, ng1 * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&d_glen, ng1 * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&d_lens, (size_t)std::max(n_seqs, 1) * sizeof(uint32_t)) != hipSuccess)
+        return fail(HB_ERR_NOMEM);
+    if ((bytes && hipMemcpy(d_hdb, hdb, bytes, hipMemcpyHostToDevice) != hipSuccess) ||
+        (n_groups && (hipMemcpy(d_gbase, gbase.data(), n_groups * sizeof(uint64_t),
+                                hipMemcpyHostToDevice) != hipSuccess ||
+                      hipMemcpy(d_glen, group_len, n_groups * sizeof(int32_t),
+                                hipMemcpyHostToDevice) != hipSuccess)))
+        return fail(HB_ERR_DEVICE);
+    hb_hdb_args a{};
+    a.hdb = d_hdb;
+    a.group_base = d_gbase;
+    a.group_len = d_glen;
+    a.n_seqs = (uint32_t)n_seqs;
+    a.blocksx = (uint32_t)blocksx;
+    a.threadsx = (uint32_t)threadsx;
+    a.lens = d_lens;
+    std::vector<uint32_t> lens((size_t)n_seqs);
+    if (hipEventRecord(ev[0], c->stream) != hipSuccess ||
+        hb_launch_hdb_lengths(a, c->stream) != hipSuccess ||
+        hipEventRecord(ev[1], c->stream) != hipSuccess ||
+        (n_seqs && hipMemcpyAsync(lens.data(), d_lens, lens.size() * sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(HB_ERR_DEVICE);
-    if (flag) return fail(HB_ERR_RESIDUE);
-    hipFree(d_flag);
+    db->h_offs.resize((size_t)n_seqs + 1);
+    db->h_offs[0] = 0;
+    for (int32_t i = 0; i < n_seqs; ++i) db->h_offs[i + 1] = db->h_offs[i] + lens[i];
+    db->total = db->h_offs[n_seqs];
+    if (hipMalloc(&db->d_res, db->total + 64) != hipSuccess) return fail(HB_ERR_NOMEM);
+    int rc = db_meta(db);
+    if (rc != HB_OK) return fail(rc);
+    a.offs = db->d_offs;
+    a.res = db->d_res;
+    if (hipMemsetAsync(db->d_res + db->total, 0, 64, c->stream) != hipSuccess ||
+        hipEventRecord(ev[2], c->stream) != hipSuccess ||
+        hb_launch_hdb_copy(a, c->stream) != hipSuccess ||
+        hipEventRecord(ev[3], c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(HB_ERR_DEVICE);
+    hipEventElapsedTime(&db->hdb_len_ms, ev[0], ev[1]);
+    hipEventElapsedTime(&db->hdb_copy_ms, ev[2], ev[3]);
+    rc = db_check_residues(db);
+    if (rc != HB_OK) return fail(rc);
+    release();
     *out = db;
-    return rc;
+    return HB_OK;
+}
+
+int hb_db_fetch(const hb_db *db, uint8_t *residues, uint64_t res_cap, uint64_t *offsets)
+{
+    if (!db || !offsets || (db->total > 0 && !residues)) return HB_ERR_ARG;
+    if (res_cap < db->total) return HB_ERR_CAPACITY;
+    hb_ctx *c = db->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HB_CHECK(hipSetDevice(c->device));
+    std::memcpy(offsets, db->h_offs.data(), db->h_offs.size() * sizeof(uint64_t));
+    if (db->total && hipMemcpy(residues, db->d_res, db->total, hipMemcpyDeviceToHost) != hipSuccess)
+        return HB_ERR_DEVICE;
+    return HB_OK;
+}
+
+int hb_db_hdb_info(const hb_db *db, float *lengths_ms, float *copy_ms)
+{
+    if (!db || !lengths_ms || !copy_ms) return HB_ERR_ARG;
+    *lengths_ms = db->hdb_len_ms;
+    *copy_ms = db->hdb_copy_ms;
+    return HB_OK;
 }
 
 uint32_t hb_db_subjects(const hb_db *db) { return db ? db->n : 0; }

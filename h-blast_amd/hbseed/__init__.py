@@ -25,6 +25,7 @@ HB_OK = 0
 ERRORS = {
     -1: "HB_ERR_ARG", -2: "HB_ERR_NOMEM", -3: "HB_ERR_DEVICE", -4: "HB_ERR_CAPACITY",
     -5: "HB_ERR_RESIDUE", -6: "HB_ERR_STATE", -7: "HB_ERR_UNSUPPORTED", -8: "HB_ERR_BUSY",
+    -9: "HB_ERR_FORMAT",
 }
 DEFAULT_XDROP = 16
 DEFAULT_CUTOFF = 41
@@ -53,7 +54,8 @@ EXPORTS = [
     "hb_encode_protein", "hb_query_prepare_masked", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
-    "hb_fasta_parse",
+    "hb_fasta_parse", "hb_hdb_gpu_seqs", "hb_hdb_layout", "hb_hdb_pack", "hb_hdbi_format",
+    "hb_hdbi_parse", "hb_db_upload_hdb", "hb_db_hdb_info", "hb_db_fetch",
     "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
     "hb_batch_results", "hb_batch_info_get", "hb_batch_free",
     "hb_serve_opts_default", "hb_serve_start", "hb_serve_submit", "hb_serve_wait",
@@ -93,6 +95,19 @@ class _ServeStats(C.Structure):
                 ("reruns", C.c_int32), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("mean_ms", C.c_double), ("max_ms", C.c_double), ("gpu_ms", C.c_double),
                 ("wall_ms", C.c_double)]
+
+
+class _HdbVolume(C.Structure):
+    _fields_ = [("name", C.c_void_p), ("name_len", C.c_uint32), ("bytes", C.c_uint64),
+                ("volume_seqs", C.c_int32), ("gpu_seqs", C.c_int32),
+                ("store_limit", C.c_int32), ("break_limit", C.c_int32),
+                ("n_groups", C.c_int32), ("group_len", C.c_void_p),
+                ("cpu_rest_bytes", C.c_uint64)]
+
+
+class _HdbIndex(C.Structure):
+    _fields_ = [("blocksx", C.c_int32), ("threadsx", C.c_int32), ("percentage", C.c_int32),
+                ("n_volumes", C.c_int32)]
 
 
 class _Params(C.Structure):
@@ -153,6 +168,16 @@ def _lib():
                                        u64, C.POINTER(u64)]
         L.hb_fasta_parse.argtypes = [C.c_char_p, u64, vp, u64, vp, vp, vp, u32,
                                      C.POINTER(u64), C.POINTER(u32)]
+        L.hb_hdb_gpu_seqs.restype = i32
+        L.hb_hdb_gpu_seqs.argtypes = [i32]
+        L.hb_hdb_layout.argtypes = [vp, i32, i32, i32, vp, i32, C.POINTER(i32), C.POINTER(u64)]
+        L.hb_hdb_pack.argtypes = [vp, vp, i32, i32, i32, vp, i32, vp, u64]
+        L.hb_hdbi_format.argtypes = [vp, vp, vp, u64, C.POINTER(u64)]
+        L.hb_hdbi_parse.argtypes = [C.c_char_p, u64, vp, vp, i32, vp, u64, C.POINTER(u64),
+                                    C.POINTER(u64)]
+        L.hb_db_upload_hdb.argtypes = [vp, vp, u64, i32, i32, i32, vp, i32, pp]
+        L.hb_db_hdb_info.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.hb_db_fetch.argtypes = [vp, vp, u64, vp]
         L.hb_batch_opts_default.argtypes = [vp]
         L.hb_batch_create.argtypes = [vp, vp, vp, vp, pp]
         L.hb_batch_add.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(i32)]
@@ -310,8 +335,17 @@ class Context:
 
 
 class Database:
-    def __init__(self, ctx: Context, residues: np.ndarray, offsets: np.ndarray):
+    """A resident subject database: packed arrays (hb_db_upload) or an H-BLAST
+    .hDb volume unpacked on the device (Database.from_hdb)."""
+
+    def __init__(self, ctx: Context, residues: np.ndarray = None, offsets: np.ndarray = None,
+                 _handle=None):
         self.ctx = ctx
+        self.oid_base = 0
+        if _handle is not None:
+            self.h = _handle
+            self.n = _lib().hb_db_subjects(_handle)
+            return
         residues = np.ascontiguousarray(residues, dtype=np.uint8)
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         self.n = len(offsets) - 1
@@ -320,14 +354,167 @@ class Database:
                "hb_db_upload")
         self.h = h
 
+    @classmethod
+    def from_hdb(cls, ctx: Context, hdb, blocksx, threadsx, n_seqs, group_len):
+        """One .hDb volume (bytes or uint8 array) -> resident database of its
+        n_seqs sequences, unpacked on the device (hb_db_upload_hdb)."""
+        buf = np.frombuffer(hdb, np.uint8) if isinstance(hdb, (bytes, bytearray)) else \
+            np.ascontiguousarray(hdb, dtype=np.uint8)
+        gl = np.ascontiguousarray(group_len, dtype=np.int32)
+        h = C.c_void_p()
+        _check(_lib().hb_db_upload_hdb(ctx.h, _ptr(buf), buf.size, blocksx, threadsx, n_seqs,
+                                       _ptr(gl), gl.size, C.byref(h)), "hb_db_upload_hdb")
+        return cls(ctx, _handle=h)
+
+    def hdb_kernel_ms(self):
+        """(lengths, copy) kernel times of the .hDb unpacking, in ms."""
+        a, b = C.c_float(), C.c_float()
+        _check(_lib().hb_db_hdb_info(self.h, C.byref(a), C.byref(b)), "hb_db_hdb_info")
+        return a.value, b.value
+
     @property
     def residues(self):
         return _lib().hb_db_residues(self.h)
+
+    def fetch(self):
+        """(residues, offsets) of the resident database (hb_db_fetch)."""
+        total = _lib().hb_db_residues(self.h)
+        res = np.zeros(max(total, 1), np.uint8)
+        offs = np.zeros(self.n + 1, np.uint64)
+        _check(_lib().hb_db_fetch(self.h, _ptr(res), res.size, _ptr(offs)), "hb_db_fetch")
+        return res[:total], offs
 
     def close(self):
         if self.h:
             _lib().hb_db_free(self.h)
             self.h = None
+
+
+def hdb_gpu_seqs(volume_seqs):
+    """Sequences of a volume that go to its .hDb (the rest stay on the CPU)."""
+    return _lib().hb_hdb_gpu_seqs(int(volume_seqs))
+
+
+def hdb_layout(offsets, n_seqs, blocksx=512, threadsx=192):
+    """(group lengths int32, file size) of a .hDb holding sequences 0..n_seqs-1."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    ng, nb = C.c_int32(), C.c_uint64()
+    _check(_lib().hb_hdb_layout(_ptr(offsets), n_seqs, blocksx, threadsx, None, 0,
+                                C.byref(ng), C.byref(nb)), "hb_hdb_layout")
+    gl = np.zeros(max(ng.value, 1), np.int32)
+    _check(_lib().hb_hdb_layout(_ptr(offsets), n_seqs, blocksx, threadsx, _ptr(gl), gl.size,
+                                C.byref(ng), C.byref(nb)), "hb_hdb_layout")
+    return gl[:ng.value].copy(), nb.value
+
+
+def hdb_pack(residues, offsets, n_seqs, blocksx=512, threadsx=192):
+    """.hDb bytes (uint8 array) and group lengths of sequences 0..n_seqs-1."""
+    residues = np.ascontiguousarray(residues, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    gl, nb = hdb_layout(offsets, n_seqs, blocksx, threadsx)
+    out = np.zeros(max(nb, 1), np.uint8)
+    _check(_lib().hb_hdb_pack(_ptr(residues), _ptr(offsets), n_seqs, blocksx, threadsx,
+                              _ptr(gl), gl.size, _ptr(out), nb), "hb_hdb_pack")
+    return out[:nb], gl
+
+
+def hdbi_format(blocksx, threadsx, volumes, percentage=99):
+    """.hDbI text for volumes (dicts with name, bytes, volume_seqs, gpu_seqs,
+    store_limit, break_limit, group_len, cpu_rest_bytes)."""
+    idx = _HdbIndex(blocksx, threadsx, percentage, len(volumes))
+    arr = (_HdbVolume * max(len(volumes), 1))()
+    keep = []
+    for i, v in enumerate(volumes):
+        name = v["name"].encode()
+        gl = np.ascontiguousarray(v["group_len"], dtype=np.int32)
+        keep += [name, gl]
+        arr[i] = _HdbVolume(C.cast(C.c_char_p(name), C.c_void_p), len(name), v["bytes"],
+                            v["volume_seqs"], v["gpu_seqs"], v["store_limit"], v["break_limit"],
+                            gl.size, _ptr(gl), v["cpu_rest_bytes"])
+    n = C.c_uint64()
+    _check(_lib().hb_hdbi_format(C.byref(idx), arr, None, 0, C.byref(n)), "hb_hdbi_format")
+    buf = C.create_string_buffer(n.value + 1)
+    _check(_lib().hb_hdbi_format(C.byref(idx), arr, buf, n.value, C.byref(n)),
+           "hb_hdbi_format")
+    return buf.raw[:n.value].decode()
+
+
+def hdbi_parse(text):
+    """.hDbI text -> dict(blocksx, threadsx, percentage, volumes=[dict ...])."""
+    if isinstance(text, str):
+        text = text.encode()
+    idx = _HdbIndex()
+    ngl, eoff = C.c_uint64(), C.c_uint64()
+    rc = _lib().hb_hdbi_parse(text, len(text), C.byref(idx), None, 0, None, 0, C.byref(ngl),
+                              C.byref(eoff))
+    if rc:
+        raise HBError(rc, f"hb_hdbi_parse (text offset {eoff.value})")
+    vols = (_HdbVolume * max(idx.n_volumes, 1))()
+    gl = np.zeros(max(ngl.value, 1), np.int32)
+    _check(_lib().hb_hdbi_parse(text, len(text), C.byref(idx), vols, idx.n_volumes, _ptr(gl),
+                                gl.size, C.byref(ngl), C.byref(eoff)), "hb_hdbi_parse")
+    base = C.cast(C.c_char_p(text), C.c_void_p).value
+    out, g0 = [], 0
+    for i in range(idx.n_volumes):
+        v = vols[i]
+        off = v.name - base
+        out.append(dict(name=text[off:off + v.name_len].decode(), bytes=v.bytes,
+                        volume_seqs=v.volume_seqs, gpu_seqs=v.gpu_seqs,
+                        store_limit=v.store_limit, break_limit=v.break_limit,
+                        group_len=gl[g0:g0 + v.n_groups].copy(),
+                        cpu_rest_bytes=v.cpu_rest_bytes))
+        g0 += v.n_groups
+    return dict(blocksx=idx.blocksx, threadsx=idx.threadsx, percentage=idx.percentage,
+                volumes=out)
+
+
+def write_hdb(prefix, residues, offsets, volume_seqs=None, blocksx=512, threadsx=192):
+    """Writes an H-BLAST GPU database as CreateH_BLAST_database lays it out:
+    volume files <name>.hDb (one volume) or <name>.NN.hDb, and the index
+    <prefix>.hDbI.  Sequences must ascend in length inside each volume;
+    volume_seqs lists the volumes' sizes (default: one volume).  Returns the
+    index dict (as hdbi_parse gives it)."""
+    residues = np.ascontiguousarray(residues, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    n = len(offsets) - 1
+    volume_seqs = [n] if volume_seqs is None else list(volume_seqs)
+    if sum(volume_seqs) != n:
+        raise ValueError("volume sizes must add up to the number of sequences")
+    d, stem = os.path.split(prefix)
+    vols, first = [], 0
+    for k, vs in enumerate(volume_seqs):
+        o = offsets[first:first + vs + 1] - offsets[first]
+        r = residues[int(offsets[first]):int(offsets[first + vs])]
+        gs = hdb_gpu_seqs(vs)
+        data, gl = hdb_pack(r, o, gs, blocksx, threadsx)
+        name = stem + (f".{k:02d}" if len(volume_seqs) > 1 else "") + ".hDb"
+        with open(os.path.join(d, name), "wb") as f:
+            f.write(data.tobytes())
+        vols.append(dict(name=name, bytes=int(data.size), volume_seqs=vs, gpu_seqs=gs,
+                         store_limit=first + gs, break_limit=first + vs, group_len=gl,
+                         cpu_rest_bytes=int(o[vs] - o[gs])))
+        first += vs
+    text = hdbi_format(blocksx, threadsx, vols)
+    with open(prefix + ".hDbI", "w") as f:
+        f.write(text)
+    return hdbi_parse(text)
+
+
+def open_hdb(ctx: Context, index_path):
+    """Loads every volume an .hDbI lists (files beside the index) -> list of
+    Database, each with oid_base = its first sequence's database oid."""
+    with open(index_path, "rb") as f:
+        idx = hdbi_parse(f.read())
+    d = os.path.dirname(index_path)
+    out = []
+    for v in idx["volumes"]:
+        with open(os.path.join(d, v["name"]), "rb") as f:
+            data = f.read()
+        db = Database.from_hdb(ctx, data, idx["blocksx"], idx["threadsx"], v["gpu_seqs"],
+                               v["group_len"])
+        db.oid_base = v["break_limit"] - v["volume_seqs"]
+        out.append(db)
+    return out
 
 
 def _params_struct(threshold=11, window=40, word_size=3, matrix=None):
@@ -589,6 +776,20 @@ def make_db(seed, n, median=300.0, sigma=0.6, lo=30, hi=5000, plant_query=None,
     _synth().hbs_db_fill(seed, n, _ptr(offs), _ptr(res), _ptr(pq),
                          0 if pq is None else len(pq), plant_frac, threads)
     return res, offs
+
+
+def sort_by_length(residues, offsets):
+    """Packed database -> (residues, offsets, order) with the sequences in
+    ascending length order (stable), as an .hDb volume needs them."""
+    offsets = np.asarray(offsets, np.uint64)
+    lens = np.diff(offsets.astype(np.int64))
+    order = np.argsort(lens, kind="stable")
+    s_offs = np.zeros(len(order) + 1, np.uint64)
+    np.cumsum(lens[order], out=s_offs[1:])
+    starts = offsets[:-1].astype(np.int64)
+    s_res = np.concatenate([residues[starts[i]:starts[i] + lens[i]] for i in order]) \
+        if len(order) else np.zeros(0, np.uint8)
+    return s_res.astype(np.uint8, copy=False), s_offs, order
 
 
 def homolog(seed, src, identity, n_indel):

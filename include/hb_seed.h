@@ -42,6 +42,7 @@ extern "C" {
 #define HB_ERR_STATE (-6)        /* handle used in the wrong state */
 #define HB_ERR_UNSUPPORTED (-7)  /* e.g. word size != 3 */
 #define HB_ERR_BUSY (-8)         /* serving queue full / not admitted */
+#define HB_ERR_FORMAT (-9)       /* malformed .hDb volume or .hDbI index */
 
 typedef struct hb_ctx hb_ctx;
 typedef struct hb_db hb_db;
@@ -116,6 +117,10 @@ int hb_db_upload(hb_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
 int hb_db_free(hb_db *db);
 uint32_t hb_db_subjects(const hb_db *db);
 uint64_t hb_db_residues(const hb_db *db);
+/* Copies the resident database back: residues (res_cap bytes, at least
+ * hb_db_residues()) and offsets (hb_db_subjects() + 1 entries); for tests
+ * and for writing a loaded volume out again. */
+int hb_db_fetch(const hb_db *db, uint8_t *residues, uint64_t res_cap, uint64_t *offsets);
 
 /* --- query ---------------------------------------------------------------
  * Builds the neighbourhood lookup table, presence vector and query tables
@@ -175,6 +180,78 @@ int hb_encode_protein(const char *text, int32_t len, uint8_t *out);
 int hb_fasta_parse(const char *text, uint64_t len, uint8_t *residues, uint64_t res_cap,
                    uint64_t *offsets, uint64_t *titles, uint32_t *titles_len,
                    uint32_t seq_cap, uint64_t *n_res, uint32_t *n_seq);
+
+/* --- H-BLAST GPU database volumes (.hDb) and their index (.hDbI) --------
+ * CreateH_BLAST_database (ncbi_blast_files/functions.cpp:785-1165) turns
+ * each length-sorted NCBI volume into one .hDb file holding the volume's
+ * shortest hb_hdb_gpu_seqs(n) sequences (the rest stay with NCBI's CPU
+ * path): groups of blocksx * threadsx sequences, each group padded to its
+ * longest member rounded up to 16 residues; inside a group, blocks of
+ * threadsx sequences whose residues are interleaved in 4-byte stripes
+ * (residues 4i..4i+3 of the block's t-th sequence at byte
+ * (t + i * threadsx) * 4 of the block), zero-padded.  The file does not
+ * store sequence lengths: a sequence ends after its last non-zero byte (a
+ * trailing gap code 0 is not recovered).  The .hDbI text index
+ * (Write_GPU_information_file*, functions.cpp:257-309) records the grid
+ * and, per volume, the file, its size, counts and group lengths.  The
+ * reference reads it with Read_GPU_DB_information_file*
+ * (functions.cpp:312-446) and the file with Read_H_BLAST_GPU_DB
+ * (H-BLAST/h_blast.cu:587-653). */
+typedef struct hb_hdb_volume {
+    const char *name;         /* .hDb file name; hb_hdbi_parse points into its text */
+    uint32_t name_len;
+    uint64_t bytes;           /* .hDb size */
+    int32_t volume_seqs;      /* sequences in the volume */
+    int32_t gpu_seqs;         /* sequences in the .hDb (the volume's first, shortest ones) */
+    int32_t store_limit;      /* database oid after the last one in the .hDb */
+    int32_t break_limit;      /* database oid after the volume's last one */
+    int32_t n_groups;
+    const int32_t *group_len; /* n_groups entries */
+    uint64_t cpu_rest_bytes;  /* residues of the sequences left to the CPU */
+} hb_hdb_volume;
+typedef struct hb_hdb_index {
+    int32_t blocksx, threadsx;  /* grid the volumes are laid out for */
+    int32_t percentage;         /* as written (99) */
+    int32_t n_volumes;
+} hb_hdb_index;
+
+/* Sequences of an n-sequence volume that go to its .hDb:
+ * floor(floor(99.9% of n) / 32) * 32 (functions.cpp:842-843). */
+int32_t hb_hdb_gpu_seqs(int32_t volume_seqs);
+/* Layout of a .hDb holding sequences 0..n_seqs-1 of the packed arrays
+ * (ascending length, else HB_ERR_ARG; functions.cpp:844-911): *n_groups =
+ * ceil(n_seqs / (blocksx * threadsx)), group_len receives that many entries
+ * (group_cap of room; NULL to size), *bytes the file size. */
+int hb_hdb_layout(const uint64_t *offsets, int32_t n_seqs, int32_t blocksx, int32_t threadsx,
+                  int32_t *group_len, int32_t group_cap, int32_t *n_groups, uint64_t *bytes);
+/* Writes the .hDb bytes of that layout (functions.cpp:1009-1073); out_bytes
+ * must equal the layout's size. */
+int hb_hdb_pack(const uint8_t *residues, const uint64_t *offsets, int32_t n_seqs,
+                int32_t blocksx, int32_t threadsx, const int32_t *group_len, int32_t n_groups,
+                uint8_t *out, uint64_t out_bytes);
+/* Writes the .hDbI text (names are NUL-terminated or name_len long); *len
+ * receives its length.  text NULL sizes; HB_ERR_CAPACITY if cap is short. */
+int hb_hdbi_format(const hb_hdb_index *index, const hb_hdb_volume *vols, char *text,
+                   uint64_t cap, uint64_t *len);
+/* Parses a .hDbI text.  index gets the header; with vols (vol_cap entries)
+ * and group_len (group_cap entries) also the volumes, whose group_len
+ * pointers point into group_len and names into text.  NULL arrays size:
+ * *n_group_len receives the total of group lengths.  HB_ERR_FORMAT for a
+ * malformed index, with *err_off the text offset of the offending line. */
+int hb_hdbi_parse(const char *text, uint64_t len, hb_hdb_index *index, hb_hdb_volume *vols,
+                  int32_t vol_cap, int32_t *group_len, uint64_t group_cap,
+                  uint64_t *n_group_len, uint64_t *err_off);
+/* Uploads one .hDb volume (host bytes) and unpacks it on the device into a
+ * resident database: subject i is the volume's i-th sequence (database oid
+ * = the volume's first oid + i).  Replaces Read_H_BLAST_GPU_DB plus the
+ * per-call upload (H-BLAST/h_blast.cu:587-653, 1094-1096).  HB_ERR_FORMAT
+ * if bytes, group lengths (multiples of 16) and counts disagree. */
+int hb_db_upload_hdb(hb_ctx *ctx, const uint8_t *hdb, uint64_t bytes, int32_t blocksx,
+                     int32_t threadsx, int32_t n_seqs, const int32_t *group_len,
+                     int32_t n_groups, hb_db **out);
+/* Device time of the last hb_db_upload_hdb's unpacking kernels (lengths,
+ * residues), in ms. */
+int hb_db_hdb_info(const hb_db *db, float *lengths_ms, float *copy_ms);
 
 /* --- plain per-query scan ----------------------------------------------
  * hb_scan_start launches the stage on the context's stream and returns

@@ -141,6 +141,39 @@ int64_t hbo_gapped_db(const hbo_query *q, const uint8_t *residues, const uint64_
                       const hbo_gap_params *gp, const int32_t *gap_cutoff, int n_threads,
                       hbo_ghsp **out, int32_t *counts);
 
+/* --- H-BLAST GPU database volume (.hDb) and index (.hDbI), hbo_hdb.c --- */
+typedef struct hbo_hdb_volume {
+    const char *name;         /* .hDb file name */
+    uint64_t bytes;           /* .hDb size */
+    int32_t volume_seqs;      /* sequences in the volume */
+    int32_t gpu_seqs;         /* sequences in the .hDb (the shortest ones) */
+    int32_t store_limit;      /* oid after the last one in the .hDb */
+    int32_t break_limit;      /* oid after the volume's last one */
+    int32_t n_groups;
+    const int32_t *group_len; /* n_groups entries */
+    uint64_t cpu_rest_bytes;  /* residues of the sequences left to the CPU */
+} hbo_hdb_volume;
+
+/* Sequences of a volume that go to the .hDb. */
+int32_t hbo_hdb_gpu_seqs(int32_t volume_seqs);
+/* Group lengths (ceil(n_seqs / (blocksx*threadsx)) entries) and file size
+ * of a .hDb holding sequences 0..n_seqs-1 (ascending length). */
+int hbo_hdb_layout(const uint64_t *offsets, int32_t n_seqs, int32_t blocksx, int32_t threadsx,
+                   int32_t *group_len, uint64_t *bytes);
+/* Writes the .hDb bytes (out_bytes from hbo_hdb_layout). */
+int hbo_hdb_pack(const uint8_t *residues, const uint64_t *offsets, int32_t n_seqs,
+                 int32_t blocksx, int32_t threadsx, const int32_t *group_len, uint8_t *out,
+                 uint64_t out_bytes);
+/* Reads a .hDb back: offsets gets n_seqs + 1 entries; residues (NULL to
+ * size) the sequences, each ending after its last non-zero byte. */
+int hbo_hdb_unpack(const uint8_t *hdb, uint64_t bytes, int32_t n_seqs, int32_t blocksx,
+                   int32_t threadsx, const int32_t *group_len, uint8_t *residues,
+                   uint64_t *offsets);
+/* The .hDbI text; returns its length (text NULL or cap too small: only
+ * the length). */
+int64_t hbo_hdbi_format(int32_t blocksx, int32_t threadsx, int32_t percentage, int32_t n_volumes,
+                        const hbo_hdb_volume *vols, char *text, uint64_t cap);
+
 #define HBO_ERR_ARG (-1)
 #define HBO_ERR_NOMEM (-2)
 #define HBO_ERR_RESIDUE (-3)

@@ -233,3 +233,92 @@ def gapped_db(oq, residues, offsets, hsps, cutoffs, gap_open=11, gap_extend=1, x
         C.memmove(recs.ctypes.data, out.value, k * GHSP_DTYPE.itemsize)
     lib().hbo_free(out)
     return counts[:2 * n].reshape(n, 2), recs
+
+
+class _HdbVolume(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("bytes", C.c_uint64), ("volume_seqs", C.c_int32),
+                ("gpu_seqs", C.c_int32), ("store_limit", C.c_int32),
+                ("break_limit", C.c_int32), ("n_groups", C.c_int32),
+                ("group_len", C.c_void_p), ("cpu_rest_bytes", C.c_uint64)]
+
+
+def _hdb_lib():
+    L = lib()
+    if not getattr(L, "_hdb_ready", False):
+        i32, vp, u64 = C.c_int32, C.c_void_p, C.c_uint64
+        L.hbo_hdb_gpu_seqs.restype = i32
+        L.hbo_hdb_gpu_seqs.argtypes = [i32]
+        L.hbo_hdb_layout.restype = C.c_int
+        L.hbo_hdb_layout.argtypes = [vp, i32, i32, i32, vp, C.POINTER(u64)]
+        L.hbo_hdb_pack.restype = C.c_int
+        L.hbo_hdb_pack.argtypes = [vp, vp, i32, i32, i32, vp, vp, u64]
+        L.hbo_hdb_unpack.restype = C.c_int
+        L.hbo_hdb_unpack.argtypes = [vp, u64, i32, i32, i32, vp, vp, vp]
+        L.hbo_hdbi_format.restype = C.c_int64
+        L.hbo_hdbi_format.argtypes = [i32, i32, i32, i32, vp, C.c_char_p, u64]
+        L._hdb_ready = True
+    return L
+
+
+def hdb_gpu_seqs(n):
+    return _hdb_lib().hbo_hdb_gpu_seqs(int(n))
+
+
+def hdb_layout(offsets, n_seqs, blocksx, threadsx):
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    G = blocksx * threadsx
+    gl = np.zeros(max((n_seqs + G - 1) // G, 1), np.int32)
+    nb = C.c_uint64()
+    rc = _hdb_lib().hbo_hdb_layout(offsets.ctypes.data, n_seqs, blocksx, threadsx,
+                                   gl.ctypes.data, C.byref(nb))
+    if rc:
+        raise RuntimeError(f"hbo_hdb_layout failed: {rc}")
+    return gl[:(n_seqs + G - 1) // G].copy(), nb.value
+
+
+def hdb_pack(residues, offsets, n_seqs, blocksx, threadsx):
+    residues = np.ascontiguousarray(residues, np.uint8)
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    gl, nb = hdb_layout(offsets, n_seqs, blocksx, threadsx)
+    out = np.zeros(max(nb, 1), np.uint8)
+    rc = _hdb_lib().hbo_hdb_pack(residues.ctypes.data, offsets.ctypes.data, n_seqs, blocksx,
+                                 threadsx, gl.ctypes.data, out.ctypes.data, nb)
+    if rc:
+        raise RuntimeError(f"hbo_hdb_pack failed: {rc}")
+    return out[:nb], gl
+
+
+def hdb_unpack(data, n_seqs, blocksx, threadsx, group_len):
+    """The reference kernel's walk over a .hDb -> (residues, offsets)."""
+    data = np.ascontiguousarray(data, np.uint8)
+    if data.size % 4:
+        data = np.concatenate([data, np.zeros(4 - data.size % 4, np.uint8)])
+    gl = np.ascontiguousarray(group_len, np.int32)
+    offs = np.zeros(n_seqs + 1, np.uint64)
+    L = _hdb_lib()
+    rc = L.hbo_hdb_unpack(data.ctypes.data, data.size, n_seqs, blocksx, threadsx,
+                          gl.ctypes.data, None, offs.ctypes.data)
+    if rc:
+        raise RuntimeError(f"hbo_hdb_unpack failed: {rc}")
+    res = np.zeros(max(int(offs[-1]), 1), np.uint8)
+    rc = L.hbo_hdb_unpack(data.ctypes.data, data.size, n_seqs, blocksx, threadsx,
+                          gl.ctypes.data, res.ctypes.data, offs.ctypes.data)
+    if rc:
+        raise RuntimeError(f"hbo_hdb_unpack failed: {rc}")
+    return res[:int(offs[-1])].copy(), offs
+
+
+def hdbi_format(blocksx, threadsx, volumes, percentage=99):
+    arr = (_HdbVolume * max(len(volumes), 1))()
+    keep = []
+    for i, v in enumerate(volumes):
+        gl = np.ascontiguousarray(v["group_len"], np.int32)
+        keep.append(gl)
+        arr[i] = _HdbVolume(v["name"].encode(), v["bytes"], v["volume_seqs"], v["gpu_seqs"],
+                            v["store_limit"], v["break_limit"], gl.size, gl.ctypes.data,
+                            v["cpu_rest_bytes"])
+    L = _hdb_lib()
+    n = L.hbo_hdbi_format(blocksx, threadsx, percentage, len(volumes), C.addressof(arr), None, 0)
+    buf = C.create_string_buffer(n + 1)
+    L.hbo_hdbi_format(blocksx, threadsx, percentage, len(volumes), C.addressof(arr), buf, n)
+    return buf.raw[:n].decode()
