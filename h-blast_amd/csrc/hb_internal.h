@@ -192,6 +192,7 @@ struct hb_hdb_args {
     uint32_t *lens;                // out: sequence lengths (last non-zero byte + 1)
     const uint64_t *offs;          // packed offsets (copy pass)
     uint8_t *res;                  // out: packed residues (copy pass)
+    uint32_t *bad;                 // out: set to 1 if a residue code is >= 28 (copy pass)
 };
 hipError_t hb_launch_hdb_lengths(const hb_hdb_args &a, hipStream_t st);
 hipError_t hb_launch_hdb_copy(const hb_hdb_args &a, hipStream_t st);

@@ -596,13 +596,14 @@ int hb_db_upload_hdb(hb_ctx *c, const uint8_t *hdb, uint64_t bytes, int32_t bloc
     uint8_t *d_hdb = nullptr;
     uint64_t *d_gbase = nullptr;
     int32_t *d_glen = nullptr;
-    uint32_t *d_lens = nullptr;
+    uint32_t *d_lens = nullptr, *d_bad = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     auto release = [&]() {
         hipFree(d_hdb);
         hipFree(d_gbase);
         hipFree(d_glen);
         hipFree(d_lens);
+        hipFree(d_bad);
         for (hipEvent_t &e : ev)
             if (e) hipEventDestroy(e);
     };
@@ -618,7 +619,8 @@ int hb_db_upload_hdb(hb_ctx *c, const uint8_t *hdb, uint64_t bytes, int32_t bloc
     if (hipMalloc(&d_hdb, std::max<uint64_t>(bytes, 4)) != hipSuccess ||
         hipMalloc(&d_gbase, ng1 * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc(&d_glen, ng1 * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&d_lens, (size_t)std::max(n_seqs, 1) * sizeof(uint32_t)) != hipSuccess)
+        hipMalloc(&d_lens, (size_t)std::max(n_seqs, 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_bad, sizeof(uint32_t)) != hipSuccess)
         return fail(HB_ERR_NOMEM);
     if ((bytes && hipMemcpy(d_hdb, hdb, bytes, hipMemcpyHostToDevice) != hipSuccess) ||
         (n_groups && (hipMemcpy(d_gbase, gbase.data(), n_groups * sizeof(uint64_t),
@@ -651,16 +653,20 @@ int hb_db_upload_hdb(hb_ctx *c, const uint8_t *hdb, uint64_t bytes, int32_t bloc
     if (rc != HB_OK) return fail(rc);
     a.offs = db->d_offs;
     a.res = db->d_res;
+    a.bad = d_bad;
+    uint32_t bad = 0;
     if (hipMemsetAsync(db->d_res + db->total, 0, 64, c->stream) != hipSuccess ||
+        hipMemsetAsync(d_bad, 0, sizeof(uint32_t), c->stream) != hipSuccess ||
         hipEventRecord(ev[2], c->stream) != hipSuccess ||
         hb_launch_hdb_copy(a, c->stream) != hipSuccess ||
         hipEventRecord(ev[3], c->stream) != hipSuccess ||
+        hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(HB_ERR_DEVICE);
     hipEventElapsedTime(&db->hdb_len_ms, ev[0], ev[1]);
     hipEventElapsedTime(&db->hdb_copy_ms, ev[2], ev[3]);
-    rc = db_check_residues(db);
-    if (rc != HB_OK) return fail(rc);
+    if (bad) return fail(HB_ERR_RESIDUE);   // checked by the copy kernel
     release();
     *out = db;
     return HB_OK;
