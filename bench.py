@@ -332,9 +332,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the known-size copy kernel (rocprof calibration)")
-    ap.add_argument("--config", default="c1", choices=["c1", "c3", "c4"],
+    ap.add_argument("--config", default="c1", choices=["c1", "c3", "c4", "qlen"],
                     help="c1: plain 1000-aa query (default); c3: batched 2,048 queries; "
-                         "c4: served Poisson stream over 20M subjects")
+                         "c4: served Poisson stream over 20M subjects; qlen: plain scans "
+                         "of long queries with the diagonal state in LDS and in global "
+                         "memory")
     ap.add_argument("--queries", type=int, default=2048, help="c3: number of queries")
     ap.add_argument("--group-residues", type=int, default=0,
                     help="c3: query residues per group (0: autotuned)")
@@ -364,6 +366,8 @@ def main():
         return bench_c3(args)
     if args.config == "c4":
         return bench_c4(args)
+    if args.config == "qlen":
+        return bench_qlen(args)
     log("generating workload")
     q, res, offs = workload_c1(args.subjects)
     n_subj = len(offs) - 1
@@ -485,6 +489,48 @@ def main():
         out["forms"] = forms
     print(json.dumps(out), flush=True)
     query.close()
+    db.close()
+    ctx.close()
+
+
+def bench_qlen(args, n_subjects=200_000,
+               lengths=(1000, 2000, 4000, 6000, 8000, 12000, 16000, 24000, 31000)):
+    """Plain BLASTP scans of one query per length (i.i.d. residues, seed
+    7000 + length) against 200,000 subjects of the default database
+    generator (seed 1100), with the per-diagonal two-hit state in LDS
+    (where it fits) and in global memory; the best of two timed scans after
+    one warm-up, device time per scan."""
+    res, offs = hbseed.make_db(1100, n_subjects)
+    ctx = hbseed.Context(0)
+    db = hbseed.Database(ctx, res, offs)
+    rows = []
+    for n in lengths:
+        q = hbseed.Query(ctx, hbseed.blastp_query(hbseed.random_protein(7000 + n, n)))
+        row = {"query_len": n, "auto": "global" if q.state_mode == hbseed.STATE_GLOBAL
+               else "lds"}
+        for mode, key in ((hbseed.STATE_LDS, "lds_ms"), (hbseed.STATE_GLOBAL, "global_ms")):
+            try:
+                q.set_state_mode(mode)
+            except hbseed.HBError:
+                row[key] = None
+                continue
+            ms = []
+            for _ in range(3):
+                s = hbseed.Scan(ctx, db, q)
+                s.wait()
+                ms.append(s.elapsed_ms())
+                s.close()
+            row[key] = min(ms[1:])
+        log(f"query {n}: {row}")
+        rows.append(row)
+        q.close()
+    out = {"metric": "ms per scan by query length", "unit": "ms", "n_gpus": 1,
+           "higher_is_better": False, "dtype": "int32", "data": "synthetic",
+           "config": {"workload": f"plain BLASTP scans of 1,000-31,000-residue queries vs "
+                                  f"{n_subjects:,} subjects", "subjects": n_subjects,
+                      "residues": int(offs[-1])},
+           "rows": rows}
+    print(json.dumps(out), flush=True)
     db.close()
     ctx.close()
 

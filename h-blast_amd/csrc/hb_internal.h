@@ -66,6 +66,9 @@ struct hb_scan_args {
     uint32_t *out_count;
     uint32_t *stats;             // plain: 3 per oid (lookup hits, extensions, HSPs); else null
     unsigned long long *group_totals;  // batched: 3 per group; else null
+    // plain form only: the diagonal state in global memory, diag_len_max
+    // int32 per wavefront of the grid (null: in LDS)
+    int32_t *gstate;
 };
 
 // per wave: diagonal state, ordering markers, 64 segment tags, job and
@@ -113,7 +116,8 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 // launching or capturing a launch with that LDS size).
 // Raises both scan kernels' dynamic LDS limit to max_lds (once per context).
 hipError_t hb_scan_allow_lds(size_t max_lds);
-hipError_t hb_scan_occupancy(bool multi, size_t lds_bytes, int block_threads, int *blocks_per_cu);
+hipError_t hb_scan_occupancy(bool multi, bool gstate, size_t lds_bytes, int block_threads,
+                             int *blocks_per_cu);
 hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);
 hipError_t hb_launch_fill_u32(uint32_t *dst, uint32_t value, uint32_t n, hipStream_t st);
 

@@ -344,6 +344,7 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+template <bool G>
 struct WaveScan {
     const hb_scan_args *a;
     // group being scanned
@@ -397,14 +398,39 @@ struct WaveScan {
         return (uint32_t)(d < 0 ? d + D : d);
     }
 
+    // Diagonal state accesses.  G = false: the state is in LDS.  G = true:
+    // it is in the wavefront's own global-memory region (queries whose
+    // state does not fit LDS); the accesses are workgroup-scope atomics and
+    // sync() waits for them, so a lane reads what another lane of the wave
+    // wrote before the last sync().
+    __device__ __forceinline__ int32_t st_ld(uint32_t d) const
+    {
+        if constexpr (G)
+            return __hip_atomic_load(&state[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            return state[d];
+    }
+    __device__ __forceinline__ void st_st(uint32_t d, int32_t v)
+    {
+        if constexpr (G)
+            __hip_atomic_store(&state[d], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            state[d] = v;
+    }
+    __device__ __forceinline__ void sync() const
+    {
+        if constexpr (G) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        wave_lds_sync();
+    }
+
     // Starts a new group: clears the diagonal state and ordering markers.
     __device__ __forceinline__ void reset()
     {
-        for (int i = lane; i < D; i += HB_WAVE) state[i] = 0;
+        for (int i = lane; i < D; i += HB_WAVE) st_st((uint32_t)i, 0);
         for (int i = lane; i < HB_MARK; i += HB_WAVE) marker[i] = 0;
         base = window;
         gen = 0;
-        wave_lds_sync();
+        sync();
     }
 
     // One pass of two-hit decisions over a set of hits (at most one per
@@ -426,15 +452,15 @@ struct WaveScan {
     {
         if (++gen == (1u << 26)) {
             for (int i = lane; i < HB_MARK; i += HB_WAVE) marker[i] = 0;
-            wave_lds_sync();
+            sync();
             gen = 1;
         }
         const uint32_t key = (gen << 6) | (uint32_t)(63 - lane);
         const uint32_t slot = d & (HB_MARK - 1);
         if (active) atomicMax(&marker[slot], key);
-        wave_lds_sync();
+        sync();
         const uint32_t mk = marker[slot];
-        const int32_t st = state[d];
+        const int32_t st = st_ld(d);
         const bool win = active && mk == key;
         const uint32_t ust = (uint32_t)st;
         const bool pend = (ust & kPend) != 0;
@@ -446,10 +472,10 @@ struct WaveScan {
         int32_t nst;
         if (flag) nst = sb >= (int32_t)(ust & kVal) ? sb : st;
         else nst = diff >= window ? sb : (job ? (int32_t)kPend : st);
-        if (win && !pend && nst != st) state[d] = nst;
-        wave_lds_sync();
+        if (win && !pend && nst != st) st_st(d, nst);
+        sync();
         const bool lose = active && !win;
-        const bool lwait = lose && ((uint32_t)state[d] & kPend) != 0;
+        const bool lwait = lose && ((uint32_t)st_ld(d) & kPend) != 0;
         const uint64_t mj = __ballot(job), mb = __ballot(wait || lwait);
         const uint32_t tso = (uint32_t)so | (sl ? 0x80000000u : 0u);
         if (job) {
@@ -461,7 +487,7 @@ struct WaveScan {
         n_blk += (uint32_t)__popcll(mb);
         n_ext0 += job && !sl ? 1u : 0u;
         n_ext1 += job && sl ? 1u : 0u;
-        wave_lds_sync();
+        sync();
         return lose && !lwait;
     }
 
@@ -499,8 +525,8 @@ struct WaveScan {
                 else
                     r = ext_two_hit<false>(mat, S, slen, qry, qlen, last_s + 3, so, q,
                                            ctx[ctx_pad + k]);
-                state[diag_of((uint32_t)q, so)] =
-                    r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + b)) : so + b;
+                st_st(diag_of((uint32_t)q, so),
+                      r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + b)) : so + b);
             }
             const bool emit = v && r.score >= ctx[2 * ctx_pad + k];
             const uint64_t m = __ballot(emit);
@@ -528,7 +554,7 @@ struct WaveScan {
             n_hsp1 += emit && sl ? 1u : 0u;
         }
         n_jobs = 0;
-        wave_lds_sync();
+        sync();
     }
 
     // Re-decides the blocked hits in their original order (in place: each
@@ -542,7 +568,7 @@ struct WaveScan {
             const uint32_t b = b0 + (uint32_t)lane;
             const bool v = b < nb;
             const uint2 e = v ? blk[b] : make_uint2(0, 0);
-            wave_lds_sync();
+            sync();
             const int32_t so = (int32_t)(e.y & 0x7fffffffu);
             decide(v, e.x, so, diag_of(e.x, so), (e.y >> 31) != 0);
         }
@@ -594,12 +620,12 @@ struct WaveScan {
         const uint32_t n = (uint32_t)__popcll(ml);
         if (n) {
             if (lost) seg[lane_rank(ml)] = (uint32_t)lane;   // source lane of each rank
-            wave_lds_sync();
+            sync();
             const int src = (uint32_t)lane < n ? (int)(seg[lane] & 63u) : lane;
             cq = (uint32_t)__shfl((int)q, src, HB_WAVE);
             cs = __shfl(so, src, HB_WAVE);
             cd = (uint32_t)__shfl((int)d, src, HB_WAVE);
-            wave_lds_sync();
+            sync();
         }
         return n;
     }
@@ -659,7 +685,7 @@ struct WaveScan {
                 }
                 const uint32_t tag = rgen << 7;
                 if (cnt && excl >= r0 && excl < r0 + take) seg[excl - r0 + c] = tag | ((uint32_t)lane + 1);
-                wave_lds_sync();
+                sync();
                 const uint32_t sv = seg[lane];
                 const uint32_t v = wave_incl_max((sv & ~127u) == tag ? (sv & 127u) : 0u);
                 const uint32_t p = min(v ? v - 1 : carry, 63u);
@@ -677,7 +703,7 @@ struct WaveScan {
                 const uint32_t q = isc ? cq : qn;
                 const int32_t so = isc ? cs_ : s0 + (int32_t)p;
                 const uint32_t d = isc ? cd : (uint32_t)dn;
-                wave_lds_sync();
+                sync();
                 const bool lost = decide_pass(isc || isnew, q, so, d, csl);
                 c = carry_losers(lost, q, so, d, cq, cs_, cd);
                 r0 += take;
@@ -703,8 +729,8 @@ struct WaveScan {
             retire(cs);
         }
         if (reset) {
-            for (int i = lane; i < D; i += HB_WAVE) state[i] = 0;
-            wave_lds_sync();
+            for (int i = lane; i < D; i += HB_WAVE) st_st((uint32_t)i, 0);
+            sync();
             base = window;
         } else {
             base += slen + window;
@@ -720,9 +746,9 @@ struct WaveScan {
         if ((uint32_t)slen <= a->scap) {
             uint8_t *B = cs ? sbuf1 : sbuf0;
             for (int i = lane; i < slen; i += HB_WAVE) B[i] = S[i];
-            wave_lds_sync();
+            sync();
             subject<true>(B, slen, oid);
-            wave_lds_sync();
+            sync();
         } else {
             finish_deferred();
             subject<false>(S, slen, oid);
@@ -753,7 +779,8 @@ __device__ __forceinline__ void load_group(const hb_scan_args &a, const hb_group
     }
 }
 
-__device__ __forceinline__ void bind_group(WaveScan &w, const hb_group_desc &G)
+template <bool GS>
+__device__ __forceinline__ void bind_group(WaveScan<GS> &w, const hb_group_desc &G)
 {
     w.cells = (const HB_GLOBAL uint64_t *)G.cells;
     w.entries = (const HB_GLOBAL uint32_t *)G.entries;
@@ -771,7 +798,10 @@ __device__ __forceinline__ void bind_group(WaveScan &w, const hb_group_desc &G)
 // Register budget for five wavefronts per SIMD (no spills at this limit):
 // the scan is latency-bound, and short queries or small groups leave LDS
 // for that many.
-template <bool MULTI>
+// GSTATE: the diagonal state lives in global memory (a.gstate, diag_len_max
+// int32 per wavefront of the grid) instead of LDS, for queries whose state
+// does not fit.
+template <bool MULTI, bool GSTATE>
 __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void hb_scan_kernel(hb_scan_args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -787,14 +817,19 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     for (int i = threadIdx.x; i < 256; i += blockDim.x)
         reinterpret_cast<uint32_t *>(sm_mat)[i] = reinterpret_cast<const uint32_t *>(a.mat32)[i];
 
-    WaveScan w;
+    WaveScan<GSTATE> w;
     w.a = &a;
     w.mat = sm_mat;
     w.qry = sm_q;
     w.ctx = sm_ctx;
     w.ctx_pad = a.ctx_pad_max;
-    w.state = reinterpret_cast<int32_t *>(wb);
-    w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)Dm);
+    if constexpr (GSTATE) {
+        w.state = a.gstate + ((size_t)blockIdx.x * (blockDim.x / HB_WAVE) + (size_t)wid) * (size_t)Dm;
+        w.marker = reinterpret_cast<uint32_t *>(wb);
+    } else {
+        w.state = reinterpret_cast<int32_t *>(wb);
+        w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)Dm);
+    }
     w.seg = w.marker + HB_MARK;
     w.jobs = reinterpret_cast<uint2 *>(w.seg + HB_WAVE);
     w.blk = w.jobs + kJobCap;
@@ -1130,10 +1165,13 @@ hipError_t hb_launch_lut(const hb_lut_job *jobs_dev, int n_jobs, int32_t qlen_ma
     return hipGetLastError();
 }
 
-static const void *scan_fn(bool multi)
+// Instantiations: batched (MULTI) with LDS state; plain with LDS or global
+// state.
+static const void *scan_fn(bool multi, bool gstate)
 {
-    return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true>)
-                 : reinterpret_cast<const void *>(hb_scan_kernel<false>);
+    if (multi) return reinterpret_cast<const void *>(hb_scan_kernel<true, false>);
+    return gstate ? reinterpret_cast<const void *>(hb_scan_kernel<false, true>)
+                  : reinterpret_cast<const void *>(hb_scan_kernel<false, false>);
 }
 
 hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int block_threads,
@@ -1141,31 +1179,37 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 {
     if (block_threads < HB_WAVE || block_threads > HB_BLOCK || block_threads % HB_WAVE)
         return hipErrorInvalidValue;
+    if (multi && a.gstate) return hipErrorInvalidValue;
     if (multi)
-        hipLaunchKernelGGL(hb_scan_kernel<true>, dim3(n_blocks), dim3(block_threads), lds_bytes, st,
-                           a);
+        hipLaunchKernelGGL((hb_scan_kernel<true, false>), dim3(n_blocks), dim3(block_threads),
+                           lds_bytes, st, a);
+    else if (a.gstate)
+        hipLaunchKernelGGL((hb_scan_kernel<false, true>), dim3(n_blocks), dim3(block_threads),
+                           lds_bytes, st, a);
     else
-        hipLaunchKernelGGL(hb_scan_kernel<false>, dim3(n_blocks), dim3(block_threads), lds_bytes,
-                           st, a);
+        hipLaunchKernelGGL((hb_scan_kernel<false, false>), dim3(n_blocks), dim3(block_threads),
+                           lds_bytes, st, a);
     return hipGetLastError();
 }
 
 hipError_t hb_scan_allow_lds(size_t max_lds)
 {
-    for (bool multi : {false, true}) {
+    for (int k = 0; k < 3; ++k) {
+        const void *fn = scan_fn(k == 1, k == 2);
         hipFuncAttributes fa;
-        hipError_t e = hipFuncGetAttributes(&fa, scan_fn(multi));
+        hipError_t e = hipFuncGetAttributes(&fa, fn);
         if (e != hipSuccess) return e;
         // the limit covers dynamic LDS only; the kernel's static LDS comes on top
-        e = hipFuncSetAttribute(scan_fn(multi), hipFuncAttributeMaxDynamicSharedMemorySize,
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(max_lds - fa.sharedSizeBytes));
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-hipError_t hb_scan_occupancy(bool multi, size_t lds_bytes, int block_threads, int *blocks_per_cu)
+hipError_t hb_scan_occupancy(bool multi, bool gstate, size_t lds_bytes, int block_threads,
+                             int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_fn(multi),
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_fn(multi, gstate),
                                                         block_threads, lds_bytes);
 }

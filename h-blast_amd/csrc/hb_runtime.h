@@ -104,8 +104,11 @@ struct hb_launch_cfg {
     int blocks = 0;
     int32_t scap = 1024;   // staged-subject capacity per wave
 };
+// gstate: the diagonal state goes to global memory (plain form only), so the
+// per-wave LDS holds everything but the state.
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
-                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg);
+                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg,
+                   bool gstate = false);
 
 // Asynchronous steps of a batch (hb_batch.cpp), used by the serving loop:
 // the batch runs on the given stream (nullptr: the context's); launch

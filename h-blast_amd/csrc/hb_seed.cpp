@@ -310,16 +310,17 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
 // wavefronts with 320-byte staging ran 8% slower than 16 with 1024), then
 // more wavefronts per workgroup (they share the query tables).
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
-                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg)
+                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg, bool gstate)
 {
     const size_t fixed = hb_scan_fixed_lds(qlen_pad_max, ctx_pad_max);
+    const int32_t lds_diag = gstate ? 0 : diag_len_max;
     int best_wpc = 0;
     for (int waves : {4, 2, 1}) {
         for (int32_t scap = HB_SCAP_MAX; scap >= 256; scap -= 64) {
-            const size_t lds = fixed + waves * hb_scan_wave_lds(diag_len_max, scap);
+            const size_t lds = fixed + waves * hb_scan_wave_lds(lds_diag, scap);
             if (lds > c->max_lds) continue;
             int per_cu = 0;
-            HB_CHECK(hb_scan_occupancy(multi, lds, waves * HB_WAVE, &per_cu));
+            HB_CHECK(hb_scan_occupancy(multi, gstate, lds, waves * HB_WAVE, &per_cu));
             const int wpc = std::min(per_cu * waves, 16);
             if (wpc > best_wpc || (wpc == best_wpc && scap > cfg->scap)) {
                 best_wpc = wpc;
@@ -345,6 +346,8 @@ struct hb_query {
     hb_group_set gs;
     uint32_t n_cells_used = 0;
     std::vector<int32_t> ctx_off, ctx_len;   // host copies (gapped stage)
+    bool gstate = false;   // diagonal state kept in global memory
+    bool lds_fits = true;  // the LDS layout can hold the diagonal state
 };
 
 struct hb_scan {
@@ -360,6 +363,8 @@ struct hb_scan {
     uint32_t n_out = 0;
     float gap_ms = 0.f;        // device time of the last gapped stage
     uint32_t gap_wide = 0;     // subjects it ran in the wide pass
+    int32_t *d_gstate = nullptr;   // per-wavefront diagonal state (long queries)
+    size_t gstate_bytes = 0;
 };
 
 extern "C" {
@@ -717,6 +722,22 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
                                    nullptr, out);
 }
 
+// Automatic placement of the diagonal state: LDS unless it does not fit or
+// the LDS it takes leaves fewer than kGstateMinWaves wavefronts per CU.
+// Measured on MI355X, 200,000 subjects (plain BLASTP, ms LDS / global):
+// 4,000 residues (7 wavefronts per CU with LDS state) 10.7 / 13.3;
+// 6,000 (5) 23.4 / 21.8; 12,000 77.8 / 44.1; 31,000 359 / 163.
+static constexpr int kGstateMinWaves = 6;
+static bool state_auto_global(const hb_ctx *c, int32_t len, int32_t D, size_t fixed)
+{
+    (void)len;
+    const size_t per_wave = hb_scan_wave_lds(D, 256);
+    if (fixed + per_wave > c->max_lds) return true;
+    // waves of one workgroup share the fixed part; up to 4 per workgroup
+    const size_t waves = (c->max_lds - fixed) / per_wave;
+    return (int)std::min<size_t>(waves, 16) < kGstateMinWaves;
+}
+
 int hb_query_prepare_masked(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
                             const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
                             const int32_t *cutoff, const hb_params *params,
@@ -743,11 +764,17 @@ int hb_query_prepare_masked(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t 
     qs[0].xdrop.assign(xdrop, xdrop + n_ctx);
     qs[0].cutoff.assign(cutoff, cutoff + n_ctx);
     if (lut_mask) qs[0].mask.assign(lut_mask, lut_mask + len);
+    // The diagonal state (4 bytes per diagonal) stays in LDS when it fits;
+    // longer queries (the reference accepts 32,768 residues,
+    // H-BLAST/h_blast.cu:1293) keep it in global memory.
     const int32_t D = (len + q->p.window + 63) & ~63;
-    if (hb_scan_fixed_lds((len + 15) & ~15, hb_ctx_pad(n_ctx)) + hb_scan_wave_lds(D, 256) > c->max_lds) {
+    const size_t fixed = hb_scan_fixed_lds((len + 15) & ~15, hb_ctx_pad(n_ctx));
+    if (fixed + hb_scan_wave_lds(0, 256) > c->max_lds) {
         delete q;
         return HB_ERR_UNSUPPORTED;
     }
+    q->lds_fits = fixed + hb_scan_wave_lds(D, 256) <= c->max_lds;
+    q->gstate = state_auto_global(c, len, D, fixed);
     std::lock_guard<std::mutex> lk(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) {
         delete q;
@@ -767,6 +794,31 @@ int hb_query_prepare_masked(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t 
     for (uint32_t w : pv) q->n_cells_used += (uint32_t)__builtin_popcount(w);
     *out = q;
     return HB_OK;
+}
+
+int hb_query_set_state_mode(hb_query *q, int mode)
+{
+    if (!q) return HB_ERR_ARG;
+    const int32_t len = q->gs.qlen_pad_max;
+    switch (mode) {
+    case HB_STATE_AUTO: {
+        const size_t fixed = hb_scan_fixed_lds(q->gs.qlen_pad_max, q->gs.ctx_pad_max);
+        q->gstate = state_auto_global(q->ctx, len, q->gs.diag_len_max, fixed);
+        return HB_OK;
+    }
+    case HB_STATE_LDS:
+        if (!q->lds_fits) return HB_ERR_UNSUPPORTED;
+        q->gstate = false;
+        return HB_OK;
+    case HB_STATE_GLOBAL: q->gstate = true; return HB_OK;
+    default: return HB_ERR_ARG;
+    }
+}
+
+int hb_query_state_mode(const hb_query *q)
+{
+    if (!q) return HB_ERR_ARG;
+    return q->gstate ? HB_STATE_GLOBAL : HB_STATE_LDS;
 }
 
 int hb_query_lut_info(const hb_query *q, uint32_t *n_entries, uint32_t *n_cells_used)
@@ -804,8 +856,13 @@ static int launch_scan(hb_scan *s)
     hb_query *q = s->q;
     hb_launch_cfg cfg;
     int rc = hb_scan_config(c, false, q->gs.qlen_pad_max, q->gs.ctx_pad_max, q->gs.diag_len_max,
-                            db->n_tiles, &cfg);
+                            db->n_tiles, &cfg, q->gstate);
     if (rc) return rc;
+    if (q->gstate && !s->d_gstate) {
+        s->gstate_bytes = (size_t)cfg.blocks * cfg.waves * (size_t)q->gs.diag_len_max * 4;
+        s->d_gstate = (int32_t *)hb_pool_get(c, s->gstate_bytes);
+        if (!s->d_gstate) return HB_ERR_NOMEM;
+    }
     hb_scan_args a;
     memset(&a, 0, sizeof(a));
     a.res = db->d_res;
@@ -819,8 +876,9 @@ static int launch_scan(hb_scan *s)
     a.qlen_pad_max = q->gs.qlen_pad_max;
     a.ctx_pad_max = q->gs.ctx_pad_max;
     a.diag_len_max = q->gs.diag_len_max;
-    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gs.diag_len_max, cfg.scap);
+    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gstate ? 0 : q->gs.diag_len_max, cfg.scap);
     a.scap = (uint32_t)cfg.scap;
+    a.gstate = s->d_gstate;   // cleared by each wavefront before use
     a.mat32 = q->gs.d_mat;
     a.window = q->p.window;
     a.work_counter = s->d_misc + 1;
@@ -844,6 +902,7 @@ int hb_scan_free(hb_scan *s)
     hb_pool_put(s->ctx, s->d_out, (size_t)s->cap * sizeof(hb_hsp));
     hb_pool_put(s->ctx, s->d_misc, 2 * sizeof(uint32_t));
     hb_pool_put(s->ctx, s->d_stats, (size_t)std::max<uint32_t>(s->db->n, 1) * 3 * sizeof(uint32_t));
+    if (s->d_gstate) hb_pool_put(s->ctx, s->d_gstate, s->gstate_bytes);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     delete s;

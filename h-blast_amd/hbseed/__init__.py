@@ -29,6 +29,7 @@ ERRORS = {
 }
 DEFAULT_XDROP = 16
 DEFAULT_CUTOFF = 41
+STATE_AUTO, STATE_LDS, STATE_GLOBAL = 0, 1, 2
 STDAA = "-ABCDEFGHIKLMNPQRSTVWXYZU*OJ"
 
 HSP_DTYPE = np.dtype([("oid", "<u4"), ("context", "<i4"), ("q_start", "<i4"),
@@ -51,7 +52,8 @@ EXPORTS = [
     "hb_mem_info", "hb_synchronize", "hb_db_upload", "hb_db_free", "hb_db_subjects",
     "hb_db_residues", "hb_query_prepare", "hb_query_free", "hb_query_lut_info",
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
-    "hb_encode_protein", "hb_query_prepare_masked", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
+    "hb_encode_protein", "hb_query_prepare_masked", "hb_query_set_state_mode",
+    "hb_query_state_mode", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
     "hb_fasta_parse", "hb_hdb_gpu_seqs", "hb_hdb_layout", "hb_hdb_pack", "hb_hdbi_format",
@@ -148,6 +150,8 @@ def _lib():
         L.hb_query_prepare.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, vp, pp]
         L.hb_query_prepare_masked.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, pp]
         L.hb_query_free.argtypes = [vp]
+        L.hb_query_set_state_mode.argtypes = [vp, C.c_int]
+        L.hb_query_state_mode.argtypes = [vp]
         L.hb_query_lut_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
         L.hb_query_lut_fetch.argtypes = [vp, vp, vp, u32]
         L.hb_blastx_concat_len.restype = i32
@@ -539,6 +543,15 @@ class Query:
                                               _ptr(spec.lut_mask), C.byref(h)),
                "hb_query_prepare")
         self.h = h
+
+    def set_state_mode(self, mode):
+        """STATE_AUTO / STATE_LDS / STATE_GLOBAL: where the scan keeps the
+        per-diagonal two-hit state (hb_query_set_state_mode)."""
+        _check(_lib().hb_query_set_state_mode(self.h, mode), "hb_query_set_state_mode")
+
+    @property
+    def state_mode(self):
+        return _lib().hb_query_state_mode(self.h)
 
     def lut(self):
         ne, nu = C.c_uint32(), C.c_uint32()

@@ -148,6 +148,22 @@ int hb_query_prepare_masked(hb_ctx *ctx, const uint8_t *seq, int32_t len, int32_
                             const hb_params *params, const uint8_t *lut_mask,
                             hb_query **out);
 int hb_query_free(hb_query *q);
+/* Where a plain scan of q keeps its per-diagonal two-hit state (the
+ * reference keeps it in global memory, ewp->diag_table, one array per
+ * thread: h_blast.cu:959-966):
+ *   HB_STATE_AUTO   - LDS when the query is short enough for LDS to hold
+ *                     it without starving the CU of wavefronts, else global
+ *                     memory (the default);
+ *   HB_STATE_LDS    - LDS (HB_ERR_UNSUPPORTED if it does not fit);
+ *   HB_STATE_GLOBAL - a global-memory array per wavefront.
+ * Results are identical; only speed differs.  Takes effect for later
+ * hb_scan_start calls. */
+#define HB_STATE_AUTO 0
+#define HB_STATE_LDS 1
+#define HB_STATE_GLOBAL 2
+int hb_query_set_state_mode(hb_query *q, int mode);
+/* The placement hb_scan_start will use (HB_STATE_LDS or HB_STATE_GLOBAL). */
+int hb_query_state_mode(const hb_query *q);
 /* Number of (query offset, cell) entries of the lookup table. */
 int hb_query_lut_info(const hb_query *q, uint32_t *n_entries, uint32_t *n_cells_used);
 /* Copies the device lookup table back (CSR over 32768 cells:

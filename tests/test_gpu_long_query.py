@@ -1,0 +1,104 @@
+"""GPU parity of the diagonal-state placements: a scan that keeps the
+two-hit state in LDS and one that keeps it in a global-memory array per
+wavefront (long queries) give identical HSPs and counters, both equal to
+the oracle, for queries from 300 residues to beyond what LDS can hold.
+"""
+import numpy as np
+import pytest
+
+import hbseed
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = hbseed.Context(0)
+    yield c
+    c.close()
+
+
+def run(ctx, db, spec, mode):
+    q = hbseed.Query(ctx, spec)
+    try:
+        q.set_state_mode(mode)
+        assert q.state_mode == (hbseed.STATE_GLOBAL if mode == hbseed.STATE_GLOBAL
+                                else hbseed.STATE_LDS)
+        s = hbseed.Scan(ctx, db, q)
+        try:
+            stats, hsps = s.results()
+        finally:
+            s.close()
+    finally:
+        q.close()
+    return stats, hsps
+
+
+def long_spec(kind, n, seed):
+    if kind == "blastp":
+        return hbseed.blastp_query(hbseed.random_protein(seed, n))
+    nt = hbseed.blastx_nt(seed, n, hbseed.random_protein(seed + 1, 100),
+                          hbseed.random_protein(seed + 2, 100))
+    return hbseed.blastx_query(nt)
+
+
+@pytest.mark.parametrize("kind,n,lds", [
+    ("blastp", 300, True), ("blastp", 2000, True), ("blastx", 6000, True),
+    ("blastp", 20000, True), ("blastp", 40000, False),
+])
+def test_state_placements_match_each_other_and_oracle(ctx, kind, n, lds):
+    spec = long_spec(kind, n, 3000 + n)
+    print(f"{kind} query of {len(spec.seq)} residues ({spec.n_ctx} contexts)", flush=True)
+    res, offs = hbseed.make_db(3100 + n, 2000, plant_query=spec.seq[:min(len(spec.seq), 3000)],
+                               plant_frac=0.2)
+    db = hbseed.Database(ctx, res, offs)
+    try:
+        sg, hg = run(ctx, db, spec, hbseed.STATE_GLOBAL)
+        if lds:
+            sl, hl = run(ctx, db, spec, hbseed.STATE_LDS)
+            assert np.array_equal(hl.view(np.int32), hg.view(np.int32))
+            assert np.array_equal(sl.view(np.uint8), sg.view(np.uint8))
+        else:
+            q = hbseed.Query(ctx, spec)
+            try:
+                with pytest.raises(hbseed.HBError) as e:
+                    q.set_state_mode(hbseed.STATE_LDS)
+                assert e.value.code == -7
+                assert q.state_mode == hbseed.STATE_GLOBAL   # automatic placement
+            finally:
+                q.close()
+    finally:
+        db.close()
+    oq = oracle_lib.OracleQuery.from_spec(spec)
+    cnt, ref = oq.scan(res, offs, threads=8)
+    assert len(hg) > 0
+    assert np.array_equal(hg.view(np.int32), ref.view(np.int32))
+    assert np.array_equal(sg["lookup_hits"], cnt[:, 0])
+    assert np.array_equal(sg["extensions"], cnt[:, 1])
+    assert np.array_equal(sg["n_hsp"], cnt[:, 2])
+
+
+def test_gapped_stage_on_a_long_query(ctx):
+    spec = hbseed.blastp_query(hbseed.random_protein(3301, 36000))
+    res, offs = hbseed.make_db(3302, 1500, plant_query=spec.seq[30000:33000], plant_frac=0.3)
+    db = hbseed.Database(ctx, res, offs)
+    q = hbseed.Query(ctx, spec)
+    try:
+        assert q.state_mode == hbseed.STATE_GLOBAL
+        s = hbseed.Scan(ctx, db, q)
+        try:
+            _, hsps = s.results()
+            gst, g = s.gapped([60])
+        finally:
+            s.close()
+    finally:
+        q.close()
+        db.close()
+    oq = oracle_lib.OracleQuery.from_spec(spec)
+    _, ref_hsps = oq.scan(res, offs, threads=8)
+    assert np.array_equal(hsps.view(np.int32), ref_hsps.view(np.int32))
+    cnt, ref = oracle_lib.gapped_db(oq, res, offs, ref_hsps, [60], threads=8)
+    assert len(g) > 0
+    assert np.array_equal(gst["n_hsp"], cnt[:, 0])
+    assert np.array_equal(g.view(np.int32), ref.view(np.int32))
