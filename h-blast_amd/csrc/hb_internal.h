@@ -66,8 +66,8 @@ struct hb_scan_args {
     uint32_t *out_count;
     uint32_t *stats;             // plain: 3 per oid (lookup hits, extensions, HSPs); else null
     unsigned long long *group_totals;  // batched: 3 per group; else null
-    // plain form only: the diagonal state in global memory, diag_len_max
-    // int32 per wavefront of the grid (null: in LDS)
+    // the diagonal state in global memory, diag_len_max int32 per
+    // wavefront of the grid (null: in LDS)
     int32_t *gstate;
 };
 

@@ -799,8 +799,8 @@ __device__ __forceinline__ void bind_group(WaveScan<GS> &w, const hb_group_desc 
 // the scan is latency-bound, and short queries or small groups leave LDS
 // for that many.
 // GSTATE: the diagonal state lives in global memory (a.gstate, diag_len_max
-// int32 per wavefront of the grid) instead of LDS, for queries whose state
-// does not fit.
+// int32 per wavefront of the grid) instead of LDS, for queries (groups)
+// whose state does not fit LDS or would leave it too few wavefronts.
 template <bool MULTI, bool GSTATE>
 __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void hb_scan_kernel(hb_scan_args a)
 {
@@ -1165,11 +1165,11 @@ hipError_t hb_launch_lut(const hb_lut_job *jobs_dev, int n_jobs, int32_t qlen_ma
     return hipGetLastError();
 }
 
-// Instantiations: batched (MULTI) with LDS state; plain with LDS or global
-// state.
 static const void *scan_fn(bool multi, bool gstate)
 {
-    if (multi) return reinterpret_cast<const void *>(hb_scan_kernel<true, false>);
+    if (multi)
+        return gstate ? reinterpret_cast<const void *>(hb_scan_kernel<true, true>)
+                      : reinterpret_cast<const void *>(hb_scan_kernel<true, false>);
     return gstate ? reinterpret_cast<const void *>(hb_scan_kernel<false, true>)
                   : reinterpret_cast<const void *>(hb_scan_kernel<false, false>);
 }
@@ -1179,8 +1179,10 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 {
     if (block_threads < HB_WAVE || block_threads > HB_BLOCK || block_threads % HB_WAVE)
         return hipErrorInvalidValue;
-    if (multi && a.gstate) return hipErrorInvalidValue;
-    if (multi)
+    if (multi && a.gstate)
+        hipLaunchKernelGGL((hb_scan_kernel<true, true>), dim3(n_blocks), dim3(block_threads),
+                           lds_bytes, st, a);
+    else if (multi)
         hipLaunchKernelGGL((hb_scan_kernel<true, false>), dim3(n_blocks), dim3(block_threads),
                            lds_bytes, st, a);
     else if (a.gstate)
@@ -1194,8 +1196,8 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 
 hipError_t hb_scan_allow_lds(size_t max_lds)
 {
-    for (int k = 0; k < 3; ++k) {
-        const void *fn = scan_fn(k == 1, k == 2);
+    for (int k = 0; k < 4; ++k) {
+        const void *fn = scan_fn((k & 1) != 0, (k & 2) != 0);
         hipFuncAttributes fa;
         hipError_t e = hipFuncGetAttributes(&fa, fn);
         if (e != hipSuccess) return e;

@@ -104,8 +104,12 @@ struct hb_launch_cfg {
     int blocks = 0;
     int32_t scap = 1024;   // staged-subject capacity per wave
 };
-// gstate: the diagonal state goes to global memory (plain form only), so the
-// per-wave LDS holds everything but the state.
+// True when a launch with diagonal arrays of D entries and `fixed` bytes of
+// per-workgroup LDS should keep the diagonal state in global memory: it
+// does not fit LDS, or it leaves too few wavefronts per CU.
+bool hb_state_auto_global(const hb_ctx *c, int32_t D, size_t fixed);
+// gstate: the diagonal state goes to global memory, so the per-wave LDS
+// holds everything but the state.
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg,
                    bool gstate = false);

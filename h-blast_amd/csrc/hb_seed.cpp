@@ -340,6 +340,21 @@ int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_
     return HB_OK;
 }
 
+// Automatic placement of the diagonal state: LDS unless it does not fit or
+// the LDS it takes leaves fewer than kGstateMinWaves wavefronts per CU.
+// Measured on MI355X, 200,000 subjects (plain BLASTP, ms LDS / global):
+// 4,000 residues (7 wavefronts per CU with LDS state) 10.7 / 13.3;
+// 6,000 (5) 23.4 / 21.8; 12,000 77.8 / 44.1; 31,000 359 / 163.
+static constexpr int kGstateMinWaves = 6;
+bool hb_state_auto_global(const hb_ctx *c, int32_t D, size_t fixed)
+{
+    const size_t per_wave = hb_scan_wave_lds(D, 256);
+    if (fixed + per_wave > c->max_lds) return true;
+    // waves of one workgroup share the fixed part; up to 4 per workgroup
+    const size_t waves = (c->max_lds - fixed) / per_wave;
+    return (int)std::min<size_t>(waves, 16) < kGstateMinWaves;
+}
+
 struct hb_query {
     hb_ctx *ctx = nullptr;
     hb_resolved_params p;
@@ -722,22 +737,6 @@ int hb_query_prepare(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
                                    nullptr, out);
 }
 
-// Automatic placement of the diagonal state: LDS unless it does not fit or
-// the LDS it takes leaves fewer than kGstateMinWaves wavefronts per CU.
-// Measured on MI355X, 200,000 subjects (plain BLASTP, ms LDS / global):
-// 4,000 residues (7 wavefronts per CU with LDS state) 10.7 / 13.3;
-// 6,000 (5) 23.4 / 21.8; 12,000 77.8 / 44.1; 31,000 359 / 163.
-static constexpr int kGstateMinWaves = 6;
-static bool state_auto_global(const hb_ctx *c, int32_t len, int32_t D, size_t fixed)
-{
-    (void)len;
-    const size_t per_wave = hb_scan_wave_lds(D, 256);
-    if (fixed + per_wave > c->max_lds) return true;
-    // waves of one workgroup share the fixed part; up to 4 per workgroup
-    const size_t waves = (c->max_lds - fixed) / per_wave;
-    return (int)std::min<size_t>(waves, 16) < kGstateMinWaves;
-}
-
 int hb_query_prepare_masked(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t n_ctx,
                             const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
                             const int32_t *cutoff, const hb_params *params,
@@ -774,7 +773,7 @@ int hb_query_prepare_masked(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t 
         return HB_ERR_UNSUPPORTED;
     }
     q->lds_fits = fixed + hb_scan_wave_lds(D, 256) <= c->max_lds;
-    q->gstate = state_auto_global(c, len, D, fixed);
+    q->gstate = hb_state_auto_global(c, D, fixed);
     std::lock_guard<std::mutex> lk(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) {
         delete q;
@@ -799,11 +798,10 @@ int hb_query_prepare_masked(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t 
 int hb_query_set_state_mode(hb_query *q, int mode)
 {
     if (!q) return HB_ERR_ARG;
-    const int32_t len = q->gs.qlen_pad_max;
     switch (mode) {
     case HB_STATE_AUTO: {
         const size_t fixed = hb_scan_fixed_lds(q->gs.qlen_pad_max, q->gs.ctx_pad_max);
-        q->gstate = state_auto_global(q->ctx, len, q->gs.diag_len_max, fixed);
+        q->gstate = hb_state_auto_global(q->ctx, q->gs.diag_len_max, fixed);
         return HB_OK;
     }
     case HB_STATE_LDS:

@@ -102,3 +102,40 @@ def test_gapped_stage_on_a_long_query(ctx):
     assert len(g) > 0
     assert np.array_equal(gst["n_hsp"], cnt[:, 0])
     assert np.array_equal(g.view(np.int32), ref.view(np.int32))
+
+
+def test_batched_and_served_long_queries(ctx):
+    """Long queries in the batched and served forms (their size classes keep
+    the diagonal state in HBM) give the plain path's HSPs, next to short
+    queries that keep it in LDS."""
+    pq = hbseed.random_protein(3401, 2000)
+    res, offs = hbseed.make_db(3400, 3000, plant_query=pq, plant_frac=0.2)
+    specs = [hbseed.blastp_query(hbseed.random_protein(3410 + i, 150 + 90 * i)) for i in range(6)]
+    long_p = hbseed.random_protein(3420, 40000)
+    long_p[5000:7000] = pq
+    specs.append(hbseed.blastp_query(long_p))
+    mid = hbseed.random_protein(3421, 20000)
+    mid[:2000] = pq
+    specs.append(hbseed.blastp_query(mid))
+    specs.append(hbseed.blastx_query(hbseed.blastx_nt(3422, 12000, pq[:100], pq[500:600])))
+    db = hbseed.Database(ctx, res, offs)
+    try:
+        plain = [hbseed.search(ctx, db, s)[1] for s in specs]
+        assert sum(len(p) for p in plain[6:]) > 0
+        b = hbseed.Batch(ctx, db, group_residues=2048)
+        ids = [b.add(s) for s in specs]
+        b.run()
+        print("batch", b.info(), flush=True)
+        for qid, p in zip(ids, plain):
+            assert np.array_equal(b.results(qid).view(np.int32), p.view(np.int32)), qid
+        b.run()   # graph replay
+        for qid, p in zip(ids, plain):
+            assert np.array_equal(b.results(qid).view(np.int32), p.view(np.int32)), qid
+        b.close()
+        with hbseed.Server(ctx, db) as srv:
+            tickets = [srv.submit(s) for s in specs]
+            for t, p in zip(tickets, plain):
+                assert srv.wait(t, 120000)
+                assert np.array_equal(srv.results(t).view(np.int32), p.view(np.int32))
+    finally:
+        db.close()
