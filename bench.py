@@ -148,20 +148,25 @@ def measure_hdb(ctx, res, offs, reps=3, cpu=True, blocksx=512, threadsx=192):
     n_res = int(s_offs[n])
     # file read once, residues written, lengths written and read, offsets read
     abytes = data.size + n_res + 16 * n
-    call, k_len, k_copy = [], [], []
+    # the volume in page-locked memory, as a reader would put it there
+    pin = ctx.pinned(data.size)
+    pin.array[:] = data
+    call, call_pinned, k_len, k_copy = [], [], [], []
     for it in range(reps + 1):   # the first load also loads the kernels' code: untimed
-        t0 = time.perf_counter()
-        db = hbseed.Database.from_hdb(ctx, data, blocksx, threadsx, n, gl)
-        dt = (time.perf_counter() - t0) * 1e3
-        a, b = db.hdb_kernel_ms()
-        if it:
-            call.append(dt)
-            k_len.append(a)
-            k_copy.append(b)
-        ok = db.n == n and db.residues == n_res
-        db.close()
-        if not ok:
-            raise RuntimeError("hb_db_upload_hdb: loaded volume has the wrong size")
+        for src, calls in ((data, call), (pin.array, call_pinned)):
+            t0 = time.perf_counter()
+            db = hbseed.Database.from_hdb(ctx, src, blocksx, threadsx, n, gl)
+            dt = (time.perf_counter() - t0) * 1e3
+            a, b = db.hdb_kernel_ms()
+            if it:
+                calls.append(dt)
+                k_len.append(a)
+                k_copy.append(b)
+            ok = db.n == n and db.residues == n_res
+            db.close()
+            if not ok:
+                raise RuntimeError("hb_db_upload_hdb: loaded volume has the wrong size")
+    pin.close()
     kms = float(np.mean(k_len) + np.mean(k_copy))
     achieved = abytes / (kms * 1e-3)
     out = {"volume": f"{n} shortest of {len(s_offs) - 1} subjects, grid {blocksx}x{threadsx}, "
@@ -170,6 +175,8 @@ def measure_hdb(ctx, res, offs, reps=3, cpu=True, blocksx=512, threadsx=192):
            "lengths_kernel_ms": float(np.mean(k_len)), "copy_kernel_ms": float(np.mean(k_copy)),
            "call_ms": float(np.mean(call)),
            "call_gb_per_s": data.size / (np.mean(call) * 1e-3) / 1e9,
+           "call_pinned_ms": float(np.mean(call_pinned)),
+           "call_pinned_gb_per_s": data.size / (np.mean(call_pinned) * 1e-3) / 1e9,
            "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                         "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None}}
     if cpu:
@@ -182,6 +189,26 @@ def measure_hdb(ctx, res, offs, reps=3, cpu=True, blocksx=512, threadsx=192):
         out["cpu_baseline"] = {
             "value": abytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
             "sample": f"the whole {data.size}-byte volume, {dt:.2f} s on one thread"}
+    return out
+
+
+def measure_upload(ctx, res, offs, reps=2):
+    """hb_db_upload of the packed database from ordinary (pageable) memory and
+    from page-locked memory (hb_host_alloc): wall time of the whole call
+    (host-to-device copy, processing order, residue check)."""
+    pin = ctx.pinned(res.size)
+    pin.array[:] = res
+    out = {"bytes": int(res.size)}
+    for key, src in (("pageable_ms", res), ("pinned_ms", pin.array)):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            d = hbseed.Database(ctx, src, offs)
+            ts.append((time.perf_counter() - t0) * 1e3)
+            d.close()
+        out[key] = min(ts)
+    pin.close()
+    out["pinned_gb_per_s"] = res.size / (out["pinned_ms"] * 1e-3) / 1e9
     return out
 
 
@@ -436,10 +463,12 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(q, res, offs)
 
-    hdb = None
+    hdb = upload = None
     if not args.no_hdb:
         log("database as an .hDb volume")
         hdb = measure_hdb(ctx, res, offs, cpu=not args.no_cpu)
+        log("database upload from pageable and page-locked memory")
+        upload = measure_upload(ctx, res, offs)
 
     # compact batched and served measurements on the same resident database
     forms = None
@@ -485,6 +514,8 @@ def main():
         out["gapped"] = gapped
     if hdb:
         out["hdb_load"] = hdb
+    if upload:
+        out["db_upload"] = upload
     if forms:
         out["forms"] = forms
     print(json.dumps(out), flush=True)

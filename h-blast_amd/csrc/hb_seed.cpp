@@ -469,6 +469,27 @@ int hb_mem_info(hb_ctx *c, uint64_t *free_b, uint64_t *total_b)
     return HB_OK;
 }
 
+int hb_host_alloc(hb_ctx *c, uint64_t bytes, void **out)
+{
+    if (!c || !out) return HB_ERR_ARG;
+    *out = nullptr;
+    HB_CHECK(hipSetDevice(c->device));
+    if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return HB_ERR_NOMEM;
+    }
+    return HB_OK;
+}
+
+int hb_host_free(hb_ctx *c, void *p)
+{
+    if (!c) return HB_ERR_ARG;
+    if (!p) return HB_OK;
+    HB_CHECK(hipSetDevice(c->device));
+    HB_CHECK(hipHostFree(p));
+    return HB_OK;
+}
+
 int hb_synchronize(hb_ctx *c)
 {
     if (!c) return HB_ERR_ARG;
@@ -498,11 +519,28 @@ static int db_meta(hb_db *db)
     // Length-descending processing order, then tiles of <= 64 subjects and
     // ~16K residues: one queue pop per tile keeps atomics off the hot path
     // and the longest subjects start first.
-    std::vector<uint32_t> order(n);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        return db->h_offs[x + 1] - db->h_offs[x] > db->h_offs[y + 1] - db->h_offs[y];
-    });
+    // (stable LSD radix sort on the length, 11 bits a pass: a comparison
+    // sort reading scattered offsets took ~0.2 s for 2M subjects)
+    std::vector<uint32_t> order(n), tmp(n), key(n), key2(n);
+    uint32_t kmax = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        order[i] = i;
+        key[i] = (uint32_t)(db->h_offs[i + 1] - db->h_offs[i]);
+        kmax = std::max(kmax, key[i]);
+    }
+    for (uint32_t i = 0; i < n; ++i) key[i] = kmax - key[i];   // ascending key = descending length
+    for (int shift = 0; shift < 32 && (kmax >> shift) != 0; shift += 11) {
+        uint32_t cnt[2049] = {0};
+        for (uint32_t i = 0; i < n; ++i) ++cnt[((key[i] >> shift) & 2047u) + 1];
+        for (int b = 0; b < 2048; ++b) cnt[b + 1] += cnt[b];
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t d = cnt[(key[i] >> shift) & 2047u]++;
+            tmp[d] = order[i];
+            key2[d] = key[i];
+        }
+        order.swap(tmp);
+        key.swap(key2);
+    }
     std::vector<uint32_t> tiles;
     tiles.push_back(0);
     uint64_t acc = 0;

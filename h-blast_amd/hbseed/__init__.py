@@ -49,7 +49,7 @@ DEFAULT_GAP_XDROP = 38
 # Every symbol include/hb_seed.h declares (checked by the CPU tests).
 EXPORTS = [
     "hb_params_default", "hb_strerror", "hb_blosum62", "hb_create", "hb_destroy",
-    "hb_mem_info", "hb_synchronize", "hb_db_upload", "hb_db_free", "hb_db_subjects",
+    "hb_mem_info", "hb_synchronize", "hb_host_alloc", "hb_host_free", "hb_db_upload", "hb_db_free", "hb_db_subjects",
     "hb_db_residues", "hb_query_prepare", "hb_query_free", "hb_query_lut_info",
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_query_prepare_masked", "hb_query_set_state_mode",
@@ -141,6 +141,8 @@ def _lib():
         L.hb_destroy.argtypes = [vp]
         L.hb_mem_info.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
         L.hb_synchronize.argtypes = [vp]
+        L.hb_host_alloc.argtypes = [vp, u64, pp]
+        L.hb_host_free.argtypes = [vp, vp]
         L.hb_db_upload.argtypes = [vp, vp, vp, u32, pp]
         L.hb_db_free.argtypes = [vp]
         L.hb_db_subjects.restype = u32
@@ -330,6 +332,35 @@ class Context:
 
     def synchronize(self):
         _check(_lib().hb_synchronize(self.h), "hb_synchronize")
+
+    def pinned(self, nbytes):
+        """A page-locked host buffer (hb_host_alloc); .array is its uint8 view."""
+        return PinnedBuffer(self, nbytes)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class PinnedBuffer:
+    """Page-locked host memory from hb_host_alloc; free with close() (the
+    numpy view must not be used afterwards)."""
+
+    def __init__(self, ctx: Context, nbytes):
+        self.ctx = ctx
+        p = C.c_void_p()
+        _check(_lib().hb_host_alloc(ctx.h, int(nbytes), C.byref(p)), "hb_host_alloc")
+        self.p = p
+        self.array = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)),
+                                           shape=(max(int(nbytes), 1),))[:int(nbytes)]
+
+    def close(self):
+        if self.p:
+            self.array = None
+            _lib().hb_host_free(self.ctx.h, self.p)
+            self.p = None
 
     def __enter__(self):
         return self

@@ -106,6 +106,14 @@ int hb_destroy(hb_ctx *ctx);
  * replaces H_BLAST_check_memory, gpu_cpu_common.h:93-102). */
 int hb_mem_info(hb_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 int hb_synchronize(hb_ctx *ctx);
+/* Page-locked host memory (hipHostMalloc) for the arrays a caller hands to
+ * hb_db_upload / hb_db_upload_hdb: reading a database file straight into it
+ * lets the upload run at the host link's full rate instead of through the
+ * runtime's pageable staging (the reference reads volumes into ordinary
+ * memory and copies per call, h_blast.cu:587-653, 1094).  *out is NULL on
+ * failure; release with hb_host_free. */
+int hb_host_alloc(hb_ctx *ctx, uint64_t bytes, void **out);
+int hb_host_free(hb_ctx *ctx, void *p);
 
 /* --- packed subject database -------------------------------------------
  * Replaces Read_H_BLAST_GPU_DB + the per-call DB upload

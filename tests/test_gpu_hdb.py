@@ -146,3 +146,22 @@ def test_loader_full_c1_volume_round_trip(ctx):
     sys.stdout.flush()
     assert np.array_equal(o, s_offs)
     assert np.array_equal(r, s_res)
+
+
+def test_pinned_sources_load_the_same_database(ctx):
+    res, offs = sorted_db(21, 4000, 1, 700)
+    n = len(offs) - 1
+    data, gl = hbseed.hdb_pack(res, offs, n, 4, 192)
+    with ctx.pinned(data.size) as pv, ctx.pinned(res.size) as pr:
+        pv.array[:] = data
+        pr.array[:] = res
+        a = hbseed.Database.from_hdb(ctx, pv.array, 4, 192, n, gl)
+        b = hbseed.Database(ctx, pr.array, offs)
+        try:
+            ra, oa = a.fetch()
+            rb, ob = b.fetch()
+        finally:
+            a.close()
+            b.close()
+    assert np.array_equal(oa, offs) and np.array_equal(ra, res)
+    assert np.array_equal(ob, offs) and np.array_equal(rb, res)
