@@ -108,6 +108,11 @@ struct hb_launch_cfg {
 // per-workgroup LDS should keep the diagonal state in global memory: it
 // does not fit LDS, or it leaves too few wavefronts per CU.
 bool hb_state_auto_global(const hb_ctx *c, int32_t D, size_t fixed);
+// Global-memory diagonal state of one launch is capped at this size: a
+// launch for a very long query then runs fewer workgroups.
+constexpr size_t HB_GSTATE_MAX_BYTES = size_t(1) << 30;
+// Bytes of that state for diagonal arrays of D entries (upper bound).
+size_t hb_gstate_bytes_max(const hb_ctx *c, int32_t D);
 // gstate: the diagonal state goes to global memory, so the per-wave LDS
 // holds everything but the state.
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,

@@ -333,6 +333,10 @@ int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_
     }
     if (best_wpc == 0) return HB_ERR_UNSUPPORTED;
     int blocks = c->n_cu * std::max(cfg->blocks, 1);
+    if (gstate) {   // keep the global-memory state within HB_GSTATE_MAX_BYTES
+        const size_t per_block = 4 * (size_t)diag_len_max * (size_t)cfg->waves;
+        blocks = std::min(blocks, (int)std::max<size_t>(1, HB_GSTATE_MAX_BYTES / per_block));
+    }
     const uint32_t per_block = multi ? 1u : (uint32_t)cfg->waves;
     const int need = (int)((work_items + per_block - 1) / per_block);
     if (need < blocks) blocks = std::max(need, 1);
@@ -353,6 +357,12 @@ bool hb_state_auto_global(const hb_ctx *c, int32_t D, size_t fixed)
     // waves of one workgroup share the fixed part; up to 4 per workgroup
     const size_t waves = (c->max_lds - fixed) / per_wave;
     return (int)std::min<size_t>(waves, 16) < kGstateMinWaves;
+}
+
+size_t hb_gstate_bytes_max(const hb_ctx *c, int32_t D)
+{
+    const size_t full = (size_t)c->n_cu * 16 * 4 * (size_t)D;   // 16 wavefronts per CU
+    return std::min(full, std::max(HB_GSTATE_MAX_BYTES, 4 * 4 * (size_t)D));
 }
 
 struct hb_query {

@@ -82,8 +82,9 @@ struct hb_server {
 
 // Device bytes one request adds to a batch: its output reservation, the
 // tables of one group (a query is never split; it shares a group or opens
-// one of its own) and its lookup-table entries (about 30 per residue for
-// BLOSUM62 at T 11; 64 are reserved).
+// one of its own), its lookup-table entries (about 30 per residue for
+// BLOSUM62 at T 11; 64 are reserved) and, for a query long enough that its
+// group keeps the diagonal state in global memory, that state.
 static void estimate(const hb_server *s, Request *r)
 {
     const double cells = (double)r->residues * (double)s->db->total;
@@ -92,6 +93,9 @@ static void estimate(const hb_server *s, Request *r)
                                HB_PV_WORDS * sizeof(uint32_t);
     r->est_bytes = r->est_hsps * sizeof(hb_hsp) + group_fixed +
                    64 * sizeof(uint32_t) * (size_t)r->residues;
+    const int32_t D = (int32_t)((r->residues + s->window + 63) & ~63);
+    if (hb_state_auto_global(s->ctx, D, hb_scan_fixed_lds((int32_t)((r->residues + 15) & ~15), 8)))
+        r->est_bytes += hb_gstate_bytes_max(s->ctx, D);
 }
 
 static void finish_requests(hb_server *s, InFlight &f, int rc)
