@@ -1,8 +1,9 @@
 // Sanitizer driver (test infrastructure): random and malformed inputs through
-// the host-only C-ABI helpers (h-blast_amd/csrc/hb_seq.cpp) and the CPU
-// oracle (oracle/hbo.c, oracle/hbo_gap.c).  tests/test_sanitizers.py builds it
+// the host-only C-ABI helpers (h-blast_amd/csrc/hb_seq.cpp, hb_hdb.cpp) and
+// the CPU oracle (oracle/hbo.c, oracle/hbo_gap.c, oracle/hbo_hdb.c).  tests/test_sanitizers.py builds it
 // with -fsanitize=address,undefined (and the threaded oracle parts with
 // -fsanitize=thread) and fails on any report.  Exit status 0 = clean.
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -146,6 +147,85 @@ static void oracle(std::mt19937_64 &rng, int rounds, int threads)
     }
 }
 
+// .hDb writer against the oracle's writer and reader; .hDbI text round trip,
+// then random corruptions of it through the parser.
+static void hdb(std::mt19937_64 &rng, int rounds)
+{
+    for (int r = 0; r < rounds; ++r) {
+        const int32_t n = (int32_t)(rng() % 300);
+        std::vector<uint64_t> lens(n);
+        for (auto &l : lens) l = rng() % 200;
+        std::sort(lens.begin(), lens.end());
+        std::vector<uint64_t> offs(n + 1, 0);
+        for (int32_t i = 0; i < n; ++i) offs[i + 1] = offs[i] + lens[i];
+        std::vector<uint8_t> res(offs[n] + 1);
+        for (auto &c : res) c = (uint8_t)(1 + rng() % 24);
+        const int32_t bx = 1 + (int32_t)(rng() % 8), tx = 1 + (int32_t)(rng() % 70);
+        int32_t ng = 0;
+        uint64_t bytes = 0;
+        CHECK(hb_hdb_layout(offs.data(), n, bx, tx, nullptr, 0, &ng, &bytes) == HB_OK);
+        std::vector<int32_t> gl(ng + 1), gl_o(ng + 1);
+        CHECK(hb_hdb_layout(offs.data(), n, bx, tx, gl.data(), ng, &ng, &bytes) == HB_OK);
+        uint64_t bytes_o = 0;
+        CHECK(hbo_hdb_layout(offs.data(), n, bx, tx, gl_o.data(), &bytes_o) == 0);
+        CHECK(bytes == bytes_o);
+        std::vector<uint8_t> a(bytes + 4), b(bytes + 4);
+        CHECK(hb_hdb_pack(res.data(), offs.data(), n, bx, tx, gl.data(), ng, a.data(), bytes) ==
+              HB_OK);
+        CHECK(hbo_hdb_pack(res.data(), offs.data(), n, bx, tx, gl_o.data(), b.data(), bytes) == 0);
+        CHECK(std::memcmp(a.data(), b.data(), bytes) == 0);
+        std::vector<uint64_t> o2(n + 1);
+        std::vector<uint8_t> r2(offs[n] + 1);
+        CHECK(hbo_hdb_unpack(a.data(), bytes, n, bx, tx, gl.data(), r2.data(), o2.data()) == 0);
+        CHECK(o2 == offs);
+        // index text with two volumes
+        hb_hdb_volume v[2] = {};
+        for (int k = 0; k < 2; ++k) {
+            v[k].name = k ? "db.01.hDb" : "db.00.hDb";
+            v[k].bytes = bytes;
+            v[k].volume_seqs = n + k;
+            v[k].gpu_seqs = n;
+            v[k].n_groups = ng;
+            v[k].group_len = gl.data();
+        }
+        hb_hdb_index ix = {bx, tx, 99, 2};
+        uint64_t len = 0;
+        CHECK(hb_hdbi_format(&ix, v, nullptr, 0, &len) == HB_OK);
+        std::string t(len, '\0');
+        CHECK(hb_hdbi_format(&ix, v, &t[0], len, &len) == HB_OK);
+        for (int m = 0; m < 20; ++m) {
+            std::string u = t;
+            const int edits = m == 0 ? 0 : 1 + (int)(rng() % 4);
+            for (int e = 0; e < edits && !u.empty(); ++e) {
+                const size_t at = rng() % u.size();
+                switch (rng() % 4) {
+                case 0: u.erase(at, 1 + rng() % 8); break;
+                case 1: u.insert(at, 1, "0123456789#\n-x"[rng() % 15]); break;
+                case 2: u[at] = "0123456789#\n-x "[rng() % 16]; break;
+                default: u.resize(at); break;
+                }
+            }
+            hb_hdb_index jx;
+            uint64_t ngl = 0, eo = 0;
+            const int rc = hb_hdbi_parse(u.data(), u.size(), &jx, nullptr, 0, nullptr, 0, &ngl, &eo);
+            if (m == 0) CHECK(rc == HB_OK && jx.n_volumes == 2 && ngl == 2 * (uint64_t)ng);
+            if (rc == HB_ERR_FORMAT) CHECK(eo <= u.size());
+            if (rc != HB_OK || jx.n_volumes > 64 || ngl > (1u << 20)) continue;
+            std::vector<hb_hdb_volume> vs(jx.n_volumes + 1);
+            std::vector<int32_t> g(ngl + 1);
+            CHECK(hb_hdbi_parse(u.data(), u.size(), &jx, vs.data(), jx.n_volumes, g.data(), ngl,
+                                &ngl, &eo) == HB_OK);
+            if (jx.n_volumes > 0 && ngl > 0)   // too little room for the group lengths
+                CHECK(hb_hdbi_parse(u.data(), u.size(), &jx, vs.data(), jx.n_volumes, g.data(),
+                                    ngl - 1, &ngl, &eo) == HB_ERR_CAPACITY);
+            if (m == 0)
+                for (int k = 0; k < 2; ++k)
+                    CHECK(vs[k].n_groups == ng && vs[k].name_len == 9 &&
+                          std::memcmp(vs[k].group_len, gl.data(), 4 * (size_t)ng) == 0);
+        }
+    }
+}
+
 int main(int argc, char **argv)
 {
     const bool threads_only = argc > 1 && std::strcmp(argv[1], "threads") == 0;
@@ -156,6 +236,7 @@ int main(int argc, char **argv)
         fasta(rng, 3000);
         translate(rng, 300);
         oracle(rng, 60, 1);
+        hdb(rng, 300);
     }
     std::fprintf(stderr, "%s: %d failed checks\n", threads_only ? "threads" : "all", fails);
     return fails ? 1 : 0;
