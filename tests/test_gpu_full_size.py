@@ -82,3 +82,26 @@ def test_blastx_1500nt_full_database(c1):
     cnt, ref = oracle_lib.OracleQuery.from_spec(spec).scan(res, offs, threads=THREADS)
     check(stats, hsps, cnt, ref)
     assert set(np.unique(hsps["context"])) == set(range(6))
+
+
+def test_batched_c3_queries_full_database(c1):
+    """The first 24 queries of the 2,048-query batched workload (log-normal
+    lengths, median 350) in one batched scan of the whole database, each
+    against the oracle alone; the autotuned group size."""
+    ctx, db, q, res, offs = c1
+    lens = hbseed.lognormal_lengths(1300, 24, 350.0, 0.7, 50, 3000)
+    specs = [hbseed.blastp_query(hbseed.random_protein(1300 * 100003 + i, int(L)))
+             for i, L in enumerate(lens)]
+    specs[3] = hbseed.blastp_query(q[:400].copy())   # one query with planted homologs
+    b = hbseed.Batch(ctx, db)
+    ids = [b.add(s) for s in specs]
+    b.run()
+    print("batch", b.info(), flush=True)
+    total = 0
+    for qid, spec in zip(ids, specs):
+        got = b.results(qid)
+        _, ref = oracle_lib.OracleQuery.from_spec(spec).scan(res, offs, threads=THREADS)
+        assert np.array_equal(got.view(np.int32), ref.view(np.int32)), qid
+        total += len(got)
+    b.close()
+    assert total > 10_000
