@@ -18,7 +18,7 @@ struct Class {
     int32_t qlen_pad_max = 0, ctx_pad_max = 4, diag_len_max = 64;
     hb_launch_cfg cfg;
     uint32_t tiles_per_item = 1;
-    bool gstate = false;               // diagonal state in global memory
+    int place = HB_PLACE_LDS32;        // where the diagonal state goes
     int32_t *d_gstate = nullptr;       // diag_len_max int32 per wavefront of the grid
     size_t gstate_bytes = 0;
 };
@@ -194,12 +194,13 @@ static int build(hb_batch *b, int32_t target)
         const uint32_t n_groups = (uint32_t)(c.g_end - c.g_begin);
         // long queries (a group of one query longer than the target): the
         // diagonal state goes to global memory as in the plain form
-        c.gstate = hb_state_auto_global(b->ctx, c.diag_len_max,
-                                        hb_scan_fixed_lds(c.qlen_pad_max, c.ctx_pad_max));
+        c.place = hb_state_choose(b->ctx, c.diag_len_max,
+                                  hb_scan_fixed_lds(c.qlen_pad_max, c.ctx_pad_max),
+                                  b->db->max_len, b->p.window);
         rc = hb_scan_config(b->ctx, true, c.qlen_pad_max, c.ctx_pad_max, c.diag_len_max,
-                            n_groups * b->db->n_tiles, &c.cfg, c.gstate);
+                            n_groups * b->db->n_tiles, &c.cfg, c.place);
         if (rc) return rc;
-        if (c.gstate) {
+        if (c.place == HB_PLACE_GLOBAL) {
             c.gstate_bytes = (size_t)c.cfg.blocks * c.cfg.waves * (size_t)c.diag_len_max * 4;
             c.d_gstate = static_cast<int32_t *>(hb_pool_get(b->ctx, c.gstate_bytes));
             if (!c.d_gstate) return HB_ERR_NOMEM;
@@ -241,9 +242,11 @@ static int enqueue(hb_batch *b, uint32_t t_begin, uint32_t t_end)
         a.qlen_pad_max = C.qlen_pad_max;
         a.ctx_pad_max = C.ctx_pad_max;
         a.diag_len_max = C.diag_len_max;
-        a.wave_lds = (uint32_t)hb_scan_wave_lds(C.gstate ? 0 : C.diag_len_max, C.cfg.scap);
+        a.wave_lds = (uint32_t)hb_scan_wave_lds(C.diag_len_max, C.cfg.scap,
+                                                hb_state_bytes(C.place));
         a.scap = (uint32_t)C.cfg.scap;
         a.gstate = C.d_gstate;   // cleared by each wavefront when it binds a group
+        a.state_limit = C.place == HB_PLACE_LDS16 ? hb_state16_limit(db->max_len) : 0;
         a.mat32 = b->gs.d_mat;
         a.window = b->p.window;
         a.work_counter = b->d_misc + 1 + ci;

@@ -69,6 +69,9 @@ struct hb_scan_args {
     // the diagonal state in global memory, diag_len_max int32 per
     // wavefront of the grid (null: in LDS)
     int32_t *gstate;
+    // > 0: 16-bit diagonal state in LDS, reset when a subject's base offset
+    // + length + window reaches it (16384 - the longest subject); 0: 32-bit
+    int32_t state_limit;
 };
 
 // per wave: diagonal state, ordering markers, 64 segment tags, job and
@@ -77,10 +80,12 @@ struct hb_scan_args {
 // modulo 512 bytes is at least 128: distances of 0 or 64 bytes (mod 512)
 // made the C1 scan 8% slower on gfx950 (measured), which points to LDS bank
 // or port conflicts between wavefronts.
-static inline size_t hb_scan_wave_lds(int32_t diag_len, int32_t scap)
+// state_bytes: bytes per diagonal of LDS state (4, 2, or 0 when the state
+// is in global memory).
+static inline size_t hb_scan_wave_lds(int32_t diag_len, int32_t scap, int state_bytes = 4)
 {
-    size_t b = 4 * (size_t)diag_len + 4 * HB_MARK + 256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) +
-               2 * (size_t)scap;
+    size_t b = (size_t)state_bytes * (size_t)diag_len + 4 * HB_MARK + 256 +
+               8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + 2 * (size_t)scap;
     const size_t r = b % 512;
     if (r < 128) b += 128 - r;
     return b;
@@ -116,8 +121,8 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 // launching or capturing a launch with that LDS size).
 // Raises both scan kernels' dynamic LDS limit to max_lds (once per context).
 hipError_t hb_scan_allow_lds(size_t max_lds);
-hipError_t hb_scan_occupancy(bool multi, bool gstate, size_t lds_bytes, int block_threads,
-                             int *blocks_per_cu);
+hipError_t hb_scan_occupancy(bool multi, bool gstate, bool w16, size_t lds_bytes,
+                             int block_threads, int *blocks_per_cu);
 hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);
 hipError_t hb_launch_fill_u32(uint32_t *dst, uint32_t value, uint32_t n, hipStream_t st);
 

@@ -344,8 +344,14 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-template <bool G>
+template <bool G, bool W16>
 struct WaveScan {
+    // State layout: W16 = false: int32, bit 31 flag, bit 30 pending, 30-bit
+    // position; W16 = true: uint16, bit 15 flag, bit 14 pending, 14-bit
+    // position (the host resets the state often enough: a->state_limit).
+    static constexpr uint32_t kF = W16 ? 0x8000u : kFlag;
+    static constexpr uint32_t kP = W16 ? 0x4000u : kPend;
+    static constexpr uint32_t kV = W16 ? 0x3fffu : kVal;
     const hb_scan_args *a;
     // group being scanned
     // Lookup-table pointers come from a descriptor in memory, so the
@@ -365,6 +371,7 @@ struct WaveScan {
     const int32_t *ctx;   // ctx_off[ctx_pad], xdrop[ctx_pad], cutoff[ctx_pad]
     int32_t ctx_pad;
     int32_t *state;
+    uint16_t *state16;   // W16
     uint32_t *marker;
     uint32_t *seg;
     uint2 *jobs;
@@ -407,6 +414,8 @@ struct WaveScan {
     {
         if constexpr (G)
             return __hip_atomic_load(&state[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else if constexpr (W16)
+            return (int32_t)(uint32_t)state16[d];
         else
             return state[d];
     }
@@ -414,6 +423,8 @@ struct WaveScan {
     {
         if constexpr (G)
             __hip_atomic_store(&state[d], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else if constexpr (W16)
+            state16[d] = (uint16_t)v;
         else
             state[d] = v;
     }
@@ -463,19 +474,21 @@ struct WaveScan {
         const int32_t st = st_ld(d);
         const bool win = active && mk == key;
         const uint32_t ust = (uint32_t)st;
-        const bool pend = (ust & kPend) != 0;
-        const bool flag = st < 0;
+        const bool pend = (ust & kP) != 0;
+        bool flag;
+        if constexpr (W16) flag = (ust & kF) != 0;
+        else flag = st < 0;
         const int32_t sb = so + (sl ? s_base1 : s_base0);
         const int32_t diff = (int32_t)((uint32_t)sb - ust);  // s - last if unflagged
         const bool job = win && !pend && !flag && diff >= 3 && diff < window;
         const bool wait = win && pend;
         int32_t nst;
-        if (flag) nst = sb >= (int32_t)(ust & kVal) ? sb : st;
-        else nst = diff >= window ? sb : (job ? (int32_t)kPend : st);
+        if (flag) nst = sb >= (int32_t)(ust & kV) ? sb : st;
+        else nst = diff >= window ? sb : (job ? (int32_t)kP : st);
         if (win && !pend && nst != st) st_st(d, nst);
         sync();
         const bool lose = active && !win;
-        const bool lwait = lose && ((uint32_t)st_ld(d) & kPend) != 0;
+        const bool lwait = lose && ((uint32_t)st_ld(d) & kP) != 0;
         const uint64_t mj = __ballot(job), mb = __ballot(wait || lwait);
         const uint32_t tso = (uint32_t)so | (sl ? 0x80000000u : 0u);
         if (job) {
@@ -526,7 +539,7 @@ struct WaveScan {
                     r = ext_two_hit<false>(mat, S, slen, qry, qlen, last_s + 3, so, q,
                                            ctx[ctx_pad + k]);
                 st_st(diag_of((uint32_t)q, so),
-                      r.right ? (int32_t)(kFlag | (uint32_t)(r.s_last - 2 + b)) : so + b);
+                      r.right ? (int32_t)(kF | (uint32_t)(r.s_last - 2 + b)) : so + b);
             }
             const bool emit = v && r.score >= ctx[2 * ctx_pad + k];
             const uint64_t m = __ballot(emit);
@@ -719,7 +732,9 @@ struct WaveScan {
         }
         if (csl) s_hits1 = hits;
         else s_hits0 = hits;
-        const bool reset = base + slen + window >= (1 << 29);
+        bool reset;
+        if constexpr (W16) reset = base + slen + window >= a->state_limit;
+        else reset = base + slen + window >= (1 << 29);
         if (SL && ds < 0 && n_jobs < kDeferJobs && !reset) {
             // few extensions queued: they run with the next subject's
             ds = cs;
@@ -779,8 +794,8 @@ __device__ __forceinline__ void load_group(const hb_scan_args &a, const hb_group
     }
 }
 
-template <bool GS>
-__device__ __forceinline__ void bind_group(WaveScan<GS> &w, const hb_group_desc &G)
+template <bool GS, bool W16>
+__device__ __forceinline__ void bind_group(WaveScan<GS, W16> &w, const hb_group_desc &G)
 {
     w.cells = (const HB_GLOBAL uint64_t *)G.cells;
     w.entries = (const HB_GLOBAL uint32_t *)G.entries;
@@ -801,7 +816,9 @@ __device__ __forceinline__ void bind_group(WaveScan<GS> &w, const hb_group_desc 
 // GSTATE: the diagonal state lives in global memory (a.gstate, diag_len_max
 // int32 per wavefront of the grid) instead of LDS, for queries (groups)
 // whose state does not fit LDS or would leave it too few wavefronts.
-template <bool MULTI, bool GSTATE>
+// W16: 16-bit state in LDS (databases whose subjects are short enough for
+// 14-bit positions between resets).
+template <bool MULTI, bool GSTATE, bool W16>
 __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void hb_scan_kernel(hb_scan_args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -817,7 +834,7 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     for (int i = threadIdx.x; i < 256; i += blockDim.x)
         reinterpret_cast<uint32_t *>(sm_mat)[i] = reinterpret_cast<const uint32_t *>(a.mat32)[i];
 
-    WaveScan<GSTATE> w;
+    WaveScan<GSTATE, W16> w;
     w.a = &a;
     w.mat = sm_mat;
     w.qry = sm_q;
@@ -826,6 +843,9 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     if constexpr (GSTATE) {
         w.state = a.gstate + ((size_t)blockIdx.x * (blockDim.x / HB_WAVE) + (size_t)wid) * (size_t)Dm;
         w.marker = reinterpret_cast<uint32_t *>(wb);
+    } else if constexpr (W16) {
+        w.state16 = reinterpret_cast<uint16_t *>(wb);
+        w.marker = reinterpret_cast<uint32_t *>(wb + 2 * (size_t)Dm);   // Dm: multiple of 64
     } else {
         w.state = reinterpret_cast<int32_t *>(wb);
         w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)Dm);
@@ -1165,13 +1185,17 @@ hipError_t hb_launch_lut(const hb_lut_job *jobs_dev, int n_jobs, int32_t qlen_ma
     return hipGetLastError();
 }
 
-static const void *scan_fn(bool multi, bool gstate)
+// Instantiations: (MULTI, GSTATE, W16) with W16 only for LDS state.
+static const void *scan_fn(bool multi, bool gstate, bool w16)
 {
-    if (multi)
-        return gstate ? reinterpret_cast<const void *>(hb_scan_kernel<true, true>)
-                      : reinterpret_cast<const void *>(hb_scan_kernel<true, false>);
-    return gstate ? reinterpret_cast<const void *>(hb_scan_kernel<false, true>)
-                  : reinterpret_cast<const void *>(hb_scan_kernel<false, false>);
+    if (gstate)
+        return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, true, false>)
+                     : reinterpret_cast<const void *>(hb_scan_kernel<false, true, false>);
+    if (w16)
+        return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, false, true>)
+                     : reinterpret_cast<const void *>(hb_scan_kernel<false, false, true>);
+    return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, false, false>)
+                 : reinterpret_cast<const void *>(hb_scan_kernel<false, false, false>);
 }
 
 hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int block_threads,
@@ -1179,25 +1203,28 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 {
     if (block_threads < HB_WAVE || block_threads > HB_BLOCK || block_threads % HB_WAVE)
         return hipErrorInvalidValue;
-    if (multi && a.gstate)
-        hipLaunchKernelGGL((hb_scan_kernel<true, true>), dim3(n_blocks), dim3(block_threads),
-                           lds_bytes, st, a);
-    else if (multi)
-        hipLaunchKernelGGL((hb_scan_kernel<true, false>), dim3(n_blocks), dim3(block_threads),
-                           lds_bytes, st, a);
-    else if (a.gstate)
-        hipLaunchKernelGGL((hb_scan_kernel<false, true>), dim3(n_blocks), dim3(block_threads),
-                           lds_bytes, st, a);
-    else
-        hipLaunchKernelGGL((hb_scan_kernel<false, false>), dim3(n_blocks), dim3(block_threads),
-                           lds_bytes, st, a);
+    const bool g = a.gstate != nullptr, w16 = a.state_limit > 0;
+    if (g && w16) return hipErrorInvalidValue;
+    const dim3 grid(n_blocks), block(block_threads);
+#define HB_SCAN_LAUNCH(M, GS, W)                                                     \
+    hipLaunchKernelGGL((hb_scan_kernel<M, GS, W>), grid, block, lds_bytes, st, a)
+    if (multi) {
+        if (g) HB_SCAN_LAUNCH(true, true, false);
+        else if (w16) HB_SCAN_LAUNCH(true, false, true);
+        else HB_SCAN_LAUNCH(true, false, false);
+    } else {
+        if (g) HB_SCAN_LAUNCH(false, true, false);
+        else if (w16) HB_SCAN_LAUNCH(false, false, true);
+        else HB_SCAN_LAUNCH(false, false, false);
+    }
+#undef HB_SCAN_LAUNCH
     return hipGetLastError();
 }
 
 hipError_t hb_scan_allow_lds(size_t max_lds)
 {
-    for (int k = 0; k < 4; ++k) {
-        const void *fn = scan_fn((k & 1) != 0, (k & 2) != 0);
+    for (int k = 0; k < 6; ++k) {
+        const void *fn = scan_fn((k & 1) != 0, k / 2 == 1, k / 2 == 2);
         hipFuncAttributes fa;
         hipError_t e = hipFuncGetAttributes(&fa, fn);
         if (e != hipSuccess) return e;
@@ -1209,9 +1236,9 @@ hipError_t hb_scan_allow_lds(size_t max_lds)
     return hipSuccess;
 }
 
-hipError_t hb_scan_occupancy(bool multi, bool gstate, size_t lds_bytes, int block_threads,
-                             int *blocks_per_cu)
+hipError_t hb_scan_occupancy(bool multi, bool gstate, bool w16, size_t lds_bytes,
+                             int block_threads, int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_fn(multi, gstate),
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_fn(multi, gstate, w16),
                                                         block_threads, lds_bytes);
 }

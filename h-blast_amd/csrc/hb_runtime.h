@@ -56,6 +56,7 @@ struct hb_db {
     uint32_t *d_tiles = nullptr;
     uint32_t n_tiles = 0;
     std::vector<uint64_t> h_offs;
+    uint64_t max_len = 0;   // longest subject
     float hdb_len_ms = 0.f, hdb_copy_ms = 0.f;   // hb_db_upload_hdb's kernels
 };
 
@@ -104,20 +105,35 @@ struct hb_launch_cfg {
     int blocks = 0;
     int32_t scap = 1024;   // staged-subject capacity per wave
 };
-// True when a launch with diagonal arrays of D entries and `fixed` bytes of
-// per-workgroup LDS should keep the diagonal state in global memory: it
-// does not fit LDS, or it leaves too few wavefronts per CU.
-bool hb_state_auto_global(const hb_ctx *c, int32_t D, size_t fixed);
+// Where a launch keeps its per-diagonal two-hit state.
+enum hb_state_place { HB_PLACE_LDS32 = 0, HB_PLACE_LDS16 = 1, HB_PLACE_GLOBAL = 2 };
+static inline int hb_state_bytes(int place)
+{
+    return place == HB_PLACE_LDS32 ? 4 : place == HB_PLACE_LDS16 ? 2 : 0;
+}
+// 16-bit state: positions of 14 bits between resets, so a subject and its
+// successor must fit: 2 * (longest subject + window) < 16384.
+static inline bool hb_state16_ok(uint64_t max_len, int32_t window)
+{
+    return 2 * (max_len + (uint64_t)window) + 64 < 16384;
+}
+// The reset threshold of a 16-bit state (hb_scan_args::state_limit).
+static inline int32_t hb_state16_limit(uint64_t max_len) { return 16384 - (int32_t)max_len; }
+// Automatic placement for diagonal arrays of D entries and `fixed` bytes of
+// per-workgroup LDS on a database whose longest subject is max_len: 32-bit
+// LDS state when it leaves 16 wavefronts per CU, else 16-bit LDS state (if
+// the subjects allow it) when that leaves at least 3, else 32-bit LDS state
+// when that leaves at least 6, else global memory.
+int hb_state_choose(const hb_ctx *c, int32_t D, size_t fixed, uint64_t max_len, int32_t window);
 // Global-memory diagonal state of one launch is capped at this size: a
 // launch for a very long query then runs fewer workgroups.
 constexpr size_t HB_GSTATE_MAX_BYTES = size_t(1) << 30;
 // Bytes of that state for diagonal arrays of D entries (upper bound).
 size_t hb_gstate_bytes_max(const hb_ctx *c, int32_t D);
-// gstate: the diagonal state goes to global memory, so the per-wave LDS
-// holds everything but the state.
+// place: where the diagonal state goes (LDS bytes per diagonal follow).
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg,
-                   bool gstate = false);
+                   int place = HB_PLACE_LDS32);
 
 // Asynchronous steps of a batch (hb_batch.cpp), used by the serving loop:
 // the batch runs on the given stream (nullptr: the context's); launch

@@ -310,17 +310,18 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
 // wavefronts with 320-byte staging ran 8% slower than 16 with 1024), then
 // more wavefronts per workgroup (they share the query tables).
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
-                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg, bool gstate)
+                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg, int place)
 {
     const size_t fixed = hb_scan_fixed_lds(qlen_pad_max, ctx_pad_max);
-    const int32_t lds_diag = gstate ? 0 : diag_len_max;
+    const bool gstate = place == HB_PLACE_GLOBAL, w16 = place == HB_PLACE_LDS16;
+    const int sb = hb_state_bytes(place);
     int best_wpc = 0;
     for (int waves : {4, 2, 1}) {
         for (int32_t scap = HB_SCAP_MAX; scap >= 256; scap -= 64) {
-            const size_t lds = fixed + waves * hb_scan_wave_lds(lds_diag, scap);
+            const size_t lds = fixed + waves * hb_scan_wave_lds(diag_len_max, scap, sb);
             if (lds > c->max_lds) continue;
             int per_cu = 0;
-            HB_CHECK(hb_scan_occupancy(multi, gstate, lds, waves * HB_WAVE, &per_cu));
+            HB_CHECK(hb_scan_occupancy(multi, gstate, w16, lds, waves * HB_WAVE, &per_cu));
             const int wpc = std::min(per_cu * waves, 16);
             if (wpc > best_wpc || (wpc == best_wpc && scap > cfg->scap)) {
                 best_wpc = wpc;
@@ -344,19 +345,29 @@ int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_
     return HB_OK;
 }
 
-// Automatic placement of the diagonal state: LDS unless it does not fit or
-// the LDS it takes leaves fewer than kGstateMinWaves wavefronts per CU.
-// Measured on MI355X, 200,000 subjects (plain BLASTP, ms LDS / global):
-// 4,000 residues (7 wavefronts per CU with LDS state) 10.7 / 13.3;
-// 6,000 (5) 23.4 / 21.8; 12,000 77.8 / 44.1; 31,000 359 / 163.
+// Automatic placement of the diagonal state.  Measured on MI355X, 200,000
+// subjects, plain BLASTP, ms with 32-bit LDS / 16-bit LDS / global state:
+// 2,000 residues 4.05 / 3.28 / 4.87; 4,000 10.7 / 8.34 / 13.2;
+// 8,000 29.6 / 19.6 / 29.7; 16,000 102 / 59.3 / 76.0 (16-bit: 3 wavefronts
+// per CU); 24,000 284 / 147 / 128 (16-bit: 2).  32-bit LDS state beats
+// global memory down to about 6 wavefronts per CU, 16-bit down to 3.
 static constexpr int kGstateMinWaves = 6;
-bool hb_state_auto_global(const hb_ctx *c, int32_t D, size_t fixed)
+static constexpr int kLds16MinWaves = 3;
+static int lds_waves(const hb_ctx *c, int32_t D, size_t fixed, int state_bytes)
 {
-    const size_t per_wave = hb_scan_wave_lds(D, 256);
-    if (fixed + per_wave > c->max_lds) return true;
+    const size_t per_wave = hb_scan_wave_lds(D, 256, state_bytes);
+    if (fixed + per_wave > c->max_lds) return 0;
     // waves of one workgroup share the fixed part; up to 4 per workgroup
-    const size_t waves = (c->max_lds - fixed) / per_wave;
-    return (int)std::min<size_t>(waves, 16) < kGstateMinWaves;
+    return (int)std::min<size_t>((c->max_lds - fixed) / per_wave, 16);
+}
+
+int hb_state_choose(const hb_ctx *c, int32_t D, size_t fixed, uint64_t max_len, int32_t window)
+{
+    const int w32 = lds_waves(c, D, fixed, 4);
+    if (w32 >= 16) return HB_PLACE_LDS32;
+    if (hb_state16_ok(max_len, window) && lds_waves(c, D, fixed, 2) >= kLds16MinWaves)
+        return HB_PLACE_LDS16;
+    return w32 >= kGstateMinWaves ? HB_PLACE_LDS32 : HB_PLACE_GLOBAL;
 }
 
 size_t hb_gstate_bytes_max(const hb_ctx *c, int32_t D)
@@ -371,8 +382,9 @@ struct hb_query {
     hb_group_set gs;
     uint32_t n_cells_used = 0;
     std::vector<int32_t> ctx_off, ctx_len;   // host copies (gapped stage)
-    bool gstate = false;   // diagonal state kept in global memory
-    bool lds_fits = true;  // the LDS layout can hold the diagonal state
+    int mode = HB_STATE_AUTO;   // requested placement of the diagonal state
+    bool lds_fits = true;       // the LDS layout can hold a 32-bit state
+    bool lds16_fits = true;     // ... a 16-bit state
 };
 
 struct hb_scan {
@@ -390,6 +402,7 @@ struct hb_scan {
     uint32_t gap_wide = 0;     // subjects it ran in the wide pass
     int32_t *d_gstate = nullptr;   // per-wavefront diagonal state (long queries)
     size_t gstate_bytes = 0;
+    int place = HB_PLACE_LDS32;    // where this scan keeps the diagonal state
 };
 
 extern "C" {
@@ -538,6 +551,7 @@ static int db_meta(hb_db *db)
         key[i] = (uint32_t)(db->h_offs[i + 1] - db->h_offs[i]);
         kmax = std::max(kmax, key[i]);
     }
+    db->max_len = kmax;
     for (uint32_t i = 0; i < n; ++i) key[i] = kmax - key[i];   // ascending key = descending length
     for (int shift = 0; shift < 32 && (kmax >> shift) != 0; shift += 11) {
         uint32_t cnt[2049] = {0};
@@ -820,8 +834,8 @@ int hb_query_prepare_masked(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t 
         delete q;
         return HB_ERR_UNSUPPORTED;
     }
-    q->lds_fits = fixed + hb_scan_wave_lds(D, 256) <= c->max_lds;
-    q->gstate = hb_state_auto_global(c, D, fixed);
+    q->lds_fits = fixed + hb_scan_wave_lds(D, 256, 4) <= c->max_lds;
+    q->lds16_fits = fixed + hb_scan_wave_lds(D, 256, 2) <= c->max_lds;
     std::lock_guard<std::mutex> lk(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) {
         delete q;
@@ -847,16 +861,16 @@ int hb_query_set_state_mode(hb_query *q, int mode)
 {
     if (!q) return HB_ERR_ARG;
     switch (mode) {
-    case HB_STATE_AUTO: {
-        const size_t fixed = hb_scan_fixed_lds(q->gs.qlen_pad_max, q->gs.ctx_pad_max);
-        q->gstate = hb_state_auto_global(q->ctx, q->gs.diag_len_max, fixed);
-        return HB_OK;
-    }
+    case HB_STATE_AUTO:
+    case HB_STATE_GLOBAL: q->mode = mode; return HB_OK;
     case HB_STATE_LDS:
         if (!q->lds_fits) return HB_ERR_UNSUPPORTED;
-        q->gstate = false;
+        q->mode = mode;
         return HB_OK;
-    case HB_STATE_GLOBAL: q->gstate = true; return HB_OK;
+    case HB_STATE_LDS16:
+        if (!q->lds16_fits) return HB_ERR_UNSUPPORTED;
+        q->mode = mode;
+        return HB_OK;
     default: return HB_ERR_ARG;
     }
 }
@@ -864,7 +878,22 @@ int hb_query_set_state_mode(hb_query *q, int mode)
 int hb_query_state_mode(const hb_query *q)
 {
     if (!q) return HB_ERR_ARG;
-    return q->gstate ? HB_STATE_GLOBAL : HB_STATE_LDS;
+    return q->mode;
+}
+
+// The placement a scan of q over db uses.
+static int state_place(const hb_query *q, const hb_db *db)
+{
+    switch (q->mode) {
+    case HB_STATE_LDS: return HB_PLACE_LDS32;
+    case HB_STATE_LDS16:
+        return hb_state16_ok(db->max_len, q->p.window) ? HB_PLACE_LDS16 : -1;
+    case HB_STATE_GLOBAL: return HB_PLACE_GLOBAL;
+    default:
+        return hb_state_choose(q->ctx, q->gs.diag_len_max,
+                               hb_scan_fixed_lds(q->gs.qlen_pad_max, q->gs.ctx_pad_max),
+                               db->max_len, q->p.window);
+    }
 }
 
 int hb_query_lut_info(const hb_query *q, uint32_t *n_entries, uint32_t *n_cells_used)
@@ -901,10 +930,13 @@ static int launch_scan(hb_scan *s)
     hb_db *db = s->db;
     hb_query *q = s->q;
     hb_launch_cfg cfg;
+    const int place = state_place(q, db);
+    if (place < 0) return HB_ERR_UNSUPPORTED;   // 16-bit state asked for, subjects too long
+    s->place = place;
     int rc = hb_scan_config(c, false, q->gs.qlen_pad_max, q->gs.ctx_pad_max, q->gs.diag_len_max,
-                            db->n_tiles, &cfg, q->gstate);
+                            db->n_tiles, &cfg, place);
     if (rc) return rc;
-    if (q->gstate && !s->d_gstate) {
+    if (place == HB_PLACE_GLOBAL && !s->d_gstate) {
         s->gstate_bytes = (size_t)cfg.blocks * cfg.waves * (size_t)q->gs.diag_len_max * 4;
         s->d_gstate = (int32_t *)hb_pool_get(c, s->gstate_bytes);
         if (!s->d_gstate) return HB_ERR_NOMEM;
@@ -922,9 +954,10 @@ static int launch_scan(hb_scan *s)
     a.qlen_pad_max = q->gs.qlen_pad_max;
     a.ctx_pad_max = q->gs.ctx_pad_max;
     a.diag_len_max = q->gs.diag_len_max;
-    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gstate ? 0 : q->gs.diag_len_max, cfg.scap);
+    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gs.diag_len_max, cfg.scap, hb_state_bytes(place));
     a.scap = (uint32_t)cfg.scap;
-    a.gstate = s->d_gstate;   // cleared by each wavefront before use
+    a.gstate = place == HB_PLACE_GLOBAL ? s->d_gstate : nullptr;   // cleared by each wavefront
+    a.state_limit = place == HB_PLACE_LDS16 ? hb_state16_limit(db->max_len) : 0;
     a.mat32 = q->gs.d_mat;
     a.window = q->p.window;
     a.work_counter = s->d_misc + 1;
@@ -953,6 +986,14 @@ int hb_scan_free(hb_scan *s)
     if (s->ev1) hipEventDestroy(s->ev1);
     delete s;
     return HB_OK;
+}
+
+int hb_scan_state_mode(const hb_scan *s)
+{
+    if (!s) return HB_ERR_ARG;
+    return s->place == HB_PLACE_GLOBAL  ? HB_STATE_GLOBAL
+           : s->place == HB_PLACE_LDS16 ? HB_STATE_LDS16
+                                        : HB_STATE_LDS;
 }
 
 int hb_scan_start(hb_ctx *c, hb_db *db, hb_query *q, hb_scan **out)

@@ -94,7 +94,8 @@ static void estimate(const hb_server *s, Request *r)
     r->est_bytes = r->est_hsps * sizeof(hb_hsp) + group_fixed +
                    64 * sizeof(uint32_t) * (size_t)r->residues;
     const int32_t D = (int32_t)((r->residues + s->window + 63) & ~63);
-    if (hb_state_auto_global(s->ctx, D, hb_scan_fixed_lds((int32_t)((r->residues + 15) & ~15), 8)))
+    if (hb_state_choose(s->ctx, D, hb_scan_fixed_lds((int32_t)((r->residues + 15) & ~15), 8),
+                        s->db->max_len, s->window) == HB_PLACE_GLOBAL)
         r->est_bytes += hb_gstate_bytes_max(s->ctx, D);
 }
 

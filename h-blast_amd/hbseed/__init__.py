@@ -29,7 +29,7 @@ ERRORS = {
 }
 DEFAULT_XDROP = 16
 DEFAULT_CUTOFF = 41
-STATE_AUTO, STATE_LDS, STATE_GLOBAL = 0, 1, 2
+STATE_AUTO, STATE_LDS, STATE_GLOBAL, STATE_LDS16 = 0, 1, 2, 3
 STDAA = "-ABCDEFGHIKLMNPQRSTVWXYZU*OJ"
 
 HSP_DTYPE = np.dtype([("oid", "<u4"), ("context", "<i4"), ("q_start", "<i4"),
@@ -53,7 +53,7 @@ EXPORTS = [
     "hb_db_residues", "hb_query_prepare", "hb_query_free", "hb_query_lut_info",
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_query_prepare_masked", "hb_query_set_state_mode",
-    "hb_query_state_mode", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
+    "hb_query_state_mode", "hb_scan_state_mode", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
     "hb_fasta_parse", "hb_hdb_gpu_seqs", "hb_hdb_layout", "hb_hdb_pack", "hb_hdbi_format",
@@ -154,6 +154,7 @@ def _lib():
         L.hb_query_free.argtypes = [vp]
         L.hb_query_set_state_mode.argtypes = [vp, C.c_int]
         L.hb_query_state_mode.argtypes = [vp]
+        L.hb_scan_state_mode.argtypes = [vp]
         L.hb_query_lut_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
         L.hb_query_lut_fetch.argtypes = [vp, vp, vp, u32]
         L.hb_blastx_concat_len.restype = i32
@@ -576,8 +577,8 @@ class Query:
         self.h = h
 
     def set_state_mode(self, mode):
-        """STATE_AUTO / STATE_LDS / STATE_GLOBAL: where the scan keeps the
-        per-diagonal two-hit state (hb_query_set_state_mode)."""
+        """STATE_AUTO / STATE_LDS / STATE_LDS16 / STATE_GLOBAL: where scans
+        keep the per-diagonal two-hit state (hb_query_set_state_mode)."""
         _check(_lib().hb_query_set_state_mode(self.h, mode), "hb_query_set_state_mode")
 
     @property
@@ -608,6 +609,12 @@ class Scan:
 
     def wait(self):
         _check(_lib().hb_scan_wait(self.h), "hb_scan_wait")
+
+    @property
+    def state_mode(self):
+        """Where this scan keeps its diagonal state (STATE_LDS / STATE_LDS16 /
+        STATE_GLOBAL)."""
+        return _lib().hb_scan_state_mode(self.h)
 
     def elapsed_ms(self):
         ms = C.c_float()

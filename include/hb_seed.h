@@ -159,9 +159,10 @@ int hb_query_free(hb_query *q);
 /* Where a plain scan of q keeps its per-diagonal two-hit state (the
  * reference keeps it in global memory, ewp->diag_table, one array per
  * thread: h_blast.cu:959-966):
- *   HB_STATE_AUTO   - LDS when the query is short enough for LDS to hold
- *                     it without starving the CU of wavefronts, else global
- *                     memory (the default);
+ *   HB_STATE_AUTO   - LDS (32-bit entries, or 16-bit ones when those leave
+ *                     the CU more wavefronts and the database's subjects
+ *                     allow them) unless LDS would starve the CU of
+ *                     wavefronts, else global memory (the default);
  *   HB_STATE_LDS    - LDS (HB_ERR_UNSUPPORTED if it does not fit);
  *   HB_STATE_GLOBAL - a global-memory array per wavefront.
  * Results are identical; only speed differs.  Takes effect for later
@@ -169,8 +170,11 @@ int hb_query_free(hb_query *q);
 #define HB_STATE_AUTO 0
 #define HB_STATE_LDS 1
 #define HB_STATE_GLOBAL 2
+#define HB_STATE_LDS16 3   /* 16-bit entries in LDS: databases whose longest
+                              subject is under ~8,100 residues (else the scan
+                              returns HB_ERR_UNSUPPORTED) */
 int hb_query_set_state_mode(hb_query *q, int mode);
-/* The placement hb_scan_start will use (HB_STATE_LDS or HB_STATE_GLOBAL). */
+/* The requested placement (HB_STATE_AUTO unless set). */
 int hb_query_state_mode(const hb_query *q);
 /* Number of (query offset, cell) entries of the lookup table. */
 int hb_query_lut_info(const hb_query *q, uint32_t *n_entries, uint32_t *n_cells_used);
@@ -295,6 +299,9 @@ int hb_scan_results(hb_scan *s, hb_subject_stats *stats, hb_hsp *hsps,
  * (gpu_cpu_common.h:71). */
 int hb_scan_elapsed_ms(hb_scan *s, float *ms);
 int hb_scan_free(hb_scan *s);
+/* Where the scan keeps its diagonal state: HB_STATE_LDS, HB_STATE_LDS16 or
+ * HB_STATE_GLOBAL (see hb_query_set_state_mode). */
+int hb_scan_state_mode(const hb_scan *s);
 
 /* Convenience: prepare + start + results for one query. */
 int hb_search(hb_ctx *ctx, hb_db *db, const uint8_t *seq, int32_t len,

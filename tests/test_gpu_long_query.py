@@ -19,14 +19,14 @@ def ctx():
     c.close()
 
 
-def run(ctx, db, spec, mode):
-    q = hbseed.Query(ctx, spec)
+def run(ctx, db, spec, mode, window=40):
+    q = hbseed.Query(ctx, spec, window=window)
     try:
         q.set_state_mode(mode)
-        assert q.state_mode == (hbseed.STATE_GLOBAL if mode == hbseed.STATE_GLOBAL
-                                else hbseed.STATE_LDS)
+        assert q.state_mode == mode
         s = hbseed.Scan(ctx, db, q)
         try:
+            assert s.state_mode == mode
             stats, hsps = s.results()
         finally:
             s.close()
@@ -43,11 +43,12 @@ def long_spec(kind, n, seed):
     return hbseed.blastx_query(nt)
 
 
-@pytest.mark.parametrize("kind,n,lds", [
-    ("blastp", 300, True), ("blastp", 2000, True), ("blastx", 6000, True),
-    ("blastp", 20000, True), ("blastp", 40000, False),
+@pytest.mark.parametrize("kind,n,lds,lds16", [
+    ("blastp", 300, True, True), ("blastp", 2000, True, True), ("blastx", 6000, True, True),
+    ("blastp", 20000, True, True), ("blastp", 40000, False, True),
+    ("blastp", 60000, False, False),
 ])
-def test_state_placements_match_each_other_and_oracle(ctx, kind, n, lds):
+def test_state_placements_match_each_other_and_oracle(ctx, kind, n, lds, lds16):
     spec = long_spec(kind, n, 3000 + n)
     print(f"{kind} query of {len(spec.seq)} residues ({spec.n_ctx} contexts)", flush=True)
     res, offs = hbseed.make_db(3100 + n, 2000, plant_query=spec.seq[:min(len(spec.seq), 3000)],
@@ -55,17 +56,20 @@ def test_state_placements_match_each_other_and_oracle(ctx, kind, n, lds):
     db = hbseed.Database(ctx, res, offs)
     try:
         sg, hg = run(ctx, db, spec, hbseed.STATE_GLOBAL)
-        if lds:
-            sl, hl = run(ctx, db, spec, hbseed.STATE_LDS)
-            assert np.array_equal(hl.view(np.int32), hg.view(np.int32))
-            assert np.array_equal(sl.view(np.uint8), sg.view(np.uint8))
-        else:
+        for mode, fits in ((hbseed.STATE_LDS, lds), (hbseed.STATE_LDS16, lds16)):
+            if fits:
+                sl, hl = run(ctx, db, spec, mode)
+                assert np.array_equal(hl.view(np.int32), hg.view(np.int32)), mode
+                assert np.array_equal(sl.view(np.uint8), sg.view(np.uint8)), mode
+                continue
             q = hbseed.Query(ctx, spec)
             try:
                 with pytest.raises(hbseed.HBError) as e:
-                    q.set_state_mode(hbseed.STATE_LDS)
+                    q.set_state_mode(mode)
                 assert e.value.code == -7
-                assert q.state_mode == hbseed.STATE_GLOBAL   # automatic placement
+                s = hbseed.Scan(ctx, db, q)   # automatic placement
+                assert s.state_mode != mode
+                s.close()
             finally:
                 q.close()
     finally:
@@ -85,9 +89,9 @@ def test_gapped_stage_on_a_long_query(ctx):
     db = hbseed.Database(ctx, res, offs)
     q = hbseed.Query(ctx, spec)
     try:
-        assert q.state_mode == hbseed.STATE_GLOBAL
         s = hbseed.Scan(ctx, db, q)
         try:
+            assert s.state_mode == hbseed.STATE_GLOBAL
             _, hsps = s.results()
             gst, g = s.gapped([60])
         finally:
@@ -137,5 +141,52 @@ def test_batched_and_served_long_queries(ctx):
             for t, p in zip(tickets, plain):
                 assert srv.wait(t, 120000)
                 assert np.array_equal(srv.results(t).view(np.int32), p.view(np.int32))
+    finally:
+        db.close()
+
+
+def test_16bit_state_at_its_subject_length_limit(ctx):
+    """Subjects up to 7,880 residues, many near it (with window 255 the
+    14-bit positions then reach 16,270 before a reset), window 40 and 255:
+    the 16-bit state gives the 32-bit state's and the oracle's results;
+    longer subjects refuse it."""
+    rng = np.random.default_rng(3500)
+    lens = np.concatenate([rng.integers(7800, 7881, 40), rng.integers(30, 600, 400)])
+    rng.shuffle(lens)
+    offs = np.zeros(len(lens) + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    q = hbseed.random_protein(3501, 3000)
+    res = np.concatenate([hbseed.random_protein(3502 + i, int(n)) for i, n in enumerate(lens)])
+    for i in range(0, len(lens), 7):   # planted copies of query segments
+        a, n = int(offs[i]), int(lens[i])
+        k = min(n, 200)
+        res[a + n - k:a + n] = q[100:100 + k]
+    spec = hbseed.blastp_query(q)
+    db = hbseed.Database(ctx, res, offs)
+    try:
+        for window in (40, 255):
+            s32, h32 = run(ctx, db, spec, hbseed.STATE_LDS, window)
+            s16, h16 = run(ctx, db, spec, hbseed.STATE_LDS16, window)
+            assert len(h32) > 0
+            assert np.array_equal(h16.view(np.int32), h32.view(np.int32)), window
+            assert np.array_equal(s16.view(np.uint8), s32.view(np.uint8)), window
+            _, ref = oracle_lib.OracleQuery.from_spec(spec, window=window).scan(res, offs,
+                                                                                threads=8)
+            assert np.array_equal(h16.view(np.int32), ref.view(np.int32)), window
+    finally:
+        db.close()
+    # a 9,000-residue subject: 16-bit positions no longer fit
+    offs2 = np.array([0, 9000, 9100], np.uint64)
+    res2 = hbseed.random_protein(3600, 9100)
+    db = hbseed.Database(ctx, res2, offs2)
+    try:
+        q = hbseed.Query(ctx, spec)
+        try:
+            q.set_state_mode(hbseed.STATE_LDS16)
+            with pytest.raises(hbseed.HBError) as e:
+                hbseed.Scan(ctx, db, q)
+            assert e.value.code == -7
+        finally:
+            q.close()
     finally:
         db.close()
