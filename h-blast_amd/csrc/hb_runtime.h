@@ -120,10 +120,10 @@ static inline bool hb_state16_ok(uint64_t max_len, int32_t window)
 // The reset threshold of a 16-bit state (hb_scan_args::state_limit).
 static inline int32_t hb_state16_limit(uint64_t max_len) { return 16384 - (int32_t)max_len; }
 // Automatic placement for diagonal arrays of D entries and `fixed` bytes of
-// per-workgroup LDS on a database whose longest subject is max_len: 32-bit
-// LDS state when it leaves 16 wavefronts per CU, else 16-bit LDS state (if
-// the subjects allow it) when that leaves at least 3, else 32-bit LDS state
-// when that leaves at least 6, else global memory.
+// per-workgroup LDS on a database whose longest subject is max_len: 16-bit
+// LDS state when the subjects allow it and it leaves at least 3 wavefronts
+// per CU, else 32-bit LDS state when that leaves at least 6, else global
+// memory.
 int hb_state_choose(const hb_ctx *c, int32_t D, size_t fixed, uint64_t max_len, int32_t window);
 // Global-memory diagonal state of one launch is capped at this size: a
 // launch for a very long query then runs fewer workgroups.

@@ -363,11 +363,12 @@ static int lds_waves(const hb_ctx *c, int32_t D, size_t fixed, int state_bytes)
 
 int hb_state_choose(const hb_ctx *c, int32_t D, size_t fixed, uint64_t max_len, int32_t window)
 {
-    const int w32 = lds_waves(c, D, fixed, 4);
-    if (w32 >= 16) return HB_PLACE_LDS32;
+    // 16-bit entries whenever the subjects allow them: never fewer wavefronts
+    // than 32-bit ones, and the C1 scan ran 0.7% faster with them even at
+    // the same 16 wavefronts per CU (15.17 vs 15.28 ms)
     if (hb_state16_ok(max_len, window) && lds_waves(c, D, fixed, 2) >= kLds16MinWaves)
         return HB_PLACE_LDS16;
-    return w32 >= kGstateMinWaves ? HB_PLACE_LDS32 : HB_PLACE_GLOBAL;
+    return lds_waves(c, D, fixed, 4) >= kGstateMinWaves ? HB_PLACE_LDS32 : HB_PLACE_GLOBAL;
 }
 
 size_t hb_gstate_bytes_max(const hb_ctx *c, int32_t D)

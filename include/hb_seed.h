@@ -159,10 +159,9 @@ int hb_query_free(hb_query *q);
 /* Where a plain scan of q keeps its per-diagonal two-hit state (the
  * reference keeps it in global memory, ewp->diag_table, one array per
  * thread: h_blast.cu:959-966):
- *   HB_STATE_AUTO   - LDS (32-bit entries, or 16-bit ones when those leave
- *                     the CU more wavefronts and the database's subjects
- *                     allow them) unless LDS would starve the CU of
- *                     wavefronts, else global memory (the default);
+ *   HB_STATE_AUTO   - LDS (16-bit entries when the database's subjects
+ *                     allow them, else 32-bit) unless LDS would starve the
+ *                     CU of wavefronts, else global memory (the default);
  *   HB_STATE_LDS    - LDS (HB_ERR_UNSUPPORTED if it does not fit);
  *   HB_STATE_GLOBAL - a global-memory array per wavefront.
  * Results are identical; only speed differs.  Takes effect for later
