@@ -257,7 +257,8 @@ def measure_batched(ctx, db, qs, steps, warmup, plain_sample, group_residues=0):
             "lookup_hits": info["lookup_hits"], "extensions": info["extensions"]}
 
 
-def measure_served(ctx, db, specs, arrivals, max_batch_residues, max_inflight, check=3):
+def measure_served(ctx, db, specs, arrivals, max_batch_residues, max_inflight, check=3,
+                   group_residues=None):
     """Submits specs[i] at time arrivals[i] (seconds) from a client thread to
     a server over the resident database, waits for every request, and
     returns throughput and latency (submit -> results ready, from the
@@ -266,7 +267,7 @@ def measure_served(ctx, db, specs, arrivals, max_batch_residues, max_inflight, c
     import threading
     n = len(specs)
     srv = hbseed.Server(ctx, db, max_batch_residues=max_batch_residues,
-                        max_inflight=max_inflight)
+                        max_inflight=max_inflight, group_residues=group_residues or None)
     # warm-up request (first launch, code-object load), not counted
     t = srv.submit(hbseed.blastp_query(hbseed.random_protein(1499, 300)))
     srv.wait(t)
@@ -366,7 +367,8 @@ def main():
                          "memory")
     ap.add_argument("--queries", type=int, default=2048, help="c3: number of queries")
     ap.add_argument("--group-residues", type=int, default=0,
-                    help="c3: query residues per group (0: autotuned)")
+                    help="c3: query residues per group (0: autotuned); c4: per served group "
+                         "(0: the server's default)")
     ap.add_argument("--plain-sample", type=int, default=48,
                     help="c3: queries timed one by one through the plain path")
     ap.add_argument("--qps", type=float, default=4.0, help="c4: Poisson arrival rate")
@@ -662,7 +664,8 @@ def bench_c4(args):
     del res
     log(f"{len(offs) - 1} subjects, {n_res} residues resident; {n_sub} arrivals over "
         f"{args.duration:.0f} s at {args.qps} q/s")
-    m = measure_served(ctx, db, specs, arrivals, args.max_batch_residues, args.max_inflight)
+    m = measure_served(ctx, db, specs, arrivals, args.max_batch_residues, args.max_inflight,
+                       group_residues=args.group_residues)
     abytes = m["batches"] * (n_res + 24 * (len(offs) - 1)) + 32 * m["hsps"]
     out = {
         "metric": "queries/s served (p50/p99 latency)",
