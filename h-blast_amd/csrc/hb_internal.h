@@ -13,7 +13,7 @@
 #define HB_JOBCAP 128       // queued two-hit extensions per wave
 #define HB_BLKCAP 128       // hits waiting on a pending diagonal per wave
 #ifndef HB_SCAP_MAX
-#define HB_SCAP_MAX 1024    // subjects up to the launch's staging size (<= this) are staged in LDS
+#define HB_SCAP_MAX 2048    // subjects up to the launch's staging size (<= this) are staged in LDS
 #endif
 #ifndef HB_MARK
 #define HB_MARK 256         // same-round ordering markers per wave (power of two)

@@ -307,8 +307,10 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
 // per CU gave 27.7 -> 20.6 -> 16.8 ms; beyond 16 the kernel is bound by
 // VALU issue and more wavefronts gained nothing), then the largest staging
 // buffer (subjects above it read their residues from global memory; 18
-// wavefronts with 320-byte staging ran 8% slower than 16 with 1024), then
-// more wavefronts per workgroup (they share the query tables).
+// wavefronts with 320-byte staging ran 8% slower than 16 with 1024, and
+// with the 16-bit diagonal state 2048 instead of 1024 took C1 from 15.17 to
+// 15.02 ms), then more wavefronts per workgroup (they share the query
+// tables).
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg, int place)
 {
