@@ -310,7 +310,8 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
 // wavefronts with 320-byte staging ran 8% slower than 16 with 1024, and
 // with the 16-bit diagonal state 2048 instead of 1024 took C1 from 15.17 to
 // 15.02 ms), then more wavefronts per workgroup (they share the query
-// tables).
+// tables).  The batched form stages at most 1024 residues: 2048 cost its
+// largest class of 256 C3 queries a fifth workgroup per CU (1214 -> 1373 ms).
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg, int place)
 {
@@ -319,7 +320,8 @@ int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_
     const int sb = hb_state_bytes(place);
     int best_wpc = 0;
     for (int waves : {4, 2, 1}) {
-        for (int32_t scap = HB_SCAP_MAX; scap >= 256; scap -= 64) {
+        for (int32_t scap = multi ? std::min(HB_SCAP_MAX, 1024) : HB_SCAP_MAX; scap >= 256;
+             scap -= 64) {
             const size_t lds = fixed + waves * hb_scan_wave_lds(diag_len_max, scap, sb);
             if (lds > c->max_lds) continue;
             int per_cu = 0;
