@@ -69,3 +69,44 @@ def test_batch_autotune(setup):
     for qid, spec in zip(ids, specs[:10]):
         assert np.array_equal(b.results(qid).view(np.int32).reshape(-1, 8), oracle(spec, res, offs))
     b.close()
+
+
+def test_batch_and_serve_with_long_subjects():
+    """A database with subjects up to 20,000 residues: positions no longer fit
+    the 16-bit diagonal state, so the batched and served launches keep the
+    32-bit state; every query still equals the oracle alone."""
+    pq = hbseed.random_protein(3701, 600)
+    res, offs = hbseed.make_db(3700, 3000, plant_query=pq, plant_frac=0.1)
+    extra = [hbseed.random_protein(3790 + i, n) for i, n in enumerate((9000, 15000, 20000))]
+    extra[1][7000:7600] = pq                          # a homolog inside a long subject
+    res = np.concatenate([res] + extra)
+    offs = np.concatenate([offs, offs[-1] + np.cumsum([len(e) for e in extra]).astype(np.uint64)])
+    assert int((offs[1:] - offs[:-1]).max()) == 20000
+    ctx = hbseed.Context(0)
+    db = hbseed.Database(ctx, res, offs)
+    try:
+        lens = hbseed.lognormal_lengths(3702, 12, 350.0, 0.7, 50, 3000)
+        specs = [hbseed.blastp_query(hbseed.random_protein(3710 + i, int(L)))
+                 for i, L in enumerate(lens)]
+        specs[0] = hbseed.blastp_query(pq)
+        b = hbseed.Batch(ctx, db, group_residues=2048)
+        ids = [b.add(s) for s in specs]
+        b.run()
+        for qid, spec in zip(ids, specs):
+            assert np.array_equal(b.results(qid).view(np.int32).reshape(-1, 8),
+                                  oracle(spec, res, offs)), qid
+        b.close()
+        with hbseed.Server(ctx, db) as srv:
+            tickets = [srv.submit(s) for s in specs[:4]]
+            for t, spec in zip(tickets, specs[:4]):
+                assert srv.wait(t, 120000)
+                assert np.array_equal(srv.results(t).view(np.int32).reshape(-1, 8),
+                                      oracle(spec, res, offs))
+        q = hbseed.Query(ctx, specs[0])
+        s = hbseed.Scan(ctx, db, q)
+        assert s.state_mode == hbseed.STATE_LDS   # 32-bit entries for these subjects
+        s.close()
+        q.close()
+    finally:
+        db.close()
+        ctx.close()
