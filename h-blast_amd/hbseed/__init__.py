@@ -58,6 +58,8 @@ EXPORTS = [
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
     "hb_fasta_parse", "hb_hdb_gpu_seqs", "hb_hdb_layout", "hb_hdb_pack", "hb_hdbi_format",
     "hb_hdbi_parse", "hb_db_upload_hdb", "hb_db_hdb_info", "hb_db_fetch",
+    "hb_runtime_options_default", "hb_runtime_options_parse", "hb_hdb_split",
+    "hb_balance_next_ratio",
     "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
     "hb_batch_results", "hb_batch_info_get", "hb_batch_free",
     "hb_serve_opts_default", "hb_serve_start", "hb_serve_submit", "hb_serve_wait",
@@ -110,6 +112,27 @@ class _HdbVolume(C.Structure):
 class _HdbIndex(C.Structure):
     _fields_ = [("blocksx", C.c_int32), ("threadsx", C.c_int32), ("percentage", C.c_int32),
                 ("n_volumes", C.c_int32)]
+
+
+class _RuntimeOptions(C.Structure):
+    _fields_ = [("gpu_ratio", C.c_float), ("fixed_ratio", C.c_int32),
+                ("capability_rate", C.c_double), ("max_hits_per_subject", C.c_int32),
+                ("mem_ecc", C.c_int32), ("load_balancer", C.c_int32),
+                ("load_factor", C.c_int32), ("subject_chunk", C.c_int32),
+                ("base_value", C.c_int32)]
+
+
+class _HdbShare(C.Structure):
+    _fields_ = [("gpu_seqs", C.c_int32), ("n_groups", C.c_int32), ("bytes", C.c_uint64)]
+
+
+class _BalanceIn(C.Structure):
+    _fields_ = [("volume", C.c_uint32), ("ratio", C.c_float), ("cpu_threads", C.c_uint32),
+                ("gpu_cards", C.c_uint32), ("volume_bytes", C.c_uint64 * 3),
+                ("ratio_ring", C.c_float * 2), ("hsp_ring", C.c_int64 * 2),
+                ("gapped", C.c_int32), ("query_length", C.c_int32),
+                ("avg_query_length", C.c_uint32), ("gpu_seconds", C.c_double),
+                ("cpu_seconds", C.c_double)]
 
 
 class _Params(C.Structure):
@@ -185,6 +208,12 @@ def _lib():
         L.hb_db_upload_hdb.argtypes = [vp, vp, u64, i32, i32, i32, vp, i32, pp]
         L.hb_db_hdb_info.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.hb_db_fetch.argtypes = [vp, vp, u64, vp]
+        L.hb_runtime_options_default.argtypes = [vp]
+        L.hb_runtime_options_default.restype = None
+        L.hb_runtime_options_parse.argtypes = [C.c_char_p, u64, vp]
+        L.hb_hdb_split.argtypes = [vp, i32, i32, u64, i32, i32, i32, C.POINTER(C.c_float),
+                                   C.POINTER(i32), vp]
+        L.hb_balance_next_ratio.argtypes = [vp, vp, C.POINTER(C.c_float)]
         L.hb_batch_opts_default.argtypes = [vp]
         L.hb_batch_create.argtypes = [vp, vp, vp, vp, pp]
         L.hb_batch_add.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(i32)]
@@ -550,6 +579,126 @@ def open_hdb(ctx: Context, index_path):
                                v["group_len"])
         db.oid_base = v["break_limit"] - v["volume_seqs"]
         out.append(db)
+    return out
+
+
+_OPTION_KEYS = [f[0] for f in _RuntimeOptions._fields_]
+
+
+def runtime_options(text=None):
+    """The H-BLAST_runtime_options settings as a dict: the values in force
+    without a file (text None), or parsed from the file's text
+    (Read_H_BLAST_runtime_options, ncbi_blast_files/functions.cpp:142-203)."""
+    o = _RuntimeOptions()
+    if text is None:
+        _lib().hb_runtime_options_default(C.byref(o))
+    else:
+        if isinstance(text, str):
+            text = text.encode()
+        _check(_lib().hb_runtime_options_parse(text, len(text), C.byref(o)),
+               "hb_runtime_options_parse")
+    return {k: getattr(o, k) for k in _OPTION_KEYS}
+
+
+def hdb_split(group_len, gpu_seqs, nbytes, blocksx, threadsx, cpu_threads, ratio=-1.0, fixed=1):
+    """The GPU's share of a volume at a workload ratio (hb_hdb_split,
+    get_GPU_DB_information) -> (dict(gpu_seqs, n_groups, bytes), ratio,
+    fixed) with the ratio and flag as the split leaves them."""
+    gl = np.ascontiguousarray(group_len, np.int32)
+    r, f, out = C.c_float(ratio), C.c_int32(fixed), _HdbShare()
+    _check(_lib().hb_hdb_split(_ptr(gl), gl.size, int(gpu_seqs), int(nbytes), blocksx, threadsx,
+                               cpu_threads, C.byref(r), C.byref(f), C.byref(out)),
+           "hb_hdb_split")
+    return dict(gpu_seqs=out.gpu_seqs, n_groups=out.n_groups, bytes=out.bytes), r.value, f.value
+
+
+def balance_next_ratio(options, volume, ratio, cpu_threads, volume_bytes, ratio_ring, hsp_ring,
+                       gpu_seconds, cpu_seconds, gapped=False, query_length=0,
+                       avg_query_length=0, gpu_cards=1):
+    """The ratio for volume + 1 (hb_balance_next_ratio, get_new_gpudb_ratio)."""
+    o = _RuntimeOptions(**options)
+    b = _BalanceIn(volume, ratio, cpu_threads, gpu_cards, (C.c_uint64 * 3)(*volume_bytes),
+                   (C.c_float * 2)(*ratio_ring), (C.c_int64 * 2)(*hsp_ring), int(bool(gapped)),
+                   query_length, avg_query_length, gpu_seconds, cpu_seconds)
+    out = C.c_float()
+    _check(_lib().hb_balance_next_ratio(C.byref(o), C.byref(b), C.byref(out)),
+           "hb_balance_next_ratio")
+    return out.value
+
+
+def cosearch(ctx: Context, index_path, spec: QuerySpec, cpu_search, cpu_threads, options=None,
+             gapped=False, **query_kw):
+    """Searches every volume of an .hDbI on the GPU and with the caller's CPU
+    work side by side, in the order GPU_BLAST_PreliminarySearchEngine
+    (ncbi_blast_files/functions.cpp:1409-1520) pipelines them: the device
+    runs volume v + 1's share while cpu_search(v, lo, hi) searches database
+    oids [lo, hi), the rest of volume v.  Each share is hdb_split's prefix of
+    the volume's .hDb; from the second volume on, an adaptive ratio follows
+    balance_next_ratio from the device time of the last share and the wall
+    time of the last cpu_search.  Returns one dict per volume: oid_base,
+    gpu_seqs, ratio, gpu_seconds, cpu_seconds, stats and hsps (database
+    oids) of the GPU share."""
+    import time
+    with open(index_path, "rb") as f:
+        idx = hdbi_parse(f.read())
+    d = os.path.dirname(index_path)
+    vols, bx, tx = idx["volumes"], idx["blocksx"], idx["threadsx"]
+    opts = runtime_options() if options is None else dict(options)
+    ratio, fixed = opts["gpu_ratio"], opts["fixed_ratio"]
+    ring_bytes, ring_ratio, ring_hsp = [0, 0, 0], [opts["gpu_ratio"], 0.0], [0, 0]
+    seq = np.asarray(spec.seq)
+    qlen, avg_q = int(seq.size), int(seq.size) // spec.n_ctx
+    out = [None] * len(vols)
+    q = Query(ctx, spec, **query_kw)
+    running = None   # (volume, Database, Scan) on the device
+    try:
+        for v in range(-1, len(vols)):
+            if v >= 0:
+                rec = out[v]
+                if running is not None:
+                    _, db, scan = running
+                    stats, hsps = scan.results()
+                    rec["gpu_seconds"] = scan.elapsed_ms() / 1000.0
+                    hsps["oid"] += np.uint32(rec["oid_base"])
+                    rec["stats"], rec["hsps"] = stats, hsps
+                    scan.close()
+                    db.close()
+                    running = None
+                else:
+                    rec["gpu_seconds"] = 0.0
+                    rec["stats"] = np.zeros(0, STATS_DTYPE)
+                    rec["hsps"] = np.zeros(0, HSP_DTYPE)
+                ring_hsp[v & 1] = int(rec["hsps"].size)
+            if v + 1 < len(vols):
+                nv, vol = v + 1, vols[v + 1]
+                ring_bytes[nv % 3] = int(vol["bytes"]) + int(vol["cpu_rest_bytes"])
+                if v > 0 and fixed != 1:
+                    ratio = balance_next_ratio(opts, v, ratio, cpu_threads, ring_bytes, ring_ratio,
+                                               ring_hsp, out[v]["gpu_seconds"],
+                                               out[v - 1]["cpu_seconds"], gapped, qlen, avg_q)
+                share, ratio, fixed = hdb_split(vol["group_len"], vol["gpu_seqs"], vol["bytes"],
+                                                bx, tx, cpu_threads, ratio, fixed)
+                ring_ratio[nv & 1] = ratio
+                out[nv] = dict(oid_base=vol["break_limit"] - vol["volume_seqs"],
+                               volume_seqs=vol["volume_seqs"], gpu_seqs=share["gpu_seqs"],
+                               ratio=ratio)
+                if share["gpu_seqs"] > 0:
+                    with open(os.path.join(d, vol["name"]), "rb") as f:
+                        data = f.read(share["bytes"])
+                    db = Database.from_hdb(ctx, data, bx, tx, share["gpu_seqs"],
+                                           vol["group_len"][:share["n_groups"]])
+                    running = (nv, db, Scan(ctx, db, q))
+            if v >= 0:
+                rec = out[v]
+                t0 = time.perf_counter()
+                cpu_search(v, rec["oid_base"] + rec["gpu_seqs"],
+                           rec["oid_base"] + rec["volume_seqs"])
+                rec["cpu_seconds"] = time.perf_counter() - t0
+    finally:
+        if running is not None:
+            running[2].close()
+            running[1].close()
+        q.close()
     return out
 
 

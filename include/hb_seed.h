@@ -280,6 +280,84 @@ int hb_db_upload_hdb(hb_ctx *ctx, const uint8_t *hdb, uint64_t bytes, int32_t bl
  * residues), in ms. */
 int hb_db_hdb_info(const hb_db *db, float *lengths_ms, float *copy_ms);
 
+/* --- CPU/GPU split of a volume (load balancing) --------------------------
+ * H-BLAST searches a volume on both sides at once: the device scans a prefix
+ * of the volume's .hDb while NCBI's CPU threads search the volume's other
+ * sequences, and the prefix moves from volume to volume so that both sides
+ * finish together (GPU_BLAST_PreliminarySearchEngine,
+ * ncbi_blast_files/functions.cpp:1409-1520).  These three host routines make
+ * that decision with the reference's arithmetic; the caller keeps the loop
+ * (upload the prefix with hb_db_upload_hdb, hb_scan_start, its own CPU work,
+ * hb_scan_wait, hb_scan_elapsed_ms). */
+
+/* The H-BLAST_runtime_options file (Read_H_BLAST_runtime_options,
+ * functions.cpp:142-203): a comment line before each of 9 values. */
+typedef struct hb_runtime_options {
+    float gpu_ratio;            /* GPU share of the workload, 0..1; -1 (no file):
+                                   hb_hdb_split picks it from the CPU thread count */
+    int32_t fixed_ratio;        /* 1: the ratio stays; 0: it adapts per volume */
+    double capability_rate;     /* throughput of the GPU over one CPU thread (12) */
+    int32_t max_hits_per_subject; /* the reference's per-subject HSP buffer (19; the
+                                   device here keeps every HSP, so unused) */
+    int32_t mem_ecc;            /* read, unused (0) */
+    int32_t load_balancer;      /* 1: charge GPU HSPs to the CPU load estimate (1) */
+    int32_t load_factor;        /* charge per HSP, ungapped (4) */
+    int32_t subject_chunk;      /* CPU threads' chunk of subjects (200; the caller's) */
+    int32_t base_value;         /* charge per HSP, gapped, for short queries (100) */
+} hb_runtime_options;
+/* The values in force when there is no options file (functions.cpp:14-26). */
+void hb_runtime_options_default(hb_runtime_options *o);
+/* Parses the file's text; values out of range take the reference's
+ * fallbacks (ratio outside 0..1 -> 0, capability <= 0 -> 12, hits <= 0 ->
+ * 20, negative factor / chunk / base -> 4 / 200 / 100).  HB_ERR_FORMAT if
+ * the text has fewer than 18 lines (the reference then reads stale values). */
+int hb_runtime_options_parse(const char *text, uint64_t len, hb_runtime_options *o);
+
+/* The GPU's share of one volume at a workload ratio (get_GPU_DB_information,
+ * functions.cpp:449-571).  Workload = group length x sequences.  With
+ * ratio < 1 the GPU keeps the .hDb's first whole groups whose workload stays
+ * within ratio x the total, then trunc(rest / (length x threadsx) x threadsx)
+ * sequences of the next group: a prefix of the file that hb_db_upload_hdb
+ * loads with out->gpu_seqs, the first out->n_groups group lengths and
+ * out->bytes bytes.  *ratio == -1 picks the start ratio from cpu_threads
+ * (1: 1, 2: 0.9, 3: 0.9, 4: 0.8, 5: 0.75, 6: 0.65, 7: 0.55, 8: 0.45, more:
+ * 0.4; from 3 threads also *fixed = 0).  Unless *fixed == 1, *ratio
+ * receives the ratio the split realises.  bytes is the .hDb size (kept when
+ * nothing is cut).  Group byte totals use 64-bit arithmetic (the
+ * reference's int products overflow past 2^31). */
+typedef struct hb_hdb_share {
+    int32_t gpu_seqs;   /* sequences the GPU scans: the .hDb's first ones */
+    int32_t n_groups;   /* their groups (the last one may be partial) */
+    uint64_t bytes;     /* the .hDb prefix holding them */
+} hb_hdb_share;
+int hb_hdb_split(const int32_t *group_len, int32_t n_groups, int32_t gpu_seqs, uint64_t bytes,
+                 int32_t blocksx, int32_t threadsx, int32_t cpu_threads, float *ratio,
+                 int32_t *fixed, hb_hdb_share *out);
+
+/* The ratio for volume + 1 after volume's GPU results came back
+ * (get_new_gpudb_ratio, functions.cpp:51-140): the slowest CPU thread's
+ * time on the previous volume and the GPU's time on this one, each scaled
+ * to the work they will do next, move the ratio by a share of their
+ * relative difference.  Rings as the reference keeps them: volume_bytes[i
+ * % 3] = .hDb bytes + residues left to the CPU of volume i; ratio_ring[i &
+ * 1] = the ratio volume i ran with; hsp_ring[i & 1] = the GPU's HSPs on
+ * volume i (sum of hb_subject_stats.n_hsp).  HB_ERR_ARG where the reference
+ * exits (a zero volume size or ring ratio) and for volume 0. */
+typedef struct hb_balance_in {
+    uint32_t volume;            /* the volume whose GPU results just came back (>= 1) */
+    float ratio;                /* the current ratio */
+    uint32_t cpu_threads, gpu_cards;
+    uint64_t volume_bytes[3];
+    float ratio_ring[2];
+    int64_t hsp_ring[2];
+    int32_t gapped;             /* 1: a gapped search follows */
+    int32_t query_length;       /* concatenated query length */
+    uint32_t avg_query_length;  /* query length / contexts (functions.cpp:1280-1286) */
+    double gpu_seconds;         /* device time of volume's scan */
+    double cpu_seconds;         /* slowest CPU thread on volume - 1's CPU share */
+} hb_balance_in;
+int hb_balance_next_ratio(const hb_runtime_options *o, const hb_balance_in *in, float *ratio);
+
 /* --- plain per-query scan ----------------------------------------------
  * hb_scan_start launches the stage on the context's stream and returns
  * (asynchronous, like H_BLAST_pre, ncbi_blast_files/gpu_cpu_common.h:48-54).

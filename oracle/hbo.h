@@ -174,6 +174,33 @@ int hbo_hdb_unpack(const uint8_t *hdb, uint64_t bytes, int32_t n_seqs, int32_t b
 int64_t hbo_hdbi_format(int32_t blocksx, int32_t threadsx, int32_t percentage, int32_t n_volumes,
                         const hbo_hdb_volume *vols, char *text, uint64_t cap);
 
+/* --- CPU/GPU split of a volume (hbo_balance.c) --------------------------- */
+typedef struct hbo_options {  /* the H-BLAST_runtime_options file */
+    float gpu_ratio;
+    int32_t fixed_ratio;
+    double capability_rate;
+    int32_t max_hits;
+    int32_t mem_ecc;
+    int32_t load_balancer;
+    int32_t load_factor;
+    int32_t subject_chunk;
+    int32_t base_value;
+} hbo_options;
+
+void hbo_options_default(hbo_options *o);
+/* 0, or HBO_ERR_ARG when the text has fewer than 18 lines. */
+int hbo_options_parse(const char *text, uint64_t len, hbo_options *o);
+/* GPU share of a volume at *ratio_io (-1: by cpu_threads); group_size =
+ * blocksx * threadsx. */
+int hbo_split(const int32_t *glen, int32_t ngroups, int32_t nseq, uint64_t file_bytes,
+              int32_t threadsx, int32_t group_size, int32_t cpu_threads, float *ratio_io,
+              int32_t *fixed_io, int32_t *out_seqs, int32_t *out_groups, uint64_t *out_bytes);
+/* Ratio for volume v + 1 (rings as the product's hb_balance_in). */
+int hbo_next_ratio(const hbo_options *o, unsigned int v, float ratio, unsigned int cpu_threads,
+                   unsigned int gpu_cards, const uint64_t *vol_bytes, const float *ratio_ring,
+                   const int64_t *hsp_ring, int gapped, int query_length,
+                   unsigned int avg_query_length, double gpu_t, double cpu_t, float *out);
+
 #define HBO_ERR_ARG (-1)
 #define HBO_ERR_NOMEM (-2)
 #define HBO_ERR_RESIDUE (-3)

@@ -322,3 +322,78 @@ def hdbi_format(blocksx, threadsx, volumes, percentage=99):
     buf = C.create_string_buffer(n + 1)
     L.hbo_hdbi_format(blocksx, threadsx, percentage, len(volumes), C.addressof(arr), buf, n)
     return buf.raw[:n].decode()
+
+
+# --- CPU/GPU split of a volume (hbo_balance.c) ---------------------------
+
+class _Options(C.Structure):
+    _fields_ = [("gpu_ratio", C.c_float), ("fixed_ratio", C.c_int32),
+                ("capability_rate", C.c_double), ("max_hits", C.c_int32),
+                ("mem_ecc", C.c_int32), ("load_balancer", C.c_int32),
+                ("load_factor", C.c_int32), ("subject_chunk", C.c_int32),
+                ("base_value", C.c_int32)]
+
+
+_OPTION_KEYS = ["gpu_ratio", "fixed_ratio", "capability_rate", "max_hits_per_subject", "mem_ecc",
+                "load_balancer", "load_factor", "subject_chunk", "base_value"]
+
+
+def _balance_lib():
+    L = lib()
+    if not getattr(L, "_balance_ready", False):
+        i32, u32, vp, u64 = C.c_int32, C.c_uint32, C.c_void_p, C.c_uint64
+        L.hbo_options_default.argtypes = [vp]
+        L.hbo_options_default.restype = None
+        L.hbo_options_parse.argtypes = [C.c_char_p, u64, vp]
+        L.hbo_options_parse.restype = C.c_int
+        L.hbo_split.argtypes = [vp, i32, i32, u64, i32, i32, i32, C.POINTER(C.c_float),
+                                C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(u64)]
+        L.hbo_split.restype = C.c_int
+        L.hbo_next_ratio.argtypes = [vp, u32, C.c_float, u32, u32, vp, vp, vp, C.c_int, C.c_int,
+                                     u32, C.c_double, C.c_double, C.POINTER(C.c_float)]
+        L.hbo_next_ratio.restype = C.c_int
+        L._balance_ready = True
+    return L
+
+
+def runtime_options(text=None):
+    """Options dict (keys as the product's), defaults or parsed; None when
+    the oracle rejects the text."""
+    o = _Options()
+    L = _balance_lib()
+    if text is None:
+        L.hbo_options_default(C.byref(o))
+    else:
+        if isinstance(text, str):
+            text = text.encode()
+        if L.hbo_options_parse(text, len(text), C.byref(o)) != 0:
+            return None
+    return dict(zip(_OPTION_KEYS, (getattr(o, f[0]) for f in _Options._fields_)))
+
+
+def split(group_len, gpu_seqs, nbytes, blocksx, threadsx, cpu_threads, ratio=-1.0, fixed=1):
+    gl = np.ascontiguousarray(group_len, np.int32)
+    r, f = C.c_float(ratio), C.c_int32(fixed)
+    s, g, b = C.c_int32(), C.c_int32(), C.c_uint64()
+    rc = _balance_lib().hbo_split(gl.ctypes.data if gl.size else None, gl.size, int(gpu_seqs),
+                                  int(nbytes), threadsx, blocksx * threadsx, cpu_threads,
+                                  C.byref(r), C.byref(f), C.byref(s), C.byref(g), C.byref(b))
+    if rc != 0:
+        raise RuntimeError(f"hbo_split failed: {rc}")
+    return dict(gpu_seqs=s.value, n_groups=g.value, bytes=b.value), r.value, f.value
+
+
+def next_ratio(options, volume, ratio, cpu_threads, volume_bytes, ratio_ring, hsp_ring,
+               gpu_seconds, cpu_seconds, gapped=False, query_length=0, avg_query_length=0,
+               gpu_cards=1):
+    """-> new ratio, or None where the oracle reports an error."""
+    o = _Options(*(options[k] for k in _OPTION_KEYS))
+    vb = np.ascontiguousarray(volume_bytes, np.uint64)
+    rr = np.ascontiguousarray(ratio_ring, np.float32)
+    hr = np.ascontiguousarray(hsp_ring, np.int64)
+    out = C.c_float()
+    rc = _balance_lib().hbo_next_ratio(C.byref(o), volume, ratio, cpu_threads, gpu_cards,
+                                       vb.ctypes.data, rr.ctypes.data, hr.ctypes.data,
+                                       int(bool(gapped)), query_length, avg_query_length,
+                                       gpu_seconds, cpu_seconds, C.byref(out))
+    return None if rc != 0 else out.value
