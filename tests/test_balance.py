@@ -32,6 +32,21 @@ def test_options_parse_by_hand():
     assert O.runtime_options(OPTS_TEXT) == want
 
 
+def test_options_file_the_reference_ships():
+    # tests/golden/H-BLAST_runtime_options is the reference's own options file
+    # (runtime_setting_files/H-BLAST_runtime_options), kept as data: ratio 1,
+    # adaptive, capability 12, 20 HSPs per subject, no ECC ("0 " with a
+    # trailing blank), new balancer, factor 4, chunk 200, base 100
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "H-BLAST_runtime_options")
+    with open(path, "rb") as f:
+        text = f.read()
+    want = dict(gpu_ratio=1.0, fixed_ratio=0, capability_rate=12.0, max_hits_per_subject=20,
+                mem_ecc=0, load_balancer=1, load_factor=4, subject_chunk=200, base_value=100)
+    assert hb.runtime_options(text) == want
+    assert O.runtime_options(text) == want
+
+
 def test_options_fallbacks_by_hand():
     # out-of-range values take the reference's fallbacks; "2" is not "1" for
     # the fixed flag; atoi/atof read a number's leading part
