@@ -22,6 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBDIR = os.path.join(os.path.dirname(_HERE), "lib")
 
 HB_OK = 0
+HB_ERR_NOMEM = -2
 ERRORS = {
     -1: "HB_ERR_ARG", -2: "HB_ERR_NOMEM", -3: "HB_ERR_DEVICE", -4: "HB_ERR_CAPACITY",
     -5: "HB_ERR_RESIDUE", -6: "HB_ERR_STATE", -7: "HB_ERR_UNSUPPORTED", -8: "HB_ERR_BUSY",
@@ -627,7 +628,7 @@ def balance_next_ratio(options, volume, ratio, cpu_threads, volume_bytes, ratio_
 
 
 def cosearch(ctx: Context, index_path, spec: QuerySpec, cpu_search, cpu_threads, options=None,
-             gapped=False, **query_kw):
+             gapped=False, device_budget=None, **query_kw):
     """Searches every volume of an .hDbI on the GPU and with the caller's CPU
     work side by side, in the order GPU_BLAST_PreliminarySearchEngine
     (ncbi_blast_files/functions.cpp:1409-1520) pipelines them: the device
@@ -635,9 +636,14 @@ def cosearch(ctx: Context, index_path, spec: QuerySpec, cpu_search, cpu_threads,
     oids [lo, hi), the rest of volume v.  Each share is hdb_split's prefix of
     the volume's .hDb; from the second volume on, an adaptive ratio follows
     balance_next_ratio from the device time of the last share and the wall
-    time of the last cpu_search.  Returns one dict per volume: oid_base,
-    gpu_seqs, ratio, gpu_seconds, cpu_seconds, stats and hsps (database
-    oids) of the GPU share."""
+    time of the last cpu_search.  A share that needs more device memory
+    than device_budget bytes (default: the device's free memory), or whose
+    upload runs out of device memory, is not scanned: cpu_search gets the
+    whole volume, as the reference sends a volume to the CPU when
+    H_BLAST_check_memory fails (functions.cpp:1518-1530, 1584-1585).
+    Returns one dict per volume: oid_base, gpu_seqs, ratio, on_cpu,
+    gpu_seconds, cpu_seconds, stats and hsps (database oids) of the GPU
+    share."""
     import time
     with open(index_path, "rb") as f:
         idx = hdbi_parse(f.read())
@@ -679,15 +685,30 @@ def cosearch(ctx: Context, index_path, spec: QuerySpec, cpu_search, cpu_threads,
                 share, ratio, fixed = hdb_split(vol["group_len"], vol["gpu_seqs"], vol["bytes"],
                                                 bx, tx, cpu_threads, ratio, fixed)
                 ring_ratio[nv & 1] = ratio
-                out[nv] = dict(oid_base=vol["break_limit"] - vol["volume_seqs"],
-                               volume_seqs=vol["volume_seqs"], gpu_seqs=share["gpu_seqs"],
-                               ratio=ratio)
-                if share["gpu_seqs"] > 0:
+                rec = dict(oid_base=vol["break_limit"] - vol["volume_seqs"],
+                           volume_seqs=vol["volume_seqs"], gpu_seqs=share["gpu_seqs"],
+                           ratio=ratio, on_cpu=False)
+                out[nv] = rec
+                # the unpacked share, its file bytes in flight, per-subject
+                # metadata and counters, and room for HSPs
+                need = 2 * share["bytes"] + 64 * share["gpu_seqs"]
+                budget = ctx.mem_info()[0] if device_budget is None else device_budget
+                if share["gpu_seqs"] > 0 and need > budget:
+                    rec["gpu_seqs"], rec["on_cpu"] = 0, True
+                if rec["gpu_seqs"] > 0:
                     with open(os.path.join(d, vol["name"]), "rb") as f:
                         data = f.read(share["bytes"])
-                    db = Database.from_hdb(ctx, data, bx, tx, share["gpu_seqs"],
-                                           vol["group_len"][:share["n_groups"]])
-                    running = (nv, db, Scan(ctx, db, q))
+                    db = None
+                    try:
+                        db = Database.from_hdb(ctx, data, bx, tx, share["gpu_seqs"],
+                                               vol["group_len"][:share["n_groups"]])
+                        running = (nv, db, Scan(ctx, db, q))
+                    except HBError as e:
+                        if db is not None:
+                            db.close()
+                        if e.code != HB_ERR_NOMEM:
+                            raise
+                        rec["gpu_seqs"], rec["on_cpu"] = 0, True
             if v >= 0:
                 rec = out[v]
                 t0 = time.perf_counter()

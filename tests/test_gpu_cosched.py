@@ -46,7 +46,7 @@ def setup(tmp_path_factory):
     ctx.close()
 
 
-def run(s, cpu_threads, options=None):
+def run(s, cpu_threads, options=None, device_budget=None):
     res, offs, oq = s["res"], s["offs"], s["oq"]
     cpu_parts = {}
 
@@ -59,7 +59,7 @@ def run(s, cpu_threads, options=None):
     print(f"cosearch {len(SIZES)} volumes, {cpu_threads} CPU threads, options {options}",
           flush=True)
     out = hbseed.cosearch(s["ctx"], s["index"], s["spec"], cpu_search, cpu_threads,
-                          options=options)
+                          options=options, device_budget=device_budget)
     return out, cpu_parts
 
 
@@ -141,3 +141,24 @@ def test_all_on_the_gpu_and_all_on_the_cpu(setup):
         check_union(s, out, cpu_parts)
         for rec, vol in zip(out, s["idx"]["volumes"]):
             assert rec["gpu_seqs"] == (vol["gpu_seqs"] if ratio else 0)
+
+
+def test_volume_over_the_device_budget_goes_to_the_cpu(setup):
+    # the reference searches a whole volume on the CPU when its GPU share
+    # does not fit the device (functions.cpp:1518-1530, 1584-1585); forced
+    # here from the host with a budget below the largest share's need
+    s = setup
+    vols = s["idx"]["volumes"]
+    opts = dict(hbseed.runtime_options(), gpu_ratio=1.0, fixed_ratio=1)
+    need = [2 * v["bytes"] + 64 * v["gpu_seqs"] for v in vols]
+    big = int(np.argmax(need))
+    assert sorted(need)[-2] < need[big]
+    out, cpu_parts = run(s, 16, options=opts, device_budget=need[big] - 1)
+    check_union(s, out, cpu_parts)
+    for k, (rec, vol) in enumerate(zip(out, vols)):
+        assert rec["on_cpu"] == (k == big)
+        assert rec["gpu_seqs"] == (0 if k == big else vol["gpu_seqs"])
+    # and with no room at all, every volume is searched on the CPU
+    out, cpu_parts = run(s, 16, options=opts, device_budget=0)
+    check_union(s, out, cpu_parts)
+    assert all(r["on_cpu"] and r["gpu_seqs"] == 0 for r in out)
