@@ -60,7 +60,7 @@ EXPORTS = [
     "hb_fasta_parse", "hb_hdb_gpu_seqs", "hb_hdb_layout", "hb_hdb_pack", "hb_hdbi_format",
     "hb_hdbi_parse", "hb_db_upload_hdb", "hb_db_hdb_info", "hb_db_fetch",
     "hb_runtime_options_default", "hb_runtime_options_parse", "hb_hdb_split",
-    "hb_balance_next_ratio",
+    "hb_balance_next_ratio", "hb_seg_params_default", "hb_seg_mask",
     "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
     "hb_batch_results", "hb_batch_info_get", "hb_batch_free",
     "hb_serve_opts_default", "hb_serve_start", "hb_serve_submit", "hb_serve_wait",
@@ -134,6 +134,11 @@ class _BalanceIn(C.Structure):
                 ("gapped", C.c_int32), ("query_length", C.c_int32),
                 ("avg_query_length", C.c_uint32), ("gpu_seconds", C.c_double),
                 ("cpu_seconds", C.c_double)]
+
+
+class _SegParams(C.Structure):
+    _fields_ = [("window", C.c_int32), ("locut", C.c_double), ("hicut", C.c_double),
+                ("maxbogus", C.c_int32), ("maxtrim", C.c_int32)]
 
 
 class _Params(C.Structure):
@@ -215,6 +220,9 @@ def _lib():
         L.hb_hdb_split.argtypes = [vp, i32, i32, u64, i32, i32, i32, C.POINTER(C.c_float),
                                    C.POINTER(i32), vp]
         L.hb_balance_next_ratio.argtypes = [vp, vp, C.POINTER(C.c_float)]
+        L.hb_seg_params_default.argtypes = [vp]
+        L.hb_seg_params_default.restype = None
+        L.hb_seg_mask.argtypes = [vp, i32, i32, vp, vp, vp, vp]
         L.hb_batch_opts_default.argtypes = [vp]
         L.hb_batch_create.argtypes = [vp, vp, vp, vp, pp]
         L.hb_batch_add.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(i32)]
@@ -342,6 +350,31 @@ def blastx_query(nt: str, xdrop=DEFAULT_XDROP, cutoff=DEFAULT_CUTOFF) -> QuerySp
     _check(L.hb_blastx_translate(nt.encode(), len(nt), _ptr(seq), _ptr(off), _ptr(ln)),
            "blastx_translate")
     return QuerySpec(seq, off, ln, [xdrop] * 6, [cutoff] * 6)
+
+
+SEG_MASK_LETTER = 21   # X
+
+
+def seg_mask(spec: QuerySpec, window=12, locut=2.2, hicut=2.5, maxbogus=2, maxtrim=100):
+    """SEG low-complexity mask of a query's contexts (hb_seg_mask): uint8
+    array, 1 inside a segment.  The defaults are NCBI's -seg 12 2.2 2.5."""
+    p = _SegParams(window, locut, hicut, maxbogus, maxtrim)
+    out = np.zeros(spec.seq.size, np.uint8)
+    _check(_lib().hb_seg_mask(_ptr(spec.seq), spec.seq.size, spec.n_ctx, _ptr(spec.ctx_off),
+                              _ptr(spec.ctx_len), C.byref(p), _ptr(out)), "hb_seg_mask")
+    return out
+
+
+def seg_query(spec: QuerySpec, hard=True, **seg_kw) -> QuerySpec:
+    """The query with SEG applied as NCBI applies a filter: no lookup-table
+    word inside a segment, and with hard masking (BLAST+'s default) the
+    segment's residues read as X in extensions too."""
+    m = seg_mask(spec, **seg_kw)
+    seq = spec.seq.copy()
+    if hard:
+        seq[m != 0] = SEG_MASK_LETTER
+    lut = m if spec.lut_mask is None else (m | (spec.lut_mask != 0)).astype(np.uint8)
+    return QuerySpec(seq, spec.ctx_off, spec.ctx_len, spec.xdrop, spec.cutoff, lut_mask=lut)
 
 
 # ------------------------------------------------------------------ runtime

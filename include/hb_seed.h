@@ -358,6 +358,28 @@ typedef struct hb_balance_in {
 } hb_balance_in;
 int hb_balance_next_ratio(const hb_runtime_options *o, const hb_balance_in *in, float *ratio);
 
+/* --- SEG low-complexity masking of a query ------------------------------
+ * NCBI BLAST+ masks BLASTX queries (and BLASTP queries with -seg) with SEG
+ * before the lookup table is built (-seg 12 2.2 2.5,
+ * ncbi_blast_files/blast_args.cpp:309-316); NCBI's blast_seg.c is not part
+ * of the reference, so hb_seg_mask restates the published algorithm
+ * (Wootton and Federhen 1993, 1996) with NCBI's protein parameters.  Its
+ * mask feeds hb_query_prepare_masked / hb_batch_add_masked (no word seeds
+ * inside a segment); hard masking also replaces the residues by X (21). */
+typedef struct hb_seg_params {
+    int32_t window;     /* window length (12) */
+    double locut;       /* trigger: window entropy <= locut bits (2.2) */
+    double hicut;       /* extension: window entropy <= hicut bits (2.5) */
+    int32_t maxbogus;   /* windows with more non-standard letters get no entropy (2) */
+    int32_t maxtrim;    /* trimming tries at most this many shorter lengths (100) */
+} hb_seg_params;
+void hb_seg_params_default(hb_seg_params *p);
+/* mask[i] = 1 for residues inside a low-complexity segment of their
+ * context, else 0 (len bytes).  params NULL: the defaults.  HB_ERR_RESIDUE
+ * for codes >= 28. */
+int hb_seg_mask(const uint8_t *seq, int32_t len, int32_t n_ctx, const int32_t *ctx_off,
+                const int32_t *ctx_len, const hb_seg_params *params, uint8_t *mask);
+
 /* --- plain per-query scan ----------------------------------------------
  * hb_scan_start launches the stage on the context's stream and returns
  * (asynchronous, like H_BLAST_pre, ncbi_blast_files/gpu_cpu_common.h:48-54).

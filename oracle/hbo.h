@@ -201,6 +201,15 @@ int hbo_next_ratio(const hbo_options *o, unsigned int v, float ratio, unsigned i
                    const int64_t *hsp_ring, int gapped, int query_length,
                    unsigned int avg_query_length, double gpu_t, double cpu_t, float *out);
 
+/* --- SEG low-complexity masking (hbo_seg.c) ------------------------------- */
+/* Shannon entropy (bits) of counts[28] over the 20 standard letters (the
+ * other codes must be 0), and ln P of that composition over n letters. */
+double hbo_seg_entropy(const int32_t *counts);
+double hbo_seg_lnprob(const int32_t *counts, int32_t n);
+int hbo_seg_mask(const uint8_t *seq, int32_t len, int32_t n_ctx, const int32_t *ctx_off,
+                 const int32_t *ctx_len, int32_t window, double locut, double hicut,
+                 int32_t maxbogus, int32_t maxtrim, uint8_t *mask);
+
 #define HBO_ERR_ARG (-1)
 #define HBO_ERR_NOMEM (-2)
 #define HBO_ERR_RESIDUE (-3)

@@ -397,3 +397,43 @@ def next_ratio(options, volume, ratio, cpu_threads, volume_bytes, ratio_ring, hs
                                        int(bool(gapped)), query_length, avg_query_length,
                                        gpu_seconds, cpu_seconds, C.byref(out))
     return None if rc != 0 else out.value
+
+
+# --- SEG low-complexity masking (hbo_seg.c) --------------------------------
+
+def _seg_lib():
+    L = lib()
+    if not getattr(L, "_seg_ready", False):
+        i32, vp = C.c_int32, C.c_void_p
+        L.hbo_seg_entropy.argtypes = [vp]
+        L.hbo_seg_entropy.restype = C.c_double
+        L.hbo_seg_lnprob.argtypes = [vp, i32]
+        L.hbo_seg_lnprob.restype = C.c_double
+        L.hbo_seg_mask.argtypes = [vp, i32, i32, vp, vp, i32, C.c_double, C.c_double, i32, i32,
+                                   vp]
+        L.hbo_seg_mask.restype = C.c_int
+        L._seg_ready = True
+    return L
+
+
+def seg_entropy(counts28):
+    c = np.ascontiguousarray(counts28, np.int32)
+    return _seg_lib().hbo_seg_entropy(c.ctypes.data)
+
+
+def seg_lnprob(counts28, n):
+    c = np.ascontiguousarray(counts28, np.int32)
+    return _seg_lib().hbo_seg_lnprob(c.ctypes.data, int(n))
+
+
+def seg_mask(seq, ctx_off, ctx_len, window=12, locut=2.2, hicut=2.5, maxbogus=2, maxtrim=100):
+    seq = np.ascontiguousarray(seq, np.uint8)
+    off = np.ascontiguousarray(ctx_off, np.int32)
+    ln = np.ascontiguousarray(ctx_len, np.int32)
+    out = np.zeros(seq.size, np.uint8)
+    rc = _seg_lib().hbo_seg_mask(seq.ctypes.data, seq.size, off.size, off.ctypes.data,
+                                 ln.ctypes.data, window, locut, hicut, maxbogus, maxtrim,
+                                 out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"hbo_seg_mask failed: {rc}")
+    return out
