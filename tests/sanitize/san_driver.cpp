@@ -1,6 +1,7 @@
 // Sanitizer driver (test infrastructure): random and malformed inputs through
-// the host-only C-ABI helpers (h-blast_amd/csrc/hb_seq.cpp, hb_hdb.cpp) and
-// the CPU oracle (oracle/hbo.c, oracle/hbo_gap.c, oracle/hbo_hdb.c).  tests/test_sanitizers.py builds it
+// the host-only C-ABI helpers (h-blast_amd/csrc/hb_seq.cpp, hb_hdb.cpp,
+// hb_balance.cpp, hb_seg.cpp) and the CPU oracle (oracle/hbo.c, hbo_gap.c,
+// hbo_hdb.c, hbo_balance.c, hbo_seg.c).  tests/test_sanitizers.py builds it
 // with -fsanitize=address,undefined (and the threaded oracle parts with
 // -fsanitize=thread) and fails on any report.  Exit status 0 = clean.
 #include <algorithm>
@@ -226,6 +227,95 @@ static void hdb(std::mt19937_64 &rng, int rounds)
     }
 }
 
+// Options texts with random lines and truncations, splits of random volumes
+// and random balancer inputs, product against the oracle.
+static void balance(std::mt19937_64 &rng, int rounds)
+{
+    static const char *tok[] = {"0", "1", "-1", "0.5", "12", "x", "", "1e9", "99999999999", "nan"};
+    for (int r = 0; r < rounds; ++r) {
+        std::string t;
+        for (int i = 0; i < 9; ++i) {
+            t += "#c\n";
+            t += tok[rng() % 10];
+            t += "\n";
+        }
+        if (rng() % 4 == 0) t.resize(rng() % (t.size() + 1));
+        hb_runtime_options a;
+        hbo_options b;
+        const int ra = hb_runtime_options_parse(t.data(), t.size(), &a);
+        const int rb = hbo_options_parse(t.data(), t.size(), &b);
+        CHECK((ra == HB_OK) == (rb == 0));
+        // a volume's groups and a ratio
+        const int32_t bx = 1 + (int32_t)(rng() % 5), tx = 1 + (int32_t)(rng() % 70);
+        const int32_t G = bx * tx, ng = 1 + (int32_t)(rng() % 10);
+        const int32_t n = (ng - 1) * G + 1 + (int32_t)(rng() % G);
+        std::vector<int32_t> gl(ng);
+        for (auto &x : gl) x = 16 * (int32_t)(rng() % 40);
+        std::sort(gl.begin(), gl.end());
+        float r1 = (rng() % 5 == 0) ? -1.0f : (float)(rng() % 1200) / 1000.0f, r2 = r1;
+        int32_t f1 = (int32_t)(rng() % 2), f2 = f1;
+        hb_hdb_share sh;
+        int32_t s2, g2;
+        uint64_t b2;
+        CHECK(hb_hdb_split(gl.data(), ng, n, 12345, bx, tx, 4, &r1, &f1, &sh) == HB_OK);
+        CHECK(hbo_split(gl.data(), ng, n, 12345, tx, G, 4, &r2, &f2, &s2, &g2, &b2) == 0);
+        CHECK(sh.gpu_seqs == s2 && sh.n_groups == g2 && sh.bytes == b2 && f1 == f2);
+        CHECK(std::memcmp(&r1, &r2, sizeof r1) == 0);
+        hb_runtime_options o;
+        hb_runtime_options_default(&o);
+        hbo_options oo;
+        hbo_options_default(&oo);
+        hb_balance_in in = {};
+        in.volume = 1 + (uint32_t)(rng() % 9);
+        in.ratio = (float)(rng() % 1000) / 1000.0f;
+        in.cpu_threads = 1 + (uint32_t)(rng() % 64);
+        in.gpu_cards = 1;
+        for (auto &x : in.volume_bytes) x = 1 + rng() % 1000000;
+        for (auto &x : in.ratio_ring) x = (float)(1 + rng() % 1000) / 1000.0f;
+        for (auto &x : in.hsp_ring) x = (int64_t)(rng() % 100000);
+        in.gapped = (int32_t)(rng() % 2);
+        in.query_length = (int32_t)(rng() % 10000);
+        in.avg_query_length = (uint32_t)(rng() % 3000);
+        in.gpu_seconds = (double)(rng() % 1000) / 100.0;
+        in.cpu_seconds = (double)(rng() % 1000) / 100.0;
+        float x1 = 0, x2 = 0;
+        CHECK(hb_balance_next_ratio(&o, &in, &x1) == HB_OK);
+        CHECK(hbo_next_ratio(&oo, in.volume, in.ratio, in.cpu_threads, in.gpu_cards,
+                             in.volume_bytes, in.ratio_ring, in.hsp_ring, in.gapped,
+                             in.query_length, in.avg_query_length, in.gpu_seconds,
+                             in.cpu_seconds, &x2) == 0);
+        CHECK(x1 == x2);
+    }
+}
+
+// SEG masks of random queries with low-complexity runs and other letters.
+static void seg(std::mt19937_64 &rng, int rounds)
+{
+    for (int r = 0; r < rounds; ++r) {
+        std::vector<uint8_t> q;
+        const int parts = 1 + (int)(rng() % 5);
+        for (int k = 0; k < parts; ++k) {
+            const int n = (int)(rng() % 120);
+            for (int i = 0; i < n; ++i) q.push_back((uint8_t)(1 + rng() % 27));
+            const int m = (int)(rng() % 50);
+            const uint8_t a = (uint8_t)(1 + rng() % 20), b = (uint8_t)(1 + rng() % 20);
+            for (int i = 0; i < m; ++i) q.push_back(i % 2 ? a : b);
+        }
+        const int32_t len = (int32_t)q.size();
+        const int32_t half = len / 2;
+        const int32_t off[2] = {0, half}, cl[2] = {half, len - half};
+        hb_seg_params p;
+        hb_seg_params_default(&p);
+        p.window = 1 + (int32_t)(rng() % 20);
+        p.maxtrim = (int32_t)(rng() % 120);
+        std::vector<uint8_t> m1(len + 1), m2(len + 1);
+        CHECK(hb_seg_mask(q.data(), len, 2, off, cl, &p, m1.data()) == HB_OK);
+        CHECK(hbo_seg_mask(q.data(), len, 2, off, cl, p.window, p.locut, p.hicut, p.maxbogus,
+                           p.maxtrim, m2.data()) == 0);
+        CHECK(std::memcmp(m1.data(), m2.data(), (size_t)len) == 0);
+    }
+}
+
 int main(int argc, char **argv)
 {
     const bool threads_only = argc > 1 && std::strcmp(argv[1], "threads") == 0;
@@ -237,6 +327,8 @@ int main(int argc, char **argv)
         translate(rng, 300);
         oracle(rng, 60, 1);
         hdb(rng, 300);
+        balance(rng, 2000);
+        seg(rng, 300);
     }
     std::fprintf(stderr, "%s: %d failed checks\n", threads_only ? "threads" : "all", fails);
     return fails ? 1 : 0;

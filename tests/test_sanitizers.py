@@ -1,7 +1,8 @@
 """Host code under sanitizers (CPU only, no GPU).
 
 Builds tests/sanitize/san_driver.cpp with the host-only C-ABI helpers
-(h-blast_amd/csrc/hb_seq.cpp, hb_hdb.cpp) and the oracle sources twice -- with
+(h-blast_amd/csrc/hb_seq.cpp, hb_hdb.cpp, hb_balance.cpp, hb_seg.cpp) and the
+oracle sources twice -- with
 -fsanitize=address,undefined and with -fsanitize=thread -- in a temporary
 directory, runs them on random and malformed inputs, and fails on any report
 or failed check.  Device code and the HIP runtime are not covered (GPU
@@ -17,9 +18,12 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = [os.path.join(ROOT, "tests", "sanitize", "san_driver.cpp"),
        os.path.join(ROOT, "h-blast_amd", "csrc", "hb_seq.cpp"),
-       os.path.join(ROOT, "h-blast_amd", "csrc", "hb_hdb.cpp")]
+       os.path.join(ROOT, "h-blast_amd", "csrc", "hb_hdb.cpp"),
+       os.path.join(ROOT, "h-blast_amd", "csrc", "hb_balance.cpp"),
+       os.path.join(ROOT, "h-blast_amd", "csrc", "hb_seg.cpp")]
 CSRC = [os.path.join(ROOT, "oracle", "hbo.c"), os.path.join(ROOT, "oracle", "hbo_gap.c"),
-        os.path.join(ROOT, "oracle", "hbo_hdb.c")]
+        os.path.join(ROOT, "oracle", "hbo_hdb.c"), os.path.join(ROOT, "oracle", "hbo_balance.c"),
+        os.path.join(ROOT, "oracle", "hbo_seg.c")]
 INC = ["-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "oracle")]
 
 pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")
