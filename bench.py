@@ -55,11 +55,12 @@ def workload_c1(n_subjects):
 
 
 def algorithmic_bytes(n_res, n_subj, n_hsp, n_entries):
-    """Bytes the stage must move per pass: residues (1 B each), subject
-    metadata (8 B offset + 4 B order index), per-subject counters written
-    (12 B), HSP records (32 B each), and the lookup table read once
-    (256 KB cells + 4 KB presence vector + 4 B per entry)."""
-    return n_res + 12 * n_subj + 12 * n_subj + 32 * n_hsp + 262144 + 4096 + 4 * n_entries
+    """Bytes the stage must move per pass: residues (1 B each), one subject
+    record read in processing order (4 B oid, 4 B length, 8 B offset),
+    per-subject counters written (12 B), HSP records (32 B each), and the
+    lookup table read once (256 KB cells + 4 KB presence vector + 4 B per
+    entry)."""
+    return n_res + 16 * n_subj + 12 * n_subj + 32 * n_hsp + 262144 + 4096 + 4 * n_entries
 
 
 def cpu_baseline(q, res, offs, seconds_target=10.0):
@@ -585,7 +586,7 @@ def bench_c3(args):
         f"{sum(len(q) for q in qs)} query residues")
     m = measure_batched(ctx, db, qs, args.steps, args.warmup, args.plain_sample,
                         group_residues=args.group_residues)
-    abytes = n_res + 24 * (len(offs) - 1) + 32 * m["hsps"]
+    abytes = n_res + 28 * (len(offs) - 1) + 32 * m["hsps"]
     out = {
         "metric": "queries/s (batched vs plain)",
         "value": m["queries_per_s"],
@@ -666,7 +667,7 @@ def bench_c4(args):
         f"{args.duration:.0f} s at {args.qps} q/s")
     m = measure_served(ctx, db, specs, arrivals, args.max_batch_residues, args.max_inflight,
                        group_residues=args.group_residues)
-    abytes = m["batches"] * (n_res + 24 * (len(offs) - 1)) + 32 * m["hsps"]
+    abytes = m["batches"] * (n_res + 28 * (len(offs) - 1)) + 32 * m["hsps"]
     out = {
         "metric": "queries/s served (p50/p99 latency)",
         "value": m["queries_per_s"],

@@ -232,7 +232,7 @@ static int enqueue(hb_batch *b, uint32_t t_begin, uint32_t t_end)
         memset(&a, 0, sizeof(a));
         a.res = db->d_res;
         a.offs = db->d_offs;
-        a.order = db->d_order;
+        a.subj = db->d_subj;
         a.tiles = db->d_tiles;
         a.tile_begin = t_begin;
         a.tile_end = t_end;

@@ -43,8 +43,10 @@ struct hb_scan_args {
     // database (resident)
     const uint8_t *res;          // all residues
     const uint64_t *offs;        // n+1 subject offsets
-    const uint32_t *order;       // processing order (length-descending oids)
-    const uint32_t *tiles;       // tile boundaries into order
+    // per subject in processing order (length-descending): {oid, length,
+    // residue offset low, high}, read sequentially by the scan
+    const uint4 *subj;
+    const uint32_t *tiles;       // tile boundaries into subj
     uint32_t tile_begin;         // tiles [tile_begin, tile_end) are scanned
     uint32_t tile_end;
     uint32_t tiles_per_item;     // batched: tiles per (group, tile chunk) work item

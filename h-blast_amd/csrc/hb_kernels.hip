@@ -754,9 +754,10 @@ struct WaveScan {
 
     __device__ __forceinline__ void scan_subject(uint32_t it)
     {
-        const uint32_t oid = a->order[it];
-        const uint64_t sb = a->offs[oid];
-        const int32_t slen = (int32_t)(a->offs[oid + 1] - sb);
+        const uint4 m = a->subj[it];
+        const uint32_t oid = m.x;
+        const int32_t slen = (int32_t)m.y;
+        const uint64_t sb = (uint64_t)m.z | ((uint64_t)m.w << 32);
         const uint8_t *S = a->res + sb;
         if ((uint32_t)slen <= a->scap) {
             uint8_t *B = cs ? sbuf1 : sbuf0;

@@ -52,7 +52,7 @@ struct hb_db {
     uint64_t total = 0;
     uint8_t *d_res = nullptr;
     uint64_t *d_offs = nullptr;
-    uint32_t *d_order = nullptr;
+    uint4 *d_subj = nullptr;     // {oid, length, offset lo, hi} in processing order
     uint32_t *d_tiles = nullptr;
     uint32_t n_tiles = 0;
     std::vector<uint64_t> h_offs;

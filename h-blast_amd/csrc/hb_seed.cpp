@@ -533,14 +533,15 @@ int hb_db_free(hb_db *db)
     hipSetDevice(db->ctx->device);
     hipFree(db->d_res);
     hipFree(db->d_offs);
-    hipFree(db->d_order);
+    hipFree(db->d_subj);
     hipFree(db->d_tiles);
     delete db;
     return HB_OK;
 }
 
 // Processing order and tiles of a database whose host offsets are set, and
-// the device copies of offsets, order and tiles.
+// the device copies of offsets, per-subject records in processing order and
+// tiles.
 static int db_meta(hb_db *db)
 {
     const uint32_t n = db->n;
@@ -570,12 +571,18 @@ static int db_meta(hb_db *db)
         order.swap(tmp);
         key.swap(key2);
     }
+    // the scan reads each subject's oid, length and offset from one 16-byte
+    // record in processing order (sequential), not offs[oid] (one scattered
+    // cache line per subject)
+    std::vector<uint4> subj(n);
     std::vector<uint32_t> tiles;
     tiles.push_back(0);
     uint64_t acc = 0;
     uint32_t cnt = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        acc += db->h_offs[order[i] + 1] - db->h_offs[order[i]];
+        const uint64_t o = db->h_offs[order[i]], len = db->h_offs[order[i] + 1] - o;
+        subj[i] = make_uint4(order[i], (uint32_t)len, (uint32_t)o, (uint32_t)(o >> 32));
+        acc += len;
         ++cnt;
         if (cnt >= 64 || acc >= 16384) {
             tiles.push_back(i + 1);
@@ -586,12 +593,12 @@ static int db_meta(hb_db *db)
     if (tiles.back() != n) tiles.push_back(n);
     db->n_tiles = (uint32_t)tiles.size() - 1;
     if (hipMalloc(&db->d_offs, ((size_t)n + 1) * sizeof(uint64_t)) != hipSuccess ||
-        hipMalloc(&db->d_order, (size_t)std::max<uint32_t>(n, 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&db->d_subj, (size_t)std::max<uint32_t>(n, 1) * sizeof(uint4)) != hipSuccess ||
         hipMalloc(&db->d_tiles, tiles.size() * sizeof(uint32_t)) != hipSuccess)
         return HB_ERR_NOMEM;
     if (hipMemcpy(db->d_offs, db->h_offs.data(), ((size_t)n + 1) * sizeof(uint64_t),
                   hipMemcpyHostToDevice) != hipSuccess ||
-        (n && hipMemcpy(db->d_order, order.data(), (size_t)n * sizeof(uint32_t),
+        (n && hipMemcpy(db->d_subj, subj.data(), (size_t)n * sizeof(uint4),
                         hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(db->d_tiles, tiles.data(), tiles.size() * sizeof(uint32_t),
                   hipMemcpyHostToDevice) != hipSuccess)
@@ -950,7 +957,7 @@ static int launch_scan(hb_scan *s)
     memset(&a, 0, sizeof(a));
     a.res = db->d_res;
     a.offs = db->d_offs;
-    a.order = db->d_order;
+    a.subj = db->d_subj;
     a.tiles = db->d_tiles;
     a.tile_begin = 0;
     a.tile_end = db->n_tiles;
