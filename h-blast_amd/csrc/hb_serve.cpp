@@ -182,9 +182,10 @@ static bool admit(hb_server *s)
     int rc = hb_batch_create_on(s->ctx, s->db, &s->params, &bo, s->streams[f.slot], &f.b);
     for (size_t i = 0; rc == HB_OK && i < f.reqs.size(); ++i) {
         const hb_concat &q = f.reqs[i]->q;
-        rc = hb_batch_add(f.b, q.seq.data(), (int32_t)q.seq.size(), (int32_t)q.ctx_off.size(),
-                          q.ctx_off.data(), q.ctx_len.data(), q.xdrop.data(), q.cutoff.data(),
-                          nullptr);
+        rc = hb_batch_add_masked(f.b, q.seq.data(), (int32_t)q.seq.size(),
+                                 (int32_t)q.ctx_off.size(), q.ctx_off.data(), q.ctx_len.data(),
+                                 q.xdrop.data(), q.cutoff.data(),
+                                 q.mask.empty() ? nullptr : q.mask.data(), nullptr);
     }
     if (rc == HB_OK)
         rc = hb_batch_reserve(f.b, (uint32_t)std::min<uint64_t>(hsps, 0xF0000000ull));
@@ -296,11 +297,20 @@ int hb_serve_submit(hb_server *s, const uint8_t *seq, int32_t len, int32_t n_ctx
                     const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
                     const int32_t *cutoff, uint64_t *ticket)
 {
+    return hb_serve_submit_masked(s, seq, len, n_ctx, ctx_off, ctx_len, xdrop, cutoff, nullptr,
+                                  ticket);
+}
+
+int hb_serve_submit_masked(hb_server *s, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                           const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
+                           const int32_t *cutoff, const uint8_t *lut_mask, uint64_t *ticket)
+{
     if (!s || !cutoff || !ticket) return HB_ERR_ARG;
     int rc = hb_check_query(seq, len, n_ctx, ctx_off, ctx_len, xdrop);
     if (rc) return rc;
     auto r = std::make_shared<Request>();
     r->q.seq.assign(seq, seq + len);
+    if (lut_mask) r->q.mask.assign(lut_mask, lut_mask + len);
     r->q.ctx_off.assign(ctx_off, ctx_off + n_ctx);
     r->q.ctx_len.assign(ctx_len, ctx_len + n_ctx);
     r->q.xdrop.assign(xdrop, xdrop + n_ctx);

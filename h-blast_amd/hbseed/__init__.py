@@ -63,7 +63,8 @@ EXPORTS = [
     "hb_balance_next_ratio", "hb_seg_params_default", "hb_seg_mask",
     "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
     "hb_batch_results", "hb_batch_info_get", "hb_batch_free",
-    "hb_serve_opts_default", "hb_serve_start", "hb_serve_submit", "hb_serve_wait",
+    "hb_serve_opts_default", "hb_serve_start", "hb_serve_submit", "hb_serve_submit_masked",
+    "hb_serve_wait",
     "hb_serve_results", "hb_serve_stats_get", "hb_serve_stop",
 ]
 
@@ -235,6 +236,8 @@ def _lib():
         L.hb_serve_opts_default.restype = None
         L.hb_serve_start.argtypes = [vp, vp, vp, vp, pp]
         L.hb_serve_submit.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, C.POINTER(u64)]
+        L.hb_serve_submit_masked.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, vp,
+                                             C.POINTER(u64)]
         L.hb_serve_wait.argtypes = [vp, u64, i32]
         L.hb_serve_results.argtypes = [vp, u64, vp, u64, C.POINTER(u64)]
         L.hb_serve_stats_get.argtypes = [vp, vp]
@@ -935,9 +938,17 @@ class Server:
         """Returns the request's ticket; raises HBError (code -8 busy, -4
         capacity) when the request is not admitted."""
         t = C.c_uint64()
-        _check(_lib().hb_serve_submit(self.h, _ptr(spec.seq), len(spec.seq), spec.n_ctx,
-                                      _ptr(spec.ctx_off), _ptr(spec.ctx_len), _ptr(spec.xdrop),
-                                      _ptr(spec.cutoff), C.byref(t)), "hb_serve_submit")
+        if spec.lut_mask is not None:
+            _check(_lib().hb_serve_submit_masked(self.h, _ptr(spec.seq), len(spec.seq),
+                                                 spec.n_ctx, _ptr(spec.ctx_off),
+                                                 _ptr(spec.ctx_len), _ptr(spec.xdrop),
+                                                 _ptr(spec.cutoff), _ptr(spec.lut_mask),
+                                                 C.byref(t)), "hb_serve_submit_masked")
+        else:
+            _check(_lib().hb_serve_submit(self.h, _ptr(spec.seq), len(spec.seq), spec.n_ctx,
+                                          _ptr(spec.ctx_off), _ptr(spec.ctx_len),
+                                          _ptr(spec.xdrop), _ptr(spec.cutoff), C.byref(t)),
+                   "hb_serve_submit")
         return t.value
 
     def wait(self, ticket, timeout_ms=-1):

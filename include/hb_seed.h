@@ -574,6 +574,11 @@ int hb_serve_start(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_ser
 int hb_serve_submit(hb_server *s, const uint8_t *seq, int32_t len, int32_t n_ctx,
                     const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
                     const int32_t *cutoff, uint64_t *ticket);
+/* As hb_serve_submit with a lookup-table mask (as hb_query_prepare_masked):
+ * a served BLASTX request masked by SEG, as BLAST+ masks it by default. */
+int hb_serve_submit_masked(hb_server *s, const uint8_t *seq, int32_t len, int32_t n_ctx,
+                           const int32_t *ctx_off, const int32_t *ctx_len, const int32_t *xdrop,
+                           const int32_t *cutoff, const uint8_t *lut_mask, uint64_t *ticket);
 /* Waits up to timeout_ms (< 0: no limit) for the request: HB_OK when its
  * results are ready, HB_ERR_BUSY on time-out, the batch's error otherwise. */
 int hb_serve_wait(hb_server *s, uint64_t ticket, int32_t timeout_ms);

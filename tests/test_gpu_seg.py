@@ -1,8 +1,8 @@
 """GPU parity of SEG-masked queries: a query with low-complexity stretches,
 masked by hb_seg_mask (hard masking: lookup-table mask plus X residues, as
 BLAST+ applies -seg), scans bit-exactly like the oracle scanning the same
-masked query, for BLASTP and six-frame BLASTX; the mask removes the
-low-complexity hits and keeps the homolog's.
+masked query, for BLASTP and six-frame BLASTX, in the plain and the served
+forms; the mask removes the low-complexity hits and keeps the homolog's.
 """
 import numpy as np
 import pytest
@@ -61,6 +61,14 @@ def test_seg_masked_queries_match_oracle():
         x0 = scan_both(ctx, db, xs, res, offs)
         x1 = scan_both(ctx, db, xm, res, offs)
         assert int((x1["oid"] >= 20000).sum()) < int((x0["oid"] >= 20000).sum())
+        # the same masked queries served together with unmasked ones: each
+        # result equals its plain scan
+        with hbseed.Server(ctx, db) as srv:
+            specs = [masked, xm, plain, xs]
+            tickets = [srv.submit(sp) for sp in specs]
+            for t, want in zip(tickets, [h1, x1, h0, x0]):
+                assert srv.wait(t, 120000)
+                assert np.array_equal(srv.results(t).view(np.int32), want.view(np.int32))
     finally:
         db.close()
         ctx.close()
