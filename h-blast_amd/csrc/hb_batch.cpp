@@ -230,7 +230,7 @@ static int enqueue(hb_batch *b, uint32_t t_begin, uint32_t t_end)
         const Class &C = b->classes[ci];
         hb_scan_args a;
         memset(&a, 0, sizeof(a));
-        a.res = db->d_res;
+        a.res = db->d_scan_res;   // subjects in processing order (a.subj's offsets)
         a.offs = db->d_offs;
         a.subj = db->d_subj;
         a.tiles = db->d_tiles;

@@ -100,6 +100,11 @@ static inline size_t hb_scan_fixed_lds(int32_t qlen_pad, int32_t ctx_pad)
 }
 
 // Kernel launchers (hb_kernels.hip).
+// Copies every subject's residues (res at offs[oid]) to its processing-order
+// place in out (the record's offset), one workgroup per tile.
+hipError_t hb_launch_gather_subjects(const uint8_t *res, const uint64_t *offs, const uint4 *subj,
+                                     const uint32_t *tiles, uint32_t n_tiles, uint8_t *out,
+                                     hipStream_t st);
 hipError_t hb_launch_check_residues(const uint8_t *res, uint64_t n, uint32_t *flag,
                                     hipStream_t st);
 // Lookup tables of n groups (grid.y = group).  qflag[q] = residue | 0x80 when

@@ -1152,6 +1152,28 @@ hipError_t hb_launch_fill_u32(uint32_t *dst, uint32_t value, uint32_t n, hipStre
     return hipGetLastError();
 }
 
+__global__ void gather_subjects_kernel(const uint8_t *res, const uint64_t *offs, const uint4 *subj,
+                                       const uint32_t *tiles, uint8_t *out)
+{
+    const uint32_t t = blockIdx.x;
+    for (uint32_t i = tiles[t]; i < tiles[t + 1]; ++i) {
+        const uint4 m = subj[i];
+        const uint8_t *src = res + offs[m.x];
+        uint8_t *dst = out + ((uint64_t)m.z | ((uint64_t)m.w << 32));
+        for (uint32_t j = threadIdx.x; j < m.y; j += blockDim.x) dst[j] = src[j];
+    }
+}
+
+hipError_t hb_launch_gather_subjects(const uint8_t *res, const uint64_t *offs, const uint4 *subj,
+                                     const uint32_t *tiles, uint32_t n_tiles, uint8_t *out,
+                                     hipStream_t st)
+{
+    if (n_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_subjects_kernel, dim3(n_tiles), dim3(256), 0, st, res, offs, subj,
+                       tiles, out);
+    return hipGetLastError();
+}
+
 hipError_t hb_launch_check_residues(const uint8_t *res, uint64_t n, uint32_t *flag,
                                     hipStream_t st)
 {

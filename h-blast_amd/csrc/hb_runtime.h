@@ -52,7 +52,8 @@ struct hb_db {
     uint64_t total = 0;
     uint8_t *d_res = nullptr;
     uint64_t *d_offs = nullptr;
-    uint4 *d_subj = nullptr;     // {oid, length, offset lo, hi} in processing order
+    uint4 *d_subj = nullptr;     // {oid, length, offset into d_scan_res} in processing order
+    uint8_t *d_scan_res = nullptr;  // residues in processing order: the scan's copy
     uint32_t *d_tiles = nullptr;
     uint32_t n_tiles = 0;
     std::vector<uint64_t> h_offs;

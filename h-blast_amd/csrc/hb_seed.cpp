@@ -534,6 +534,7 @@ int hb_db_free(hb_db *db)
     hipFree(db->d_res);
     hipFree(db->d_offs);
     hipFree(db->d_subj);
+    hipFree(db->d_scan_res);
     hipFree(db->d_tiles);
     delete db;
     return HB_OK;
@@ -573,15 +574,18 @@ static int db_meta(hb_db *db)
     }
     // the scan reads each subject's oid, length and offset from one 16-byte
     // record in processing order (sequential), not offs[oid] (one scattered
-    // cache line per subject)
+    // cache line per subject); the offset points into the scan's copy of
+    // the residues, laid out in the same order, so consecutive subjects share
+    // no cache line with subjects read at other times
     std::vector<uint4> subj(n);
     std::vector<uint32_t> tiles;
     tiles.push_back(0);
-    uint64_t acc = 0;
+    uint64_t acc = 0, dst = 0;
     uint32_t cnt = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        const uint64_t o = db->h_offs[order[i]], len = db->h_offs[order[i] + 1] - o;
-        subj[i] = make_uint4(order[i], (uint32_t)len, (uint32_t)o, (uint32_t)(o >> 32));
+        const uint64_t len = db->h_offs[order[i] + 1] - db->h_offs[order[i]];
+        subj[i] = make_uint4(order[i], (uint32_t)len, (uint32_t)dst, (uint32_t)(dst >> 32));
+        dst += len;
         acc += len;
         ++cnt;
         if (cnt >= 64 || acc >= 16384) {
@@ -602,6 +606,20 @@ static int db_meta(hb_db *db)
                         hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(db->d_tiles, tiles.data(), tiles.size() * sizeof(uint32_t),
                   hipMemcpyHostToDevice) != hipSuccess)
+        return HB_ERR_DEVICE;
+    return HB_OK;
+}
+
+// The scan's copy of the residues in processing order (after d_res holds
+// the database).
+static int db_scan_copy(hb_db *db)
+{
+    hb_ctx *c = db->ctx;
+    if (hipMalloc(&db->d_scan_res, db->total + 64) != hipSuccess) return HB_ERR_NOMEM;
+    if (hipMemsetAsync(db->d_scan_res + db->total, 0, 64, c->stream) != hipSuccess ||
+        hb_launch_gather_subjects(db->d_res, db->d_offs, db->d_subj, db->d_tiles, db->n_tiles,
+                                  db->d_scan_res, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
         return HB_ERR_DEVICE;
     return HB_OK;
 }
@@ -654,6 +672,7 @@ int hb_db_upload(hb_ctx *c, const uint8_t *residues, const uint64_t *offsets, ui
         return fail(HB_ERR_DEVICE);
     int rc = db_meta(db);
     if (rc == HB_OK) rc = db_check_residues(db);
+    if (rc == HB_OK) rc = db_scan_copy(db);
     if (rc != HB_OK) return fail(rc);
     *out = db;
     return HB_OK;
@@ -761,6 +780,8 @@ int hb_db_upload_hdb(hb_ctx *c, const uint8_t *hdb, uint64_t bytes, int32_t bloc
     hipEventElapsedTime(&db->hdb_len_ms, ev[0], ev[1]);
     hipEventElapsedTime(&db->hdb_copy_ms, ev[2], ev[3]);
     if (bad) return fail(HB_ERR_RESIDUE);   // checked by the copy kernel
+    const int rc2 = db_scan_copy(db);
+    if (rc2 != HB_OK) return fail(rc2);
     release();
     *out = db;
     return HB_OK;
@@ -955,7 +976,7 @@ static int launch_scan(hb_scan *s)
     }
     hb_scan_args a;
     memset(&a, 0, sizeof(a));
-    a.res = db->d_res;
+    a.res = db->d_scan_res;   // subjects in processing order (a.subj's offsets)
     a.offs = db->d_offs;
     a.subj = db->d_subj;
     a.tiles = db->d_tiles;
