@@ -118,7 +118,8 @@ int hb_host_free(hb_ctx *ctx, void *p);
 /* --- packed subject database -------------------------------------------
  * Replaces Read_H_BLAST_GPU_DB + the per-call DB upload
  * (H-BLAST/h_blast.cu:587-653, 1094-1096): the database is uploaded once
- * and stays resident in HBM across scans.  Subject i is
+ * and stays resident in HBM across scans (its residues twice: in oid order,
+ * and in the scan's length-descending order).  Subject i is
  * residues[offsets[i] .. offsets[i+1]); offsets has n_subjects+1 entries. */
 int hb_db_upload(hb_ctx *ctx, const uint8_t *residues, const uint64_t *offsets,
                  uint32_t n_subjects, hb_db **out);
