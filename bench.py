@@ -531,9 +531,9 @@ def bench_qlen(args, n_subjects=200_000,
                lengths=(1000, 2000, 4000, 6000, 8000, 12000, 16000, 24000, 31000)):
     """Plain BLASTP scans of one query per length (i.i.d. residues, seed
     7000 + length) against 200,000 subjects of the default database
-    generator (seed 1100), with the per-diagonal two-hit state in LDS as
-    32-bit and as 16-bit entries (where they fit) and in global memory; the
-    best of two timed scans after one warm-up, device time per scan."""
+    generator (seed 1100), with the per-diagonal two-hit state in LDS
+    (where it fits) and in global memory; the best of two timed scans after
+    one warm-up, device time per scan."""
     res, offs = hbseed.make_db(1100, n_subjects)
     ctx = hbseed.Context(0)
     db = hbseed.Database(ctx, res, offs)
@@ -541,11 +541,10 @@ def bench_qlen(args, n_subjects=200_000,
     for n in lengths:
         q = hbseed.Query(ctx, hbseed.blastp_query(hbseed.random_protein(7000 + n, n)))
         s = hbseed.Scan(ctx, db, q)
-        row = {"query_len": n, "auto": {hbseed.STATE_LDS: "lds", hbseed.STATE_LDS16: "lds16",
+        row = {"query_len": n, "auto": {hbseed.STATE_LDS: "lds",
                                         hbseed.STATE_GLOBAL: "global"}[s.state_mode]}
         s.close()
-        for mode, key in ((hbseed.STATE_LDS, "lds_ms"), (hbseed.STATE_LDS16, "lds16_ms"),
-                          (hbseed.STATE_GLOBAL, "global_ms")):
+        for mode, key in ((hbseed.STATE_LDS, "lds_ms"), (hbseed.STATE_GLOBAL, "global_ms")):
             try:
                 q.set_state_mode(mode)
             except hbseed.HBError:

@@ -18,8 +18,8 @@ struct Class {
     int32_t qlen_pad_max = 0, ctx_pad_max = 4, diag_len_max = 64;
     hb_launch_cfg cfg;
     uint32_t tiles_per_item = 1;
-    int place = HB_PLACE_LDS32;        // where the diagonal state goes
-    int32_t *d_gstate = nullptr;       // diag_len_max int32 per wavefront of the grid
+    int place = HB_PLACE_LDS;          // where the diagonal state goes
+    unsigned long long *d_gstate = nullptr;   // diag_len_max words per wavefront of the grid
     size_t gstate_bytes = 0;
 };
 
@@ -174,6 +174,11 @@ static int build(hb_batch *b, int32_t target)
     std::vector<hb_concat> groups;
     std::vector<int32_t> ctx_base, diag;
     make_groups(b, target, &groups, &ctx_base, &diag);
+    // every query fits alone (checked when added); a group of several fits
+    // unless the group size itself is too large for LDS
+    for (const hb_concat &G : groups)
+        if (!hb_query_lds_ok(b->ctx, (int32_t)G.seq.size(), (int32_t)G.ctx_off.size()))
+            return HB_ERR_UNSUPPORTED;
     int rc = hb_group_set_build(b->ctx, groups, ctx_base, b->p, b->st, &b->gs);
     if (rc) return rc;
     const int32_t ng = (int32_t)groups.size();
@@ -201,8 +206,9 @@ static int build(hb_batch *b, int32_t target)
                             n_groups * b->db->n_tiles, &c.cfg, c.place);
         if (rc) return rc;
         if (c.place == HB_PLACE_GLOBAL) {
-            c.gstate_bytes = (size_t)c.cfg.blocks * c.cfg.waves * (size_t)c.diag_len_max * 4;
-            c.d_gstate = static_cast<int32_t *>(hb_pool_get(b->ctx, c.gstate_bytes));
+            c.gstate_bytes =
+                (size_t)c.cfg.blocks * c.cfg.waves * (size_t)c.diag_len_max * HB_GSTATE_WORD;
+            c.d_gstate = static_cast<unsigned long long *>(hb_pool_get(b->ctx, c.gstate_bytes));
             if (!c.d_gstate) return HB_ERR_NOMEM;
         }
     }
@@ -242,11 +248,11 @@ static int enqueue(hb_batch *b, uint32_t t_begin, uint32_t t_end)
         a.qlen_pad_max = C.qlen_pad_max;
         a.ctx_pad_max = C.ctx_pad_max;
         a.diag_len_max = C.diag_len_max;
-        a.wave_lds = (uint32_t)hb_scan_wave_lds(C.diag_len_max, C.cfg.scap,
+        a.wave_lds = (uint32_t)hb_scan_wave_lds(C.diag_len_max, C.cfg.ring,
                                                 hb_state_bytes(C.place));
-        a.scap = (uint32_t)C.cfg.scap;
+        a.ring = (uint32_t)C.cfg.ring;
         a.gstate = C.d_gstate;   // cleared by each wavefront when it binds a group
-        a.state_limit = C.place == HB_PLACE_LDS16 ? hb_state16_limit(db->max_len) : 0;
+        a.state_limit = hb_state_limit(C.place, db->max_len);
         a.mat32 = b->gs.d_mat;
         a.window = b->p.window;
         a.work_counter = b->d_misc + 1 + ci;
@@ -548,6 +554,9 @@ int hb_batch_add_masked(hb_batch *b, const uint8_t *seq, int32_t len, int32_t n_
     if (b->built) return HB_ERR_STATE;
     int rc = hb_check_query(seq, len, n_ctx, ctx_off, ctx_len, xdrop);
     if (rc) return rc;
+    // the query (and its group) must fit a workgroup's LDS: refused here, not
+    // when the batch is built
+    if (!hb_query_lds_ok(b->ctx, len, n_ctx)) return HB_ERR_UNSUPPORTED;
     hb_concat q;
     q.seq.assign(seq, seq + len);
     q.ctx_off.assign(ctx_off, ctx_off + n_ctx);

@@ -10,15 +10,19 @@
 #define HB_NCELLS 32768     // 3 letters x 5 bits
 #define HB_PV_WORDS 1024    // HB_NCELLS / 32
 #define HB_WAVE 64
-#define HB_JOBCAP 128       // queued two-hit extensions per wave
+#define HB_JOBCAP 128       // queued two-hit extensions per wave (a ring)
 #define HB_BLKCAP 128       // hits waiting on a pending diagonal per wave
-#ifndef HB_SCAP_MAX
-#define HB_SCAP_MAX 2048    // subjects up to the launch's staging size (<= this) are staged in LDS
-#endif
-#ifndef HB_MARK
-#define HB_MARK 256         // same-round ordering markers per wave (power of two)
+#define HB_SLOTS 8          // staged subjects per wave whose extensions may still be queued
+#ifndef HB_RING_MAX
+#define HB_RING_MAX 4096    // largest staging ring per wave (subjects up to the ring are staged)
 #endif
 #define HB_BLOCK 256        // at most 4 wavefronts per workgroup
+// Diagonal state kept in LDS: one 32-bit word per diagonal, 23-bit positions
+// between resets (hb_scan_args::state_limit), so the longest subject and
+// its successor must fit: 2 * (longest + window) + 64 < 2^23.
+#define HB_LDS_POS_LIMIT (1 << 23)
+// ... in global memory: one 64-bit word per diagonal, 30-bit positions.
+#define HB_GLOBAL_POS_LIMIT (1 << 29)
 
 // One query group: a concatenated query (one or more queries / frames as
 // contexts) with its lookup table.  The plain path scans one group; the
@@ -41,7 +45,9 @@ struct hb_group_desc {
 // Kernel arguments of the scan.
 struct hb_scan_args {
     // database (resident)
-    const uint8_t *res;          // all residues
+    // the scan's copy of the residues, in processing order: each subject starts
+    // 16-byte aligned and its span is padded to a multiple of 16 bytes
+    const uint8_t *res;
     const uint64_t *offs;        // n+1 subject offsets
     // per subject in processing order (length-descending): {oid, length,
     // residue offset low, high}, read sequentially by the scan
@@ -58,7 +64,8 @@ struct hb_scan_args {
     int32_t ctx_pad_max;         // multiple of 4
     int32_t diag_len_max;
     uint32_t wave_lds;           // LDS bytes per wave
-    uint32_t scap;               // subjects up to this length are staged in LDS
+    uint32_t ring;               // staging ring bytes per wave (multiple of 16): subjects up
+                                 // to this length are staged in LDS
     const int8_t *mat32;         // 32x32 scores, -128 padded
     int32_t window;
     // work queue and output
@@ -68,26 +75,27 @@ struct hb_scan_args {
     uint32_t *out_count;
     uint32_t *stats;             // plain: 3 per oid (lookup hits, extensions, HSPs); else null
     unsigned long long *group_totals;  // batched: 3 per group; else null
-    // the diagonal state in global memory, diag_len_max int32 per
+    // the diagonal state in global memory, diag_len_max 64-bit words per
     // wavefront of the grid (null: in LDS)
-    int32_t *gstate;
-    // > 0: 16-bit diagonal state in LDS, reset when a subject's base offset
-    // + length + window reaches it (16384 - the longest subject); 0: 32-bit
+    unsigned long long *gstate;
+    // the state is reset when a subject's base offset + length + window
+    // reaches this (the position limit of the placement - the longest subject)
     int32_t state_limit;
 };
 
-// per wave: diagonal state, ordering markers, 64 segment tags, job and
-// blocked queues (8 B entries), two staged subjects
+// per wave: diagonal words, 64 segment tags, job and blocked queues (8 B
+// entries), the staged subjects' slots (32 B each) and the staging ring.
 // The regions of a workgroup's wavefronts are padded so that their distance
 // modulo 512 bytes is at least 128: distances of 0 or 64 bytes (mod 512)
 // made the C1 scan 8% slower on gfx950 (measured), which points to LDS bank
 // or port conflicts between wavefronts.
-// state_bytes: bytes per diagonal of LDS state (4, 2, or 0 when the state
-// is in global memory).
-static inline size_t hb_scan_wave_lds(int32_t diag_len, int32_t scap, int state_bytes = 4)
+// state_bytes: bytes per diagonal of LDS state (4, or 0 when the state is
+// in global memory).
+static inline size_t hb_scan_wave_lds(int32_t diag_len, int32_t ring, int state_bytes = 4)
 {
-    size_t b = (size_t)state_bytes * (size_t)diag_len + 4 * HB_MARK + 256 +
-               8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + 2 * (size_t)scap;
+    size_t b = (size_t)state_bytes * (size_t)diag_len + 256 +
+               8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + 32 * HB_SLOTS + (size_t)ring;
+    b = (b + 15) & ~(size_t)15;
     const size_t r = b % 512;
     if (r < 128) b += 128 - r;
     return b;
@@ -128,7 +136,7 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 // launching or capturing a launch with that LDS size).
 // Raises both scan kernels' dynamic LDS limit to max_lds (once per context).
 hipError_t hb_scan_allow_lds(size_t max_lds);
-hipError_t hb_scan_occupancy(bool multi, bool gstate, bool w16, size_t lds_bytes,
+hipError_t hb_scan_occupancy(bool multi, bool gstate, size_t lds_bytes,
                              int block_threads, int *blocks_per_cu);
 hipError_t hb_launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);
 hipError_t hb_launch_fill_u32(uint32_t *dst, uint32_t value, uint32_t n, hipStream_t st);

@@ -10,16 +10,18 @@
 //     resident); a wavefront prefix sum flattens the window's hits in
 //     (s, q) order;
 //   * hits are decided 64 per round in one pass: the per-diagonal state
-//     {last_hit + offset | flag | pending} lives in LDS (int32 per diagonal),
-//     hits of a round on one diagonal are ordered by an LDS atomic-max
-//     marker, and a hit that loses is carried to the next round ahead of
-//     newer hits, so every diagonal sees its hits in subject order -- the
-//     exact sequential semantics of the CPU two-hit word finder (the
-//     reference kernel's deferral, h_blast_kernel.cu:873-942, is not
-//     reproduced);
+//     {last_hit + offset | flag | pending} lives in one 32-bit LDS word per
+//     diagonal whose top bits order the hits of a round that share the
+//     diagonal (an atomic max of 64 - lane; the lowest lane decides), and a
+//     hit that loses is carried to the next round ahead of newer hits, so
+//     every diagonal sees its hits in subject order -- the exact sequential
+//     semantics of the CPU two-hit word finder (the reference kernel's
+//     deferral, h_blast_kernel.cu:873-942, is not reproduced);
 //   * extensions are queued and run 64 at a time, one per lane, as
-//     branch-free X-drop walks over LDS data; hits on diagonals whose
-//     extension is pending wait in an ordered list and are replayed after.
+//     branch-free X-drop walks over LDS data; subjects stay staged in a
+//     per-wavefront ring until their extensions ran, so a batch collects
+//     the extensions of several subjects; hits on diagonals whose extension
+//     is pending wait in an ordered list and are replayed after.
 #include <hip/hip_runtime.h>
 
 #include "hb_internal.h"
@@ -317,41 +319,53 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 
 // ---------------------------------------------------------------- scan
 // Per-wavefront LDS of the scan (D = diagonal array length of the launch):
-//   state[D]   int32  diagonal state: bit 31 = extended-right flag,
-//                     bit 30 = extension pending, bits 0..29 = last hit + offset
-//   marker[HB_MARK] uint32 same-pass ordering marker (generation << 6 | 63 - lane)
-//                     of the diagonals d with d mod HB_MARK = slot; hits of
-//                     different diagonals that share a slot are ordered like
-//                     hits of one diagonal, which is safe (only slower)
+//   word[D]    uint32 per diagonal: bits 0..22 last hit + offset, bit 23
+//              extension pending, bit 24 extended-right flag, bits 25..31
+//              the ordering marker of the decision pass under way (0
+//              between passes).  With the state in global memory (long
+//              queries) the word is a uint64 in the wavefront's own region:
+//              30-bit position, bit 30 pending, bit 31 flag, marker in the
+//              high half.
 //   seg[64]    uint32 tagged segment starts of the current hit round
-//   jobs[kJobCap] uint2 queued two-hit extensions (q | (s - last) << 24, s)
-//   blk[kBlkCap]  uint2 hits waiting on a pending diagonal (q, s)
-//   sbuf[scap]    residues of the current subject (if it fits; the host picks
-//                 scap <= HB_SCAP_MAX so that the LDS layout gives the most
-//                 wavefronts per CU)
-// Per-workgroup LDS: score matrix, context tables (stride
-// ctx_pad_max) and query of the group being scanned.
-constexpr int kJobCap = HB_JOBCAP;
-constexpr int kBlkCap = HB_BLKCAP;
-static_assert((HB_MARK & (HB_MARK - 1)) == 0 && HB_MARK >= HB_WAVE, "HB_MARK: power of two >= 64");
-constexpr uint32_t kDeferJobs = 40;   // a staged subject ending with fewer queued extensions defers them
-constexpr uint32_t kFlag = 0x80000000u;
-constexpr uint32_t kPend = 0x40000000u;
-constexpr uint32_t kVal = 0x3fffffffu;
+//   jobs[kJobCap] uint2 queued two-hit extensions (a ring, oldest first):
+//              (q | (s - last) << 24, s | slot << 28)
+//   blk[kBlkCap]  uint2 hits waiting on a pending diagonal (q, s | slot << 28)
+//   slot[HB_SLOTS] per staged subject whose extensions may still be queued:
+//              {oid, length, diagonal offset, ring offset} and {lookup hits,
+//              extensions, HSPs, 0}
+//   ring[ring] residues of those subjects (16-byte aligned), filled from
+//              the front and emptied whole when every queued extension ran
+// Per-workgroup LDS: score matrix, context tables (stride ctx_pad_max) and
+// query of the group being scanned.
+constexpr uint32_t kJobCap = HB_JOBCAP;
+constexpr uint32_t kBlkCap = HB_BLKCAP;
+static_assert((kJobCap & (kJobCap - 1)) == 0 && kJobCap >= 2 * HB_WAVE, "job ring");
+static_assert(kBlkCap >= 2 * HB_WAVE, "blocked list");
+static_assert(HB_SLOTS <= 8, "slot field: 3 bits");
+constexpr uint32_t kSoMask = 0x0fffffffu;   // subject offset in job and blocked records
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-template <bool G, bool W16>
+// Diagonal word formats: marker shift, flag, pending, position mask.
+template <bool G> struct DiagWord;
+template <> struct DiagWord<false> {
+    using T = uint32_t;
+    static constexpr int kMS = 25;
+    static constexpr uint32_t kF = 1u << 24, kP = 1u << 23, kV = (1u << 23) - 1, kSM = (1u << 25) - 1;
+};
+template <> struct DiagWord<true> {
+    using T = unsigned long long;
+    static constexpr int kMS = 32;
+    static constexpr uint32_t kF = 1u << 31, kP = 1u << 30, kV = (1u << 30) - 1, kSM = 0xffffffffu;
+};
+
+template <bool G>
 struct WaveScan {
-    // State layout: W16 = false: int32, bit 31 flag, bit 30 pending, 30-bit
-    // position; W16 = true: uint16, bit 15 flag, bit 14 pending, 14-bit
-    // position (the host resets the state often enough: a->state_limit).
-    static constexpr uint32_t kF = W16 ? 0x8000u : kFlag;
-    static constexpr uint32_t kP = W16 ? 0x4000u : kPend;
-    static constexpr uint32_t kV = W16 ? 0x3fffu : kVal;
+    using W = DiagWord<G>;
+    using word_t = typename W::T;
     const hb_scan_args *a;
     // group being scanned
     // Lookup-table pointers come from a descriptor in memory, so the
@@ -365,68 +379,74 @@ struct WaveScan {
     int32_t n_ctx;
     int32_t ctx_base;
     int32_t D;
+    uint32_t dmul;     // ceil(2^32 / D): x mod D by a multiply-high
     // LDS tables
     const int8_t *mat;
     const uint8_t *qry;
     const int32_t *ctx;   // ctx_off[ctx_pad], xdrop[ctx_pad], cutoff[ctx_pad]
     int32_t ctx_pad;
-    int32_t *state;
-    uint16_t *state16;   // W16
-    uint32_t *marker;
+    word_t *wd;        // diagonal words (LDS, or the wavefront's global region)
     uint32_t *seg;
     uint2 *jobs;
     uint2 *blk;
-    uint8_t *sbuf0, *sbuf1;  // two staging buffers (scap bytes each)
+    uint4 *sla;        // per slot {oid, length, diagonal offset, ring offset}
+    uint4 *slb;        // per slot {lookup hits, extensions, HSPs, 0}
+    uint8_t *ring;
     int lane;
     int32_t window;
-    int32_t base;      // diagonal offset: stale entries read as "no recent hit"
-    uint32_t gen;      // marker generation
+    int32_t base;      // diagonal offset of the next subject: stale entries read as "no recent hit"
     uint32_t rgen;     // segment-tag generation
+    uint32_t j_head;   // oldest queued extension (wave-uniform)
     uint32_t n_jobs;   // wave-uniform
     uint32_t n_blk;    // wave-uniform
-    // Subject slots.  The subject being scanned is in slot cs; a staged
-    // subject that ended with few extensions queued may leave them (and its
-    // blocked hits) to run with the next subject's: it then stays in slot
-    // ds until they ran.  Job and blocked-hit records carry their slot in
-    // bit 31 of the subject offset.
-    // (Per-slot values are separate scalars: an indexed member array would
-    // move the whole struct to scratch memory.)
-    int cs;            // slot of the subject being scanned (wave-uniform)
-    int ds;            // slot of the deferred subject, -1 if none
-    uint32_t s_oid0, s_oid1, s_hits0, s_hits1;
-    int32_t s_len0, s_len1, s_base0, s_base1;
-    uint32_t n_ext0, n_ext1;  // per lane
-    uint32_t n_hsp0, n_hsp1;  // per lane
+    uint32_t n_live;   // staged subjects not yet retired (slots 0 .. n_live - 1)
+    uint32_t ring_head;
     uint64_t tot_hits, tot_ext, tot_hsp;  // wave-uniform running totals
 
+    __device__ __forceinline__ uint32_t mod_d(uint32_t x) const
+    {
+        // x < 2^28, D < 2^25: the estimate is the quotient or one more
+        const uint32_t qe = __umulhi(x, dmul);
+        const int32_t r = (int32_t)(x - qe * (uint32_t)D);
+        return (uint32_t)(r < 0 ? r + D : r);
+    }
     __device__ __forceinline__ uint32_t diag_of(uint32_t q, int32_t so) const
     {
-        int32_t d = (int32_t)q - so % D;
+        const int32_t d = (int32_t)q - (int32_t)mod_d((uint32_t)so);
         return (uint32_t)(d < 0 ? d + D : d);
     }
 
-    // Diagonal state accesses.  G = false: the state is in LDS.  G = true:
-    // it is in the wavefront's own global-memory region (queries whose
-    // state does not fit LDS); the accesses are workgroup-scope atomics and
+    // Diagonal word accesses.  G = false: LDS.  G = true: the wavefront's
+    // own global-memory region; the accesses are workgroup-scope atomics and
     // sync() waits for them, so a lane reads what another lane of the wave
     // wrote before the last sync().
-    __device__ __forceinline__ int32_t st_ld(uint32_t d) const
+    __device__ __forceinline__ word_t w_max(uint32_t d, word_t v)
     {
         if constexpr (G)
-            return __hip_atomic_load(&state[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        else if constexpr (W16)
-            return (int32_t)(uint32_t)state16[d];
+            return __hip_atomic_fetch_max(&wd[d], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         else
-            return state[d];
+            return atomicMax(&wd[d], v);
     }
-    __device__ __forceinline__ void st_st(uint32_t d, int32_t v)
+    __device__ __forceinline__ void w_or(uint32_t d, word_t v)
     {
         if constexpr (G)
-            __hip_atomic_store(&state[d], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        else if constexpr (W16)
-            state16[d] = (uint16_t)v;
+            __hip_atomic_fetch_or(&wd[d], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         else
-            state[d] = v;
+            atomicOr(&wd[d], v);
+    }
+    __device__ __forceinline__ word_t w_ld(uint32_t d) const
+    {
+        if constexpr (G)
+            return __hip_atomic_load(&wd[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            return wd[d];
+    }
+    __device__ __forceinline__ void w_st(uint32_t d, word_t v)
+    {
+        if constexpr (G)
+            __hip_atomic_store(&wd[d], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            wd[d] = v;
     }
     __device__ __forceinline__ void sync() const
     {
@@ -434,24 +454,24 @@ struct WaveScan {
         wave_lds_sync();
     }
 
-    // Starts a new group: clears the diagonal state and ordering markers.
-    __device__ __forceinline__ void reset()
+    // Clears the diagonal words (a new group, or positions near the limit).
+    __device__ __forceinline__ void clear_words()
     {
-        for (int i = lane; i < D; i += HB_WAVE) st_st((uint32_t)i, 0);
-        for (int i = lane; i < HB_MARK; i += HB_WAVE) marker[i] = 0;
-        base = window;
-        gen = 0;
+        for (int i = lane; i < D; i += HB_WAVE) w_st((uint32_t)i, 0);
         sync();
+        base = window;
     }
 
     // One pass of two-hit decisions over a set of hits (at most one per
     // lane; active lanes are in subject order by lane).  Among the hits of
-    // the set that share a diagonal only the lowest lane decides (LDS atomic
-    // max on the marker); a hit whose diagonal waits on a pending extension
-    // joins the blocked list, also a losing hit whose diagonal became
-    // pending in this pass (behind the winner, in lane order).  Returns true
-    // for the other losing hits: they decide in a later pass, before any
-    // later hit of their diagonal.
+    // the set that share a diagonal only the lowest lane decides: each hit
+    // raises the marker bits of its diagonal's word to 64 - lane with an
+    // atomic max (the max clears the state bits, and the lane that finds the
+    // marker still 0 ORs the state back), the hit that finds its own marker
+    // after the pass's sync decides, and its store of the new state clears
+    // the marker.  A winner whose diagonal waits on a pending extension joins
+    // the blocked list.  Returns true for the losing hits: they decide in a
+    // later pass, before any later hit of their diagonal.
     // Per hit (state per diagonal, flagged = extended to the right):
     //   pending            -> wait
     //   flagged            -> s < last: skip; else last = s, unflag
@@ -459,120 +479,113 @@ struct WaveScan {
     //   diff < 3           -> skip
     //   otherwise          -> extension (diagonal pending until it ran)
     __device__ __forceinline__ bool decide_pass(bool active, uint32_t q, int32_t so, uint32_t d,
-                                                bool sl)
+                                                int32_t sbase, uint32_t sl)
     {
-        if (++gen == (1u << 26)) {
-            for (int i = lane; i < HB_MARK; i += HB_WAVE) marker[i] = 0;
-            sync();
-            gen = 1;
-        }
-        const uint32_t key = (gen << 6) | (uint32_t)(63 - lane);
-        const uint32_t slot = d & (HB_MARK - 1);
-        if (active) atomicMax(&marker[slot], key);
+        const uint32_t mk = (uint32_t)(HB_WAVE - lane);
+        word_t old = 0;
+        if (active) old = w_max(d, (word_t)mk << W::kMS);
+        const uint32_t ost = (uint32_t)(old & W::kSM);
+        if (active && (uint32_t)(old >> W::kMS) == 0 && ost != 0) w_or(d, (word_t)ost);
         sync();
-        const uint32_t mk = marker[slot];
-        const int32_t st = st_ld(d);
-        const bool win = active && mk == key;
-        const uint32_t ust = (uint32_t)st;
-        const bool pend = (ust & kP) != 0;
-        bool flag;
-        if constexpr (W16) flag = (ust & kF) != 0;
-        else flag = st < 0;
-        const int32_t sb = so + (sl ? s_base1 : s_base0);
-        const int32_t diff = (int32_t)((uint32_t)sb - ust);  // s - last if unflagged
+        word_t wv = 0;
+        if (active) wv = w_ld(d);
+        const bool win = active && (uint32_t)(wv >> W::kMS) == mk;
+        const uint32_t st = (uint32_t)(wv & W::kSM);
+        const bool pend = (st & W::kP) != 0;
+        const bool flag = (st & W::kF) != 0;
+        const int32_t sb = so + sbase;
+        const int32_t diff = sb - (int32_t)st;   // s - last if unflagged and not pending
         const bool job = win && !pend && !flag && diff >= 3 && diff < window;
         const bool wait = win && pend;
-        int32_t nst;
-        if (flag) nst = sb >= (int32_t)(ust & kV) ? sb : st;
-        else nst = diff >= window ? sb : (job ? (int32_t)kP : st);
-        if (win && !pend && nst != st) st_st(d, nst);
-        sync();
-        const bool lose = active && !win;
-        const bool lwait = lose && ((uint32_t)st_ld(d) & kP) != 0;
-        const uint64_t mj = __ballot(job), mb = __ballot(wait || lwait);
-        const uint32_t tso = (uint32_t)so | (sl ? 0x80000000u : 0u);
-        if (job) {
-            const uint32_t delta = (uint32_t)diff;  // 3..window-1
-            jobs[n_jobs + lane_rank(mj)] = make_uint2(q | (delta << 24), tso);
-        }
-        if (wait || lwait) blk[n_blk + lane_rank(mb)] = make_uint2(q, tso);
+        uint32_t nst;
+        if (flag) nst = sb >= (int32_t)(st & W::kV) ? (uint32_t)sb : st;
+        else nst = diff >= window ? (uint32_t)sb : (job ? W::kP : st);
+        if (win) w_st(d, (word_t)nst);
+        const uint64_t mj = __ballot(job), mb = __ballot(wait);
+        const uint32_t tso = (uint32_t)so | (sl << 28);
+        if (job)
+            jobs[(j_head + n_jobs + lane_rank(mj)) & (kJobCap - 1)] =
+                make_uint2(q | ((uint32_t)diff << 24), tso);
+        if (wait) blk[n_blk + lane_rank(mb)] = make_uint2(q, tso);
         n_jobs += (uint32_t)__popcll(mj);
         n_blk += (uint32_t)__popcll(mb);
-        n_ext0 += job && !sl ? 1u : 0u;
-        n_ext1 += job && sl ? 1u : 0u;
         sync();
-        return lose && !lwait;
+        return active && !win;
     }
 
     // Decides a set of hits completely (passes until no hit is left).
-    __device__ __forceinline__ void decide(bool active, uint32_t q, int32_t so, uint32_t d, bool sl)
+    __device__ __forceinline__ void decide(bool active, uint32_t q, int32_t so, uint32_t d,
+                                           int32_t sbase, uint32_t sl)
     {
-        while (__ballot(active)) active = decide_pass(active, q, so, d, sl);
+        while (__ballot(active)) active = decide_pass(active, q, so, d, sbase, sl);
     }
 
-    // Runs every queued extension (one per lane), writes HSPs with score >=
-    // cutoff and the diagonals' new states.
-    // With SL the jobs may belong to either slot (staged subjects); without,
-    // all of them belong to the subject S being scanned (slot cs).
+    // Runs the (up to 64) oldest queued extensions, one per lane, writes HSPs
+    // with score >= cutoff and the diagonals' new states.  SL: the jobs'
+    // subjects are staged in the ring; else all of them belong to the
+    // subject S in global memory (slot 0).
     template <bool SL, typename SP>
-    __device__ __forceinline__ void run_jobs(SP S)
+    __device__ __forceinline__ void run_batch(SP S)
     {
-        for (uint32_t j0 = 0; j0 < n_jobs; j0 += HB_WAVE) {
-            const uint32_t j = j0 + (uint32_t)lane;
-            const bool v = j < n_jobs;
-            ext_result r = {0, 0, 0, 0, 0, 0};
-            int32_t k = 0, q = 0, so = 0;
-            bool sl = false;
-            if (v) {
-                const uint2 jb = jobs[j];
-                q = (int32_t)(jb.x & 0xffffffu);
-                so = (int32_t)(jb.y & 0x7fffffffu);
-                sl = (jb.y >> 31) != 0;
-                const int32_t last_s = so - (int32_t)(jb.x >> 24);
-                const int32_t slen = sl ? s_len1 : s_len0;
-                const int32_t b = sl ? s_base1 : s_base0;
-                k = context_of(ctx, n_ctx, q);
-                if (SL)
-                    r = ext_two_hit<true>(mat, sl ? sbuf1 : sbuf0, slen, qry, qlen, last_s + 3,
-                                          so, q, ctx[ctx_pad + k]);
-                else
-                    r = ext_two_hit<false>(mat, S, slen, qry, qlen, last_s + 3, so, q,
-                                           ctx[ctx_pad + k]);
-                st_st(diag_of((uint32_t)q, so),
-                      r.right ? (int32_t)(kF | (uint32_t)(r.s_last - 2 + b)) : so + b);
-            }
-            const bool emit = v && r.score >= ctx[2 * ctx_pad + k];
-            const uint64_t m = __ballot(emit);
-            if (m) {
-                uint32_t slot0 = 0;
-                if (lane == 0) slot0 = atomicAdd(a->out_count, (uint32_t)__popcll(m));
-                slot0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)slot0);
-                if (emit) {
-                    const uint32_t slot = slot0 + lane_rank(m);
-                    if (slot < a->out_cap) {
-                        hb_hsp o;
-                        o.oid = sl ? s_oid1 : s_oid0;
-                        o.context = ctx_base + k;
-                        o.q_start = r.q_start;
-                        o.s_start = r.s_start;
-                        o.q_off = q;
-                        o.s_off = so;
-                        o.len = r.len;
-                        o.score = r.score;
-                        a->out[slot] = o;
-                    }
-                }
-            }
-            n_hsp0 += emit && !sl ? 1u : 0u;
-            n_hsp1 += emit && sl ? 1u : 0u;
+        const uint32_t nb = min(n_jobs, (uint32_t)HB_WAVE);
+        const bool v = (uint32_t)lane < nb;
+        ext_result r = {0, 0, 0, 0, 0, 0};
+        int32_t k = 0, q = 0, so = 0;
+        uint32_t sl = 0, oid = 0;
+        if (v) {
+            const uint2 jb = jobs[(j_head + (uint32_t)lane) & (kJobCap - 1)];
+            q = (int32_t)(jb.x & 0xffffffu);
+            so = (int32_t)(jb.y & kSoMask);
+            sl = jb.y >> 28;
+            const int32_t last_s = so - (int32_t)(jb.x >> 24);
+            const uint4 m = sla[sl];
+            oid = m.x;
+            k = context_of(ctx, n_ctx, q);
+            if (SL)
+                r = ext_two_hit<true>(mat, ring + m.w, (int32_t)m.y, qry, qlen, last_s + 3, so,
+                                      q, ctx[ctx_pad + k]);
+            else
+                r = ext_two_hit<false>(mat, S, (int32_t)m.y, qry, qlen, last_s + 3, so, q,
+                                       ctx[ctx_pad + k]);
+            const int32_t b = (int32_t)m.z;
+            w_st(diag_of((uint32_t)q, so),
+                 (word_t)(r.right ? (W::kF | (uint32_t)(r.s_last - 2 + b)) : (uint32_t)(so + b)));
+            atomicAdd(&slb[sl].y, 1u);
         }
-        n_jobs = 0;
+        const bool emit = v && r.score >= ctx[2 * ctx_pad + k];
+        const uint64_t m = __ballot(emit);
+        if (m) {
+            uint32_t slot0 = 0;
+            if (lane == 0) slot0 = atomicAdd(a->out_count, (uint32_t)__popcll(m));
+            slot0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)slot0);
+            if (emit) {
+                const uint32_t slot = slot0 + lane_rank(m);
+                if (slot < a->out_cap) {
+                    hb_hsp o;
+                    o.oid = oid;
+                    o.context = ctx_base + k;
+                    o.q_start = r.q_start;
+                    o.s_start = r.s_start;
+                    o.q_off = q;
+                    o.s_off = so;
+                    o.len = r.len;
+                    o.score = r.score;
+                    a->out[slot] = o;
+                }
+                atomicAdd(&slb[sl].z, 1u);
+            }
+        }
+        j_head = (j_head + nb) & (kJobCap - 1);
+        n_jobs -= nb;
         sync();
     }
 
     // Re-decides the blocked hits in their original order (in place: each
     // round reads its 64 entries before any new entry is written, and new
-    // entries never outnumber the ones read).
+    // entries never outnumber the ones read).  Every blocked hit waits on a
+    // pending diagonal; only the diagonals of the batch just run are no
+    // longer pending, so a replay queues at most as many extensions as that
+    // batch ran.
     __device__ __forceinline__ void replay()
     {
         const uint32_t nb = n_blk;
@@ -581,47 +594,60 @@ struct WaveScan {
             const uint32_t b = b0 + (uint32_t)lane;
             const bool v = b < nb;
             const uint2 e = v ? blk[b] : make_uint2(0, 0);
+            const int32_t so = (int32_t)(e.y & kSoMask);
+            const uint32_t sl = e.y >> 28;
+            const int32_t sbase = v ? (int32_t)sla[sl].z : 0;
             sync();
-            const int32_t so = (int32_t)(e.y & 0x7fffffffu);
-            decide(v, e.x, so, diag_of(e.x, so), (e.y >> 31) != 0);
+            decide(v, e.x, so, diag_of(e.x, so), sbase, sl);
+        }
+    }
+
+    // Runs extensions until fewer than 64 are queued and the blocked list
+    // has room for another round.
+    template <bool SL, typename SP>
+    __device__ __forceinline__ void drain_some(SP S)
+    {
+        while (n_jobs >= (uint32_t)HB_WAVE || (n_blk > kBlkCap - HB_WAVE && n_jobs)) {
+            run_batch<SL>(S);
+            replay();
         }
     }
 
     // Runs every queued extension and replays the blocked hits until none is
-    // left; a deferred subject is then complete.
+    // left; every staged subject is then complete and is retired.
     template <bool SL, typename SP>
-    __device__ __forceinline__ void drain(SP S)
+    __device__ __forceinline__ void finish(SP S)
     {
         while (n_jobs) {
-            run_jobs<SL>(S);
+            run_batch<SL>(S);
             replay();
         }
-        if (ds >= 0) {
-            retire(ds);
-            ds = -1;
-        }
+        retire_all();
     }
 
-    // Writes a finished subject's counters.
-    __device__ __forceinline__ void retire(int sl)
+    // Writes the counters of the subjects in slots 0 .. n_live - 1 and
+    // empties the ring.
+    __device__ __forceinline__ void retire_all()
     {
-        const uint32_t e = wave_sum(sl ? n_ext1 : n_ext0), hs = wave_sum(sl ? n_hsp1 : n_hsp0);
-        const uint32_t oid = sl ? s_oid1 : s_oid0, hits = sl ? s_hits1 : s_hits0;
-        if (a->stats && lane == 0) {
-            a->stats[3 * (size_t)oid + 0] = hits;
-            a->stats[3 * (size_t)oid + 1] = e;
-            a->stats[3 * (size_t)oid + 2] = hs;
+        if (n_live == 0) return;
+        uint32_t h = 0, e = 0, hs = 0;
+        if ((uint32_t)lane < n_live) {
+            const uint4 A = sla[lane], B = slb[lane];
+            h = B.x;
+            e = B.y;
+            hs = B.z;
+            if (a->stats) {
+                a->stats[3 * (size_t)A.x + 0] = h;
+                a->stats[3 * (size_t)A.x + 1] = e;
+                a->stats[3 * (size_t)A.x + 2] = hs;
+            }
         }
-        tot_hits += hits;
-        tot_ext += e;
-        tot_hsp += hs;
-    }
-
-    // Completes a deferred subject (before its slot, the diagonal state or
-    // the group tables change).
-    __device__ __forceinline__ void finish_deferred()
-    {
-        if (ds >= 0) drain<true>(sbuf0);
+        tot_hits += wave_sum(h);
+        tot_ext += wave_sum(e);
+        tot_hsp += wave_sum(hs);
+        n_live = 0;
+        ring_head = 0;
+        sync();
     }
 
     // Moves the hits that lost a pass (lost = true) to lanes 0..n-1 in lane
@@ -643,29 +669,16 @@ struct WaveScan {
         return n;
     }
 
-    // Scans one subject.  Hits are taken in (s, q) order, up to 64 per round;
-    // a hit that lost its diagonal to a lower lane in a round is carried to
-    // the front lanes of the next round (ahead of every newer hit, so each
-    // diagonal still sees its hits in subject order), and a round makes a
-    // single decision pass.
+    // Scans one subject (slot sl, diagonal offset sbase).  Hits are taken in
+    // (s, q) order, up to 64 per round; a hit that lost its diagonal to a
+    // lower lane in a round is carried to the front lanes of the next round
+    // (ahead of every newer hit, so each diagonal still sees its hits in
+    // subject order), and a round makes a single decision pass.  Returns the
+    // subject's lookup hits.
     template <bool SL, typename SP>
-    __device__ __forceinline__ void subject(SP S, int32_t slen, uint32_t oid)
+    __device__ __forceinline__ uint32_t subject(SP S, int32_t slen, uint32_t sl, int32_t sbase)
     {
         uint32_t hits = 0;
-        const bool csl = cs != 0;
-        if (csl) {
-            s_oid1 = oid;
-            s_len1 = slen;
-            s_base1 = base;
-            n_ext1 = 0;
-            n_hsp1 = 0;
-        } else {
-            s_oid0 = oid;
-            s_len0 = slen;
-            s_base0 = base;
-            n_ext0 = 0;
-            n_hsp0 = 0;
-        }
         int32_t s0m = 0;  // s0 mod D
         uint32_t c = 0;   // carried hits, in lanes 0..c-1
         uint32_t cq = 0, cd = 0;
@@ -689,7 +702,7 @@ struct WaveScan {
             hits += H;
             uint32_t carry = 0;
             for (uint32_t r0 = 0; r0 < H;) {
-                if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain<SL>(S);
+                drain_some<SL>(S);
                 // new hits r0 .. r0 + take - 1 go to lanes c .. c + take - 1
                 const uint32_t take = min((uint32_t)HB_WAVE - c, H - r0);
                 if (++rgen == (1u << 25)) {
@@ -717,7 +730,7 @@ struct WaveScan {
                 const int32_t so = isc ? cs_ : s0 + (int32_t)p;
                 const uint32_t d = isc ? cd : (uint32_t)dn;
                 sync();
-                const bool lost = decide_pass(isc || isnew, q, so, d, csl);
+                const bool lost = decide_pass(isc || isnew, q, so, d, sbase, sl);
                 c = carry_losers(lost, q, so, d, cq, cs_, cd);
                 r0 += take;
             }
@@ -725,49 +738,59 @@ struct WaveScan {
             if (s0m >= D) s0m -= D;
         }
         while (c) {   // hits still carried at the end of the subject
-            if (n_jobs >= HB_WAVE || n_blk > (uint32_t)(kBlkCap - HB_WAVE)) drain<SL>(S);
+            drain_some<SL>(S);
             const bool isc = (uint32_t)lane < c;
-            const bool lost = decide_pass(isc, cq, cs_, cd, csl);
+            const bool lost = decide_pass(isc, cq, cs_, cd, sbase, sl);
             c = carry_losers(lost, cq, cs_, cd, cq, cs_, cd);
         }
-        if (csl) s_hits1 = hits;
-        else s_hits0 = hits;
-        bool reset;
-        if constexpr (W16) reset = base + slen + window >= a->state_limit;
-        else reset = base + slen + window >= (1 << 29);
-        if (SL && ds < 0 && n_jobs < kDeferJobs && !reset) {
-            // few extensions queued: they run with the next subject's
-            ds = cs;
-            cs ^= 1;
-        } else {
-            drain<SL>(S);   // also completes a deferred subject
-            retire(cs);
-        }
-        if (reset) {
-            for (int i = lane; i < D; i += HB_WAVE) st_st((uint32_t)i, 0);
-            sync();
-            base = window;
-        } else {
-            base += slen + window;
-        }
+        return hits;
     }
 
-    __device__ __forceinline__ void scan_subject(uint32_t it)
+    // Scans the subject of processing-order record m.  A subject that fits
+    // the ring is staged there (16-byte loads and stores) and its queued
+    // extensions wait until 64 are queued or the ring or the slots run out,
+    // so that extension batches stay full; a longer one is scanned from
+    // global memory after every staged subject is complete.
+    __device__ __forceinline__ void scan_subject(const uint4 m)
     {
-        const uint4 m = a->subj[it];
         const uint32_t oid = m.x;
         const int32_t slen = (int32_t)m.y;
-        const uint64_t sb = (uint64_t)m.z | ((uint64_t)m.w << 32);
-        const uint8_t *S = a->res + sb;
-        if ((uint32_t)slen <= a->scap) {
-            uint8_t *B = cs ? sbuf1 : sbuf0;
-            for (int i = lane; i < slen; i += HB_WAVE) B[i] = S[i];
+        const uint8_t *S = a->res + ((uint64_t)m.z | ((uint64_t)m.w << 32));
+        const uint32_t span = ((uint32_t)slen + 15u) & ~15u;
+        if ((uint32_t)slen <= a->ring) {
+            if (n_live == HB_SLOTS || ring_head + span > a->ring) finish<true>(ring);
+            const uint32_t sl = n_live, buf = ring_head;
+            const uint4 *src = reinterpret_cast<const uint4 *>(S);
+            uint4 *dst = reinterpret_cast<uint4 *>(ring + buf);
+            for (uint32_t i = (uint32_t)lane; i < span / 16; i += HB_WAVE) dst[i] = src[i];
+            if (lane == 0) {
+                sla[sl] = make_uint4(oid, (uint32_t)slen, (uint32_t)base, buf);
+                slb[sl] = make_uint4(0, 0, 0, 0);
+            }
             sync();
-            subject<true>(B, slen, oid);
+            ring_head += span;
+            n_live += 1;
+            const uint32_t hits = subject<true>(ring + buf, slen, sl, base);
+            if (lane == 0) slb[sl].x = hits;
             sync();
         } else {
-            finish_deferred();
-            subject<false>(S, slen, oid);
+            finish<true>(ring);
+            if (lane == 0) {
+                sla[0] = make_uint4(oid, (uint32_t)slen, (uint32_t)base, 0);
+                slb[0] = make_uint4(0, 0, 0, 0);
+            }
+            sync();
+            n_live = 1;
+            const uint32_t hits = subject<false>(S, slen, 0, base);
+            if (lane == 0) slb[0].x = hits;
+            sync();
+            finish<false>(S);
+        }
+        if (base + slen + window >= a->state_limit) {
+            finish<true>(ring);
+            clear_words();
+        } else {
+            base += slen + window;
         }
     }
 
@@ -795,8 +818,8 @@ __device__ __forceinline__ void load_group(const hb_scan_args &a, const hb_group
     }
 }
 
-template <bool GS, bool W16>
-__device__ __forceinline__ void bind_group(WaveScan<GS, W16> &w, const hb_group_desc &G)
+template <bool GS>
+__device__ __forceinline__ void bind_group(WaveScan<GS> &w, const hb_group_desc &G)
 {
     w.cells = (const HB_GLOBAL uint64_t *)G.cells;
     w.entries = (const HB_GLOBAL uint32_t *)G.entries;
@@ -805,6 +828,7 @@ __device__ __forceinline__ void bind_group(WaveScan<GS, W16> &w, const hb_group_
     w.n_ctx = G.n_ctx;
     w.ctx_base = G.ctx_base;
     w.D = G.diag_len;
+    w.dmul = (uint32_t)(0xffffffffu / (uint32_t)G.diag_len) + 1u;
 }
 
 // MULTI = false: one group; each wavefront pops subject tiles on its own.
@@ -814,12 +838,11 @@ __device__ __forceinline__ void bind_group(WaveScan<GS, W16> &w, const hb_group_
 // Register budget for five wavefronts per SIMD (no spills at this limit):
 // the scan is latency-bound, and short queries or small groups leave LDS
 // for that many.
-// GSTATE: the diagonal state lives in global memory (a.gstate, diag_len_max
-// int32 per wavefront of the grid) instead of LDS, for queries (groups)
-// whose state does not fit LDS or would leave it too few wavefronts.
-// W16: 16-bit state in LDS (databases whose subjects are short enough for
-// 14-bit positions between resets).
-template <bool MULTI, bool GSTATE, bool W16>
+// GSTATE: the diagonal words live in global memory (a.gstate, diag_len_max
+// per wavefront of the grid) instead of LDS, for queries (groups) whose
+// state does not fit LDS or would leave it too few wavefronts, and for
+// databases whose subjects are too long for the LDS word's positions.
+template <bool MULTI, bool GSTATE>
 __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void hb_scan_kernel(hb_scan_args a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -835,34 +858,32 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     for (int i = threadIdx.x; i < 256; i += blockDim.x)
         reinterpret_cast<uint32_t *>(sm_mat)[i] = reinterpret_cast<const uint32_t *>(a.mat32)[i];
 
-    WaveScan<GSTATE, W16> w;
+    WaveScan<GSTATE> w;
     w.a = &a;
     w.mat = sm_mat;
     w.qry = sm_q;
     w.ctx = sm_ctx;
     w.ctx_pad = a.ctx_pad_max;
     if constexpr (GSTATE) {
-        w.state = a.gstate + ((size_t)blockIdx.x * (blockDim.x / HB_WAVE) + (size_t)wid) * (size_t)Dm;
-        w.marker = reinterpret_cast<uint32_t *>(wb);
-    } else if constexpr (W16) {
-        w.state16 = reinterpret_cast<uint16_t *>(wb);
-        w.marker = reinterpret_cast<uint32_t *>(wb + 2 * (size_t)Dm);   // Dm: multiple of 64
+        w.wd = a.gstate + ((size_t)blockIdx.x * (blockDim.x / HB_WAVE) + (size_t)wid) * (size_t)Dm;
+        w.seg = reinterpret_cast<uint32_t *>(wb);
     } else {
-        w.state = reinterpret_cast<int32_t *>(wb);
-        w.marker = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)Dm);
+        w.wd = reinterpret_cast<uint32_t *>(wb);
+        w.seg = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)Dm);   // Dm: multiple of 64
     }
-    w.seg = w.marker + HB_MARK;
     w.jobs = reinterpret_cast<uint2 *>(w.seg + HB_WAVE);
     w.blk = w.jobs + kJobCap;
-    w.sbuf0 = reinterpret_cast<uint8_t *>(w.blk + kBlkCap);
-    w.sbuf1 = w.sbuf0 + a.scap;
-    w.cs = 0;
-    w.ds = -1;
+    w.sla = reinterpret_cast<uint4 *>(w.blk + kBlkCap);
+    w.slb = w.sla + HB_SLOTS;
+    w.ring = reinterpret_cast<uint8_t *>(w.slb + HB_SLOTS);
     w.lane = lane;
     w.window = a.window;
     w.rgen = 0;
+    w.j_head = 0;
     w.n_jobs = 0;
     w.n_blk = 0;
+    w.n_live = 0;
+    w.ring_head = 0;
     w.tot_hits = w.tot_ext = w.tot_hsp = 0;
     w.seg[lane] = 0;
     const uint32_t n_tiles = a.tile_end - a.tile_begin;
@@ -871,7 +892,7 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
         const hb_group_desc G = a.groups[0];
         load_group(a, G, sm_ctx, sm_q);
         bind_group(w, G);
-        w.reset();
+        w.clear_words();
         __syncthreads();
         for (;;) {
             uint32_t t = 0;
@@ -880,9 +901,9 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
             if (t >= n_tiles) break;
             t += a.tile_begin;
             const uint32_t i_end = a.tiles[t + 1];
-            for (uint32_t it = a.tiles[t]; it < i_end; ++it) w.scan_subject(it);
+            for (uint32_t it = a.tiles[t]; it < i_end; ++it) w.scan_subject(a.subj[it]);
         }
-        w.finish_deferred();
+        w.template finish<true>(w.ring);
         w.flush_totals(0);
     } else {
         uint32_t cur = 0xffffffffu;
@@ -899,14 +920,14 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
             const uint32_t t1 = min(t0 + tpi, a.tile_end);
             if (g != cur) {
                 if (cur != 0xffffffffu) {
-                    w.finish_deferred();   // still with the old group's tables
+                    w.template finish<true>(w.ring);   // still with the old group's tables
                     w.flush_totals(cur);
                 }
                 __syncthreads();
                 const hb_group_desc G = a.groups[g];
                 load_group(a, G, sm_ctx, sm_q);
                 bind_group(w, G);
-                w.reset();
+                w.clear_words();
                 cur = g;
                 __syncthreads();
             }
@@ -921,12 +942,12 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
                 if (lane == 0) it = atomicAdd(&s_next, 1u);
                 it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
                 if (it >= i_end) break;
-                w.scan_subject(it);
+                w.scan_subject(a.subj[it]);
             }
             __syncthreads();
         }
         if (cur != 0xffffffffu) {
-            w.finish_deferred();
+            w.template finish<true>(w.ring);
             w.flush_totals(cur);
         }
     }
@@ -1208,17 +1229,14 @@ hipError_t hb_launch_lut(const hb_lut_job *jobs_dev, int n_jobs, int32_t qlen_ma
     return hipGetLastError();
 }
 
-// Instantiations: (MULTI, GSTATE, W16) with W16 only for LDS state.
-static const void *scan_fn(bool multi, bool gstate, bool w16)
+// Instantiations: (MULTI, GSTATE).
+static const void *scan_fn(bool multi, bool gstate)
 {
     if (gstate)
-        return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, true, false>)
-                     : reinterpret_cast<const void *>(hb_scan_kernel<false, true, false>);
-    if (w16)
-        return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, false, true>)
-                     : reinterpret_cast<const void *>(hb_scan_kernel<false, false, true>);
-    return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, false, false>)
-                 : reinterpret_cast<const void *>(hb_scan_kernel<false, false, false>);
+        return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, true>)
+                     : reinterpret_cast<const void *>(hb_scan_kernel<false, true>);
+    return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, false>)
+                 : reinterpret_cast<const void *>(hb_scan_kernel<false, false>);
 }
 
 hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int block_threads,
@@ -1226,19 +1244,16 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 {
     if (block_threads < HB_WAVE || block_threads > HB_BLOCK || block_threads % HB_WAVE)
         return hipErrorInvalidValue;
-    const bool g = a.gstate != nullptr, w16 = a.state_limit > 0;
-    if (g && w16) return hipErrorInvalidValue;
+    if (a.ring % 16 || a.wave_lds % 16 || a.state_limit <= 0) return hipErrorInvalidValue;
+    const bool g = a.gstate != nullptr;
     const dim3 grid(n_blocks), block(block_threads);
-#define HB_SCAN_LAUNCH(M, GS, W)                                                     \
-    hipLaunchKernelGGL((hb_scan_kernel<M, GS, W>), grid, block, lds_bytes, st, a)
+#define HB_SCAN_LAUNCH(M, GS) hipLaunchKernelGGL((hb_scan_kernel<M, GS>), grid, block, lds_bytes, st, a)
     if (multi) {
-        if (g) HB_SCAN_LAUNCH(true, true, false);
-        else if (w16) HB_SCAN_LAUNCH(true, false, true);
-        else HB_SCAN_LAUNCH(true, false, false);
+        if (g) HB_SCAN_LAUNCH(true, true);
+        else HB_SCAN_LAUNCH(true, false);
     } else {
-        if (g) HB_SCAN_LAUNCH(false, true, false);
-        else if (w16) HB_SCAN_LAUNCH(false, false, true);
-        else HB_SCAN_LAUNCH(false, false, false);
+        if (g) HB_SCAN_LAUNCH(false, true);
+        else HB_SCAN_LAUNCH(false, false);
     }
 #undef HB_SCAN_LAUNCH
     return hipGetLastError();
@@ -1246,8 +1261,8 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
 
 hipError_t hb_scan_allow_lds(size_t max_lds)
 {
-    for (int k = 0; k < 6; ++k) {
-        const void *fn = scan_fn((k & 1) != 0, k / 2 == 1, k / 2 == 2);
+    for (int k = 0; k < 4; ++k) {
+        const void *fn = scan_fn((k & 1) != 0, (k & 2) != 0);
         hipFuncAttributes fa;
         hipError_t e = hipFuncGetAttributes(&fa, fn);
         if (e != hipSuccess) return e;
@@ -1259,9 +1274,9 @@ hipError_t hb_scan_allow_lds(size_t max_lds)
     return hipSuccess;
 }
 
-hipError_t hb_scan_occupancy(bool multi, bool gstate, bool w16, size_t lds_bytes,
-                             int block_threads, int *blocks_per_cu)
+hipError_t hb_scan_occupancy(bool multi, bool gstate, size_t lds_bytes, int block_threads,
+                             int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_fn(multi, gstate, w16),
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_fn(multi, gstate),
                                                         block_threads, lds_bytes);
 }

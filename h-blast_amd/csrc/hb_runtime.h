@@ -53,7 +53,9 @@ struct hb_db {
     uint8_t *d_res = nullptr;
     uint64_t *d_offs = nullptr;
     uint4 *d_subj = nullptr;     // {oid, length, offset into d_scan_res} in processing order
-    uint8_t *d_scan_res = nullptr;  // residues in processing order: the scan's copy
+    uint8_t *d_scan_res = nullptr;  // residues in processing order: the scan's copy (each
+                                    // subject 16-byte aligned, spans padded to 16 bytes)
+    uint64_t scan_bytes = 0;        // bytes of that copy's subject spans
     uint32_t *d_tiles = nullptr;
     uint32_t n_tiles = 0;
     std::vector<uint64_t> h_offs;
@@ -104,37 +106,44 @@ struct hb_launch_cfg {
     int waves = 4;
     size_t lds = 0;
     int blocks = 0;
-    int32_t scap = 1024;   // staged-subject capacity per wave
+    int32_t ring = 1024;   // staging ring bytes per wave
 };
-// Where a launch keeps its per-diagonal two-hit state.
-enum hb_state_place { HB_PLACE_LDS32 = 0, HB_PLACE_LDS16 = 1, HB_PLACE_GLOBAL = 2 };
-static inline int hb_state_bytes(int place)
+// Where a launch keeps its per-diagonal two-hit state: 32-bit words in LDS,
+// or 64-bit words in a global-memory region per wavefront.
+enum hb_state_place { HB_PLACE_LDS = 0, HB_PLACE_GLOBAL = 2 };
+static inline int hb_state_bytes(int place) { return place == HB_PLACE_LDS ? 4 : 0; }
+// LDS words hold 23-bit positions between resets, so a subject and its
+// successor must fit: 2 * (longest subject + window) + 64 < 2^23.
+static inline bool hb_state_lds_ok(uint64_t max_len, int32_t window)
 {
-    return place == HB_PLACE_LDS32 ? 4 : place == HB_PLACE_LDS16 ? 2 : 0;
+    return 2 * (max_len + (uint64_t)window) + 64 < (uint64_t)HB_LDS_POS_LIMIT;
 }
-// 16-bit state: positions of 14 bits between resets, so a subject and its
-// successor must fit: 2 * (longest subject + window) < 16384.
-static inline bool hb_state16_ok(uint64_t max_len, int32_t window)
+// The reset threshold of the state (hb_scan_args::state_limit).
+static inline int32_t hb_state_limit(int place, uint64_t max_len)
 {
-    return 2 * (max_len + (uint64_t)window) + 64 < 16384;
+    return place == HB_PLACE_LDS ? HB_LDS_POS_LIMIT - (int32_t)max_len : HB_GLOBAL_POS_LIMIT;
 }
-// The reset threshold of a 16-bit state (hb_scan_args::state_limit).
-static inline int32_t hb_state16_limit(uint64_t max_len) { return 16384 - (int32_t)max_len; }
 // Automatic placement for diagonal arrays of D entries and `fixed` bytes of
-// per-workgroup LDS on a database whose longest subject is max_len: 16-bit
-// LDS state when the subjects allow it and it leaves at least 3 wavefronts
-// per CU, else 32-bit LDS state when that leaves at least 6, else global
+// per-workgroup LDS on a database whose longest subject is max_len: LDS when
+// the subjects allow it and it leaves enough wavefronts per CU, else global
 // memory.
 int hb_state_choose(const hb_ctx *c, int32_t D, size_t fixed, uint64_t max_len, int32_t window);
+// The scan keeps a concatenated query, its context tables and the score
+// matrix in LDS (with the diagonal state in global memory at the least), and
+// the lookup-table build keeps the query in LDS too: queries past about
+// 150,000 concatenated residues do not fit a workgroup's 160 KB.
+bool hb_query_lds_ok(const hb_ctx *c, int32_t qlen, int32_t n_ctx);
 // Global-memory diagonal state of one launch is capped at this size: a
 // launch for a very long query then runs fewer workgroups.
 constexpr size_t HB_GSTATE_MAX_BYTES = size_t(1) << 30;
+// Bytes per diagonal of the global-memory state.
+constexpr size_t HB_GSTATE_WORD = 8;
 // Bytes of that state for diagonal arrays of D entries (upper bound).
 size_t hb_gstate_bytes_max(const hb_ctx *c, int32_t D);
 // place: where the diagonal state goes (LDS bytes per diagonal follow).
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg,
-                   int place = HB_PLACE_LDS32);
+                   int place = HB_PLACE_LDS);
 
 // Asynchronous steps of a batch (hb_batch.cpp), used by the serving loop:
 // the batch runs on the given stream (nullptr: the context's); launch

@@ -302,36 +302,35 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
     return HB_OK;
 }
 
-// Picks wavefronts per workgroup and the staged-subject capacity: the most
+// Picks wavefronts per workgroup and the staging ring size: the most
 // wavefronts per CU up to 16 first (measured on C1: 8 -> 12 -> 16 wavefronts
 // per CU gave 27.7 -> 20.6 -> 16.8 ms; beyond 16 the kernel is bound by
-// VALU issue and more wavefronts gained nothing), then the largest staging
-// buffer (subjects above it read their residues from global memory; 18
-// wavefronts with 320-byte staging ran 8% slower than 16 with 1024, and
-// with the 16-bit diagonal state 2048 instead of 1024 took C1 from 15.17 to
-// 15.02 ms), then more wavefronts per workgroup (they share the query
-// tables).  The batched form stages at most 1024 residues: 2048 cost its
-// largest class of 256 C3 queries a fifth workgroup per CU (1214 -> 1373 ms).
+// VALU issue and more wavefronts gained nothing), then the largest ring
+// (subjects above it read their residues from global memory, and a larger
+// ring holds the extensions of more subjects before they must run), then
+// more wavefronts per workgroup (they share the query tables).  The batched
+// form stages at most 2048 residues per wave: its largest class of C3
+// queries lost a workgroup per CU to larger staging.
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg, int place)
 {
     const size_t fixed = hb_scan_fixed_lds(qlen_pad_max, ctx_pad_max);
-    const bool gstate = place == HB_PLACE_GLOBAL, w16 = place == HB_PLACE_LDS16;
+    const bool gstate = place == HB_PLACE_GLOBAL;
     const int sb = hb_state_bytes(place);
     int best_wpc = 0;
     for (int waves : {4, 2, 1}) {
-        for (int32_t scap = multi ? std::min(HB_SCAP_MAX, 1024) : HB_SCAP_MAX; scap >= 256;
-             scap -= 64) {
-            const size_t lds = fixed + waves * hb_scan_wave_lds(diag_len_max, scap, sb);
+        for (int32_t ring = multi ? std::min(HB_RING_MAX, 2048) : HB_RING_MAX; ring >= 256;
+             ring -= 64) {
+            const size_t lds = fixed + waves * hb_scan_wave_lds(diag_len_max, ring, sb);
             if (lds > c->max_lds) continue;
             int per_cu = 0;
-            HB_CHECK(hb_scan_occupancy(multi, gstate, w16, lds, waves * HB_WAVE, &per_cu));
+            HB_CHECK(hb_scan_occupancy(multi, gstate, lds, waves * HB_WAVE, &per_cu));
             const int wpc = std::min(per_cu * waves, 16);
-            if (wpc > best_wpc || (wpc == best_wpc && scap > cfg->scap)) {
+            if (wpc > best_wpc || (wpc == best_wpc && ring > cfg->ring)) {
                 best_wpc = wpc;
                 cfg->waves = waves;
                 cfg->lds = lds;
-                cfg->scap = scap;
+                cfg->ring = ring;
                 cfg->blocks = per_cu;
             }
         }
@@ -339,7 +338,7 @@ int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_
     if (best_wpc == 0) return HB_ERR_UNSUPPORTED;
     int blocks = c->n_cu * std::max(cfg->blocks, 1);
     if (gstate) {   // keep the global-memory state within HB_GSTATE_MAX_BYTES
-        const size_t per_block = 4 * (size_t)diag_len_max * (size_t)cfg->waves;
+        const size_t per_block = HB_GSTATE_WORD * (size_t)diag_len_max * (size_t)cfg->waves;
         blocks = std::min(blocks, (int)std::max<size_t>(1, HB_GSTATE_MAX_BYTES / per_block));
     }
     const uint32_t per_block = multi ? 1u : (uint32_t)cfg->waves;
@@ -349,14 +348,9 @@ int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_
     return HB_OK;
 }
 
-// Automatic placement of the diagonal state.  Measured on MI355X, 200,000
-// subjects, plain BLASTP, ms with 32-bit LDS / 16-bit LDS / global state:
-// 2,000 residues 4.05 / 3.28 / 4.87; 4,000 10.7 / 8.34 / 13.2;
-// 8,000 29.6 / 19.6 / 29.7; 16,000 102 / 59.3 / 76.0 (16-bit: 3 wavefronts
-// per CU); 24,000 284 / 147 / 128 (16-bit: 2).  32-bit LDS state beats
-// global memory down to about 6 wavefronts per CU, 16-bit down to 3.
-static constexpr int kGstateMinWaves = 6;
-static constexpr int kLds16MinWaves = 3;
+// Automatic placement of the diagonal state.  LDS words beat the
+// global-memory state down to about kGstateMinWaves wavefronts per CU.
+static constexpr int kGstateMinWaves = 4;
 static int lds_waves(const hb_ctx *c, int32_t D, size_t fixed, int state_bytes)
 {
     const size_t per_wave = hb_scan_wave_lds(D, 256, state_bytes);
@@ -367,18 +361,22 @@ static int lds_waves(const hb_ctx *c, int32_t D, size_t fixed, int state_bytes)
 
 int hb_state_choose(const hb_ctx *c, int32_t D, size_t fixed, uint64_t max_len, int32_t window)
 {
-    // 16-bit entries whenever the subjects allow them: never fewer wavefronts
-    // than 32-bit ones, and the C1 scan ran 0.7% faster with them even at
-    // the same 16 wavefronts per CU (15.17 vs 15.28 ms)
-    if (hb_state16_ok(max_len, window) && lds_waves(c, D, fixed, 2) >= kLds16MinWaves)
-        return HB_PLACE_LDS16;
-    return lds_waves(c, D, fixed, 4) >= kGstateMinWaves ? HB_PLACE_LDS32 : HB_PLACE_GLOBAL;
+    if (hb_state_lds_ok(max_len, window) && lds_waves(c, D, fixed, 4) >= kGstateMinWaves)
+        return HB_PLACE_LDS;
+    return HB_PLACE_GLOBAL;
+}
+
+bool hb_query_lds_ok(const hb_ctx *c, int32_t qlen, int32_t n_ctx)
+{
+    const size_t qpad = (size_t)((qlen + 15) & ~15);
+    const size_t fixed = hb_scan_fixed_lds((int32_t)qpad, hb_ctx_pad(n_ctx));
+    return fixed + hb_scan_wave_lds(0, 256, 0) <= c->max_lds && 1024 + qpad <= c->max_lds;
 }
 
 size_t hb_gstate_bytes_max(const hb_ctx *c, int32_t D)
 {
-    const size_t full = (size_t)c->n_cu * 16 * 4 * (size_t)D;   // 16 wavefronts per CU
-    return std::min(full, std::max(HB_GSTATE_MAX_BYTES, 4 * 4 * (size_t)D));
+    const size_t full = (size_t)c->n_cu * 16 * HB_GSTATE_WORD * (size_t)D;   // 16 wavefronts per CU
+    return std::min(full, std::max(HB_GSTATE_MAX_BYTES, 4 * HB_GSTATE_WORD * (size_t)D));
 }
 
 struct hb_query {
@@ -388,8 +386,8 @@ struct hb_query {
     uint32_t n_cells_used = 0;
     std::vector<int32_t> ctx_off, ctx_len;   // host copies (gapped stage)
     int mode = HB_STATE_AUTO;   // requested placement of the diagonal state
-    bool lds_fits = true;       // the LDS layout can hold a 32-bit state
-    bool lds16_fits = true;     // ... a 16-bit state
+    bool lds_fits = true;       // the LDS layout can hold the diagonal words
+    int32_t reset_interval = 0; // hb_query_set_reset_interval (0: the placement's limit)
 };
 
 struct hb_scan {
@@ -405,9 +403,9 @@ struct hb_scan {
     uint32_t n_out = 0;
     float gap_ms = 0.f;        // device time of the last gapped stage
     uint32_t gap_wide = 0;     // subjects it ran in the wide pass
-    int32_t *d_gstate = nullptr;   // per-wavefront diagonal state (long queries)
+    unsigned long long *d_gstate = nullptr;   // per-wavefront diagonal words (long queries)
     size_t gstate_bytes = 0;
-    int place = HB_PLACE_LDS32;    // where this scan keeps the diagonal state
+    int place = HB_PLACE_LDS;      // where this scan keeps the diagonal state
 };
 
 extern "C" {
@@ -585,7 +583,7 @@ static int db_meta(hb_db *db)
     for (uint32_t i = 0; i < n; ++i) {
         const uint64_t len = db->h_offs[order[i] + 1] - db->h_offs[order[i]];
         subj[i] = make_uint4(order[i], (uint32_t)len, (uint32_t)dst, (uint32_t)(dst >> 32));
-        dst += len;
+        dst += (len + 15) & ~15ull;   // the scan stages subjects with 16-byte loads
         acc += len;
         ++cnt;
         if (cnt >= 64 || acc >= 16384) {
@@ -596,6 +594,7 @@ static int db_meta(hb_db *db)
     }
     if (tiles.back() != n) tiles.push_back(n);
     db->n_tiles = (uint32_t)tiles.size() - 1;
+    db->scan_bytes = dst;
     if (hipMalloc(&db->d_offs, ((size_t)n + 1) * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc(&db->d_subj, (size_t)std::max<uint32_t>(n, 1) * sizeof(uint4)) != hipSuccess ||
         hipMalloc(&db->d_tiles, tiles.size() * sizeof(uint32_t)) != hipSuccess)
@@ -615,8 +614,10 @@ static int db_meta(hb_db *db)
 static int db_scan_copy(hb_db *db)
 {
     hb_ctx *c = db->ctx;
-    if (hipMalloc(&db->d_scan_res, db->total + 64) != hipSuccess) return HB_ERR_NOMEM;
-    if (hipMemsetAsync(db->d_scan_res + db->total, 0, 64, c->stream) != hipSuccess ||
+    // padding bytes between subject spans read as 0 (the staged copies hold
+    // them past each subject's end)
+    if (hipMalloc(&db->d_scan_res, db->scan_bytes + 64) != hipSuccess) return HB_ERR_NOMEM;
+    if (hipMemsetAsync(db->d_scan_res, 0, db->scan_bytes + 64, c->stream) != hipSuccess ||
         hb_launch_gather_subjects(db->d_res, db->d_offs, db->d_subj, db->d_tiles, db->n_tiles,
                                   db->d_scan_res, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
@@ -780,9 +781,18 @@ int hb_db_upload_hdb(hb_ctx *c, const uint8_t *hdb, uint64_t bytes, int32_t bloc
     hipEventElapsedTime(&db->hdb_len_ms, ev[0], ev[1]);
     hipEventElapsedTime(&db->hdb_copy_ms, ev[2], ev[3]);
     if (bad) return fail(HB_ERR_RESIDUE);   // checked by the copy kernel
+    // the volume's bytes are no longer needed: free them before the scan's
+    // copy is allocated, so the upload holds at most the file plus the
+    // residues, then the residues twice
+    release();
+    d_hdb = nullptr;
+    d_gbase = nullptr;
+    d_glen = nullptr;
+    d_lens = nullptr;
+    d_bad = nullptr;
+    for (hipEvent_t &e : ev) e = nullptr;
     const int rc2 = db_scan_copy(db);
     if (rc2 != HB_OK) return fail(rc2);
-    release();
     *out = db;
     return HB_OK;
 }
@@ -860,15 +870,16 @@ int hb_query_prepare_masked(hb_ctx *c, const uint8_t *seq, int32_t len, int32_t 
     if (lut_mask) qs[0].mask.assign(lut_mask, lut_mask + len);
     // The diagonal state (4 bytes per diagonal) stays in LDS when it fits;
     // longer queries (the reference accepts 32,768 residues,
-    // H-BLAST/h_blast.cu:1293) keep it in global memory.
+    // H-BLAST/h_blast.cu:1293) keep it in global memory.  The query itself,
+    // its context tables and the matrix are always in LDS, and so is the
+    // lookup-table build's copy of the query.
     const int32_t D = (len + q->p.window + 63) & ~63;
-    const size_t fixed = hb_scan_fixed_lds((len + 15) & ~15, hb_ctx_pad(n_ctx));
-    if (fixed + hb_scan_wave_lds(0, 256) > c->max_lds) {
+    if (!hb_query_lds_ok(c, len, n_ctx)) {
         delete q;
         return HB_ERR_UNSUPPORTED;
     }
+    const size_t fixed = hb_scan_fixed_lds((len + 15) & ~15, hb_ctx_pad(n_ctx));
     q->lds_fits = fixed + hb_scan_wave_lds(D, 256, 4) <= c->max_lds;
-    q->lds16_fits = fixed + hb_scan_wave_lds(D, 256, 2) <= c->max_lds;
     std::lock_guard<std::mutex> lk(c->mu);
     if (hipSetDevice(c->device) != hipSuccess) {
         delete q;
@@ -897,15 +908,19 @@ int hb_query_set_state_mode(hb_query *q, int mode)
     case HB_STATE_AUTO:
     case HB_STATE_GLOBAL: q->mode = mode; return HB_OK;
     case HB_STATE_LDS:
+    case HB_STATE_LDS16:   // one LDS format now; the old 16-bit request means LDS
         if (!q->lds_fits) return HB_ERR_UNSUPPORTED;
-        q->mode = mode;
-        return HB_OK;
-    case HB_STATE_LDS16:
-        if (!q->lds16_fits) return HB_ERR_UNSUPPORTED;
-        q->mode = mode;
+        q->mode = HB_STATE_LDS;
         return HB_OK;
     default: return HB_ERR_ARG;
     }
+}
+
+int hb_query_set_reset_interval(hb_query *q, int32_t interval)
+{
+    if (!q || interval < 0) return HB_ERR_ARG;
+    q->reset_interval = interval;
+    return HB_OK;
 }
 
 int hb_query_state_mode(const hb_query *q)
@@ -918,9 +933,8 @@ int hb_query_state_mode(const hb_query *q)
 static int state_place(const hb_query *q, const hb_db *db)
 {
     switch (q->mode) {
-    case HB_STATE_LDS: return HB_PLACE_LDS32;
-    case HB_STATE_LDS16:
-        return hb_state16_ok(db->max_len, q->p.window) ? HB_PLACE_LDS16 : -1;
+    case HB_STATE_LDS:
+        return hb_state_lds_ok(db->max_len, q->p.window) ? HB_PLACE_LDS : -1;
     case HB_STATE_GLOBAL: return HB_PLACE_GLOBAL;
     default:
         return hb_state_choose(q->ctx, q->gs.diag_len_max,
@@ -964,14 +978,15 @@ static int launch_scan(hb_scan *s)
     hb_query *q = s->q;
     hb_launch_cfg cfg;
     const int place = state_place(q, db);
-    if (place < 0) return HB_ERR_UNSUPPORTED;   // 16-bit state asked for, subjects too long
+    if (place < 0) return HB_ERR_UNSUPPORTED;   // LDS state asked for, subjects too long
     s->place = place;
     int rc = hb_scan_config(c, false, q->gs.qlen_pad_max, q->gs.ctx_pad_max, q->gs.diag_len_max,
                             db->n_tiles, &cfg, place);
     if (rc) return rc;
     if (place == HB_PLACE_GLOBAL && !s->d_gstate) {
-        s->gstate_bytes = (size_t)cfg.blocks * cfg.waves * (size_t)q->gs.diag_len_max * 4;
-        s->d_gstate = (int32_t *)hb_pool_get(c, s->gstate_bytes);
+        s->gstate_bytes =
+            (size_t)cfg.blocks * cfg.waves * (size_t)q->gs.diag_len_max * HB_GSTATE_WORD;
+        s->d_gstate = (unsigned long long *)hb_pool_get(c, s->gstate_bytes);
         if (!s->d_gstate) return HB_ERR_NOMEM;
     }
     hb_scan_args a;
@@ -987,10 +1002,16 @@ static int launch_scan(hb_scan *s)
     a.qlen_pad_max = q->gs.qlen_pad_max;
     a.ctx_pad_max = q->gs.ctx_pad_max;
     a.diag_len_max = q->gs.diag_len_max;
-    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gs.diag_len_max, cfg.scap, hb_state_bytes(place));
-    a.scap = (uint32_t)cfg.scap;
+    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gs.diag_len_max, cfg.ring, hb_state_bytes(place));
+    a.ring = (uint32_t)cfg.ring;
     a.gstate = place == HB_PLACE_GLOBAL ? s->d_gstate : nullptr;   // cleared by each wavefront
-    a.state_limit = place == HB_PLACE_LDS16 ? hb_state16_limit(db->max_len) : 0;
+    a.state_limit = hb_state_limit(place, db->max_len);
+    if (q->reset_interval > 0) {
+        // a subject and its successor must fit below the interval
+        if ((uint64_t)q->reset_interval <= 2 * (db->max_len + (uint64_t)q->p.window) + 64)
+            return HB_ERR_UNSUPPORTED;
+        a.state_limit = std::min(a.state_limit, q->reset_interval - (int32_t)db->max_len);
+    }
     a.mat32 = q->gs.d_mat;
     a.window = q->p.window;
     a.work_counter = s->d_misc + 1;
@@ -1024,9 +1045,7 @@ int hb_scan_free(hb_scan *s)
 int hb_scan_state_mode(const hb_scan *s)
 {
     if (!s) return HB_ERR_ARG;
-    return s->place == HB_PLACE_GLOBAL  ? HB_STATE_GLOBAL
-           : s->place == HB_PLACE_LDS16 ? HB_STATE_LDS16
-                                        : HB_STATE_LDS;
+    return s->place == HB_PLACE_GLOBAL ? HB_STATE_GLOBAL : HB_STATE_LDS;
 }
 
 int hb_scan_start(hb_ctx *c, hb_db *db, hb_query *q, hb_scan **out)

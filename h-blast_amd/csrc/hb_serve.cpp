@@ -308,6 +308,9 @@ int hb_serve_submit_masked(hb_server *s, const uint8_t *seq, int32_t len, int32_
     if (!s || !cutoff || !ticket) return HB_ERR_ARG;
     int rc = hb_check_query(seq, len, n_ctx, ctx_off, ctx_len, xdrop);
     if (rc) return rc;
+    // a query that cannot fit a workgroup's LDS is refused here, so it never
+    // fails the batch it would have joined
+    if (!hb_query_lds_ok(s->ctx, len, n_ctx)) return HB_ERR_UNSUPPORTED;
     auto r = std::make_shared<Request>();
     r->q.seq.assign(seq, seq + len);
     if (lut_mask) r->q.mask.assign(lut_mask, lut_mask + len);

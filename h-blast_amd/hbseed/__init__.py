@@ -54,6 +54,7 @@ EXPORTS = [
     "hb_db_residues", "hb_query_prepare", "hb_query_free", "hb_query_lut_info",
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_query_prepare_masked", "hb_query_set_state_mode",
+    "hb_query_set_reset_interval",
     "hb_query_state_mode", "hb_scan_state_mode", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
@@ -183,6 +184,7 @@ def _lib():
         L.hb_query_prepare_masked.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, pp]
         L.hb_query_free.argtypes = [vp]
         L.hb_query_set_state_mode.argtypes = [vp, C.c_int]
+        L.hb_query_set_reset_interval.argtypes = [vp, i32]
         L.hb_query_state_mode.argtypes = [vp]
         L.hb_scan_state_mode.argtypes = [vp]
         L.hb_query_lut_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
@@ -783,9 +785,17 @@ class Query:
         self.h = h
 
     def set_state_mode(self, mode):
-        """STATE_AUTO / STATE_LDS / STATE_LDS16 / STATE_GLOBAL: where scans
-        keep the per-diagonal two-hit state (hb_query_set_state_mode)."""
+        """STATE_AUTO / STATE_LDS / STATE_GLOBAL: where scans keep the
+        per-diagonal two-hit state (hb_query_set_state_mode; STATE_LDS16 is
+        accepted as STATE_LDS)."""
         _check(_lib().hb_query_set_state_mode(self.h, mode), "hb_query_set_state_mode")
+
+    def set_reset_interval(self, interval):
+        """Testing aid: clear the diagonal state every `interval` accumulated
+        subject residues per wavefront (hb_query_set_reset_interval; 0 = the
+        placement's own limit)."""
+        _check(_lib().hb_query_set_reset_interval(self.h, int(interval)),
+               "hb_query_set_reset_interval")
 
     @property
     def state_mode(self):
@@ -818,8 +828,7 @@ class Scan:
 
     @property
     def state_mode(self):
-        """Where this scan keeps its diagonal state (STATE_LDS / STATE_LDS16 /
-        STATE_GLOBAL)."""
+        """Where this scan keeps its diagonal state (STATE_LDS / STATE_GLOBAL)."""
         return _lib().hb_scan_state_mode(self.h)
 
     def elapsed_ms(self):

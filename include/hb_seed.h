@@ -160,22 +160,32 @@ int hb_query_free(hb_query *q);
 /* Where a plain scan of q keeps its per-diagonal two-hit state (the
  * reference keeps it in global memory, ewp->diag_table, one array per
  * thread: h_blast.cu:959-966):
- *   HB_STATE_AUTO   - LDS (16-bit entries when the database's subjects
- *                     allow them, else 32-bit) unless LDS would starve the
- *                     CU of wavefronts, else global memory (the default);
- *   HB_STATE_LDS    - LDS (HB_ERR_UNSUPPORTED if it does not fit);
- *   HB_STATE_GLOBAL - a global-memory array per wavefront.
+ *   HB_STATE_AUTO   - LDS (one 32-bit word per diagonal) unless LDS would
+ *                     starve the CU of wavefronts or the database holds a
+ *                     subject of 4,194,000 residues or more, else global
+ *                     memory (the default);
+ *   HB_STATE_LDS    - LDS (HB_ERR_UNSUPPORTED if the query's words do not
+ *                     fit, or, at the scan, if the subjects are too long);
+ *   HB_STATE_GLOBAL - a global-memory array per wavefront (64-bit words).
  * Results are identical; only speed differs.  Takes effect for later
  * hb_scan_start calls. */
 #define HB_STATE_AUTO 0
 #define HB_STATE_LDS 1
 #define HB_STATE_GLOBAL 2
-#define HB_STATE_LDS16 3   /* 16-bit entries in LDS: databases whose longest
-                              subject is under ~8,100 residues (else the scan
-                              returns HB_ERR_UNSUPPORTED) */
+#define HB_STATE_LDS16 3   /* accepted for compatibility: the same as
+                              HB_STATE_LDS (there is one LDS format) */
 int hb_query_set_state_mode(hb_query *q, int mode);
-/* The requested placement (HB_STATE_AUTO unless set). */
+/* The requested placement (HB_STATE_AUTO unless set; HB_STATE_LDS after
+ * HB_STATE_LDS16 was set). */
 int hb_query_state_mode(const hb_query *q);
+/* Testing aid: the scan clears its diagonal state whenever the positions it
+ * stores (subject offsets accumulated over the subjects a wavefront scans)
+ * would pass a limit set by the placement; this lowers that limit to
+ * `interval` residues (0: the placement's own limit) so that tests reach
+ * the reset on small databases.  The interval must exceed twice the
+ * longest subject plus the window (else the scan returns
+ * HB_ERR_UNSUPPORTED).  Results are identical for every interval. */
+int hb_query_set_reset_interval(hb_query *q, int32_t interval);
 /* Number of (query offset, cell) entries of the lookup table. */
 int hb_query_lut_info(const hb_query *q, uint32_t *n_entries, uint32_t *n_cells_used);
 /* Copies the device lookup table back (CSR over 32768 cells:
@@ -399,8 +409,8 @@ int hb_scan_results(hb_scan *s, hb_subject_stats *stats, hb_hsp *hsps,
  * (gpu_cpu_common.h:71). */
 int hb_scan_elapsed_ms(hb_scan *s, float *ms);
 int hb_scan_free(hb_scan *s);
-/* Where the scan keeps its diagonal state: HB_STATE_LDS, HB_STATE_LDS16 or
- * HB_STATE_GLOBAL (see hb_query_set_state_mode). */
+/* Where the scan keeps its diagonal state: HB_STATE_LDS or HB_STATE_GLOBAL
+ * (see hb_query_set_state_mode). */
 int hb_scan_state_mode(const hb_scan *s);
 
 /* Convenience: prepare + start + results for one query. */

@@ -19,11 +19,13 @@ def ctx():
     c.close()
 
 
-def run(ctx, db, spec, mode, window=40):
+def run(ctx, db, spec, mode, window=40, interval=0):
     q = hbseed.Query(ctx, spec, window=window)
     try:
         q.set_state_mode(mode)
         assert q.state_mode == mode
+        if interval:
+            q.set_reset_interval(interval)
         s = hbseed.Scan(ctx, db, q)
         try:
             assert s.state_mode == mode
@@ -43,12 +45,11 @@ def long_spec(kind, n, seed):
     return hbseed.blastx_query(nt)
 
 
-@pytest.mark.parametrize("kind,n,lds,lds16", [
-    ("blastp", 300, True, True), ("blastp", 2000, True, True), ("blastx", 6000, True, True),
-    ("blastp", 20000, True, True), ("blastp", 40000, False, True),
-    ("blastp", 60000, False, False),
+@pytest.mark.parametrize("kind,n,lds", [
+    ("blastp", 300, True), ("blastp", 2000, True), ("blastx", 6000, True),
+    ("blastp", 20000, True), ("blastp", 40000, False), ("blastp", 60000, False),
 ])
-def test_state_placements_match_each_other_and_oracle(ctx, kind, n, lds, lds16):
+def test_state_placements_match_each_other_and_oracle(ctx, kind, n, lds):
     spec = long_spec(kind, n, 3000 + n)
     print(f"{kind} query of {len(spec.seq)} residues ({spec.n_ctx} contexts)", flush=True)
     res, offs = hbseed.make_db(3100 + n, 2000, plant_query=spec.seq[:min(len(spec.seq), 3000)],
@@ -56,7 +57,7 @@ def test_state_placements_match_each_other_and_oracle(ctx, kind, n, lds, lds16):
     db = hbseed.Database(ctx, res, offs)
     try:
         sg, hg = run(ctx, db, spec, hbseed.STATE_GLOBAL)
-        for mode, fits in ((hbseed.STATE_LDS, lds), (hbseed.STATE_LDS16, lds16)):
+        for mode, fits in ((hbseed.STATE_LDS, lds),):
             if fits:
                 sl, hl = run(ctx, db, spec, mode)
                 assert np.array_equal(hl.view(np.int32), hg.view(np.int32)), mode
@@ -145,11 +146,14 @@ def test_batched_and_served_long_queries(ctx):
         db.close()
 
 
-def test_16bit_state_at_its_subject_length_limit(ctx):
-    """Subjects up to 7,880 residues, many near it (with window 255 the
-    14-bit positions then reach 16,270 before a reset), window 40 and 255:
-    the 16-bit state gives the 32-bit state's and the oracle's results;
-    longer subjects refuse it."""
+def test_state_resets_keep_results(ctx):
+    """The scan clears its diagonal state whenever the positions it stores
+    (subject offsets accumulated over a wavefront's subjects) would pass the
+    placement's limit, which the synthetic databases never reach.  Reset
+    intervals down to the smallest allowed one (twice the longest subject
+    plus the window) force a reset every few subjects: subjects up to 7,880
+    residues, many near it, windows 40 and 255, LDS and global placements
+    all give the oracle's results; a smaller interval is refused."""
     rng = np.random.default_rng(3500)
     lens = np.concatenate([rng.integers(7800, 7881, 40), rng.integers(30, 600, 400)])
     rng.shuffle(lens)
@@ -162,31 +166,51 @@ def test_16bit_state_at_its_subject_length_limit(ctx):
         k = min(n, 200)
         res[a + n - k:a + n] = q[100:100 + k]
     spec = hbseed.blastp_query(q)
+    longest = int(lens.max())
     db = hbseed.Database(ctx, res, offs)
     try:
         for window in (40, 255):
-            s32, h32 = run(ctx, db, spec, hbseed.STATE_LDS, window)
-            s16, h16 = run(ctx, db, spec, hbseed.STATE_LDS16, window)
-            assert len(h32) > 0
-            assert np.array_equal(h16.view(np.int32), h32.view(np.int32)), window
-            assert np.array_equal(s16.view(np.uint8), s32.view(np.uint8)), window
             _, ref = oracle_lib.OracleQuery.from_spec(spec, window=window).scan(res, offs,
                                                                                 threads=8)
-            assert np.array_equal(h16.view(np.int32), ref.view(np.int32)), window
+            assert len(ref) > 0
+            s0, h0 = run(ctx, db, spec, hbseed.STATE_LDS, window)
+            assert np.array_equal(h0.view(np.int32), ref.view(np.int32)), window
+            smallest = 2 * (longest + window) + 65
+            for mode in (hbseed.STATE_LDS, hbseed.STATE_GLOBAL):
+                for interval in (smallest, smallest + 5000, 60000):
+                    print(f"window {window} mode {mode} reset interval {interval}", flush=True)
+                    si, hi = run(ctx, db, spec, mode, window, interval)
+                    assert np.array_equal(hi.view(np.int32), ref.view(np.int32)), interval
+                    assert np.array_equal(si.view(np.uint8), s0.view(np.uint8)), interval
+            qq = hbseed.Query(ctx, spec, window=window)
+            try:
+                qq.set_reset_interval(smallest - 1)
+                with pytest.raises(hbseed.HBError) as e:
+                    hbseed.Scan(ctx, db, qq)
+                assert e.value.code == -7
+            finally:
+                qq.close()
     finally:
         db.close()
-    # a 9,000-residue subject: 16-bit positions no longer fit
-    offs2 = np.array([0, 9000, 9100], np.uint64)
-    res2 = hbseed.random_protein(3600, 9100)
-    db = hbseed.Database(ctx, res2, offs2)
+
+
+def test_lds16_request_is_the_lds_placement(ctx):
+    """The 16-bit request of the earlier interface selects the one LDS format."""
+    spec = hbseed.blastp_query(hbseed.random_protein(3601, 400))
+    res, offs = hbseed.make_db(3602, 500, plant_query=spec.seq, plant_frac=0.2)
+    db = hbseed.Database(ctx, res, offs)
+    q = hbseed.Query(ctx, spec)
     try:
-        q = hbseed.Query(ctx, spec)
+        q.set_state_mode(hbseed.STATE_LDS16)
+        assert q.state_mode == hbseed.STATE_LDS
+        s = hbseed.Scan(ctx, db, q)
         try:
-            q.set_state_mode(hbseed.STATE_LDS16)
-            with pytest.raises(hbseed.HBError) as e:
-                hbseed.Scan(ctx, db, q)
-            assert e.value.code == -7
+            assert s.state_mode == hbseed.STATE_LDS
+            _, h = s.results()
         finally:
-            q.close()
+            s.close()
     finally:
+        q.close()
         db.close()
+    _, ref = oracle_lib.OracleQuery.from_spec(spec).scan(res, offs, threads=8)
+    assert np.array_equal(h.view(np.int32), ref.view(np.int32))
