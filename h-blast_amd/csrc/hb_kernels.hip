@@ -470,8 +470,14 @@ struct WaveScan {
     // marker still 0 ORs the state back), the hit that finds its own marker
     // after the pass's sync decides, and its store of the new state clears
     // the marker.  A winner whose diagonal waits on a pending extension joins
-    // the blocked list.  Returns true for the losing hits: they decide in a
-    // later pass, before any later hit of their diagonal.
+    // the blocked list.  The losers then read the winner's new state: a
+    // loser whose diagonal is now pending joins the blocked list behind the
+    // winner, and a loser that would skip against that state is done (a skip
+    // changes nothing, and skipping is monotone along a diagonal: a hit
+    // skips a flagged state iff s < last, an unflagged one iff s - last < 3,
+    // so every earlier loser of the diagonal skips too).  Returns true for
+    // the other losers: they decide in a later pass, before any later hit of
+    // their diagonal.
     // Per hit (state per diagonal, flagged = extended to the right):
     //   pending            -> wait
     //   flagged            -> s < last: skip; else last = s, unflag
@@ -481,14 +487,14 @@ struct WaveScan {
     __device__ __forceinline__ bool decide_pass(bool active, uint32_t q, int32_t so, uint32_t d,
                                                 int32_t sbase, uint32_t sl)
     {
+        // d is a valid index in every lane, active or not
         const uint32_t mk = (uint32_t)(HB_WAVE - lane);
         word_t old = 0;
         if (active) old = w_max(d, (word_t)mk << W::kMS);
         const uint32_t ost = (uint32_t)(old & W::kSM);
         if (active && (uint32_t)(old >> W::kMS) == 0 && ost != 0) w_or(d, (word_t)ost);
         sync();
-        word_t wv = 0;
-        if (active) wv = w_ld(d);
+        const word_t wv = w_ld(d);
         const bool win = active && (uint32_t)(wv >> W::kMS) == mk;
         const uint32_t st = (uint32_t)(wv & W::kSM);
         const bool pend = (st & W::kP) != 0;
@@ -497,20 +503,29 @@ struct WaveScan {
         const int32_t diff = sb - (int32_t)st;   // s - last if unflagged and not pending
         const bool job = win && !pend && !flag && diff >= 3 && diff < window;
         const bool wait = win && pend;
-        uint32_t nst;
-        if (flag) nst = sb >= (int32_t)(st & W::kV) ? (uint32_t)sb : st;
-        else nst = diff >= window ? (uint32_t)sb : (job ? W::kP : st);
+        const uint32_t fst = flag ? (sb >= (int32_t)(st & W::kV) ? (uint32_t)sb : st)
+                                  : (diff >= window ? (uint32_t)sb : st);
+        const uint32_t nst = job ? W::kP : fst;
         if (win) w_st(d, (word_t)nst);
-        const uint64_t mj = __ballot(job), mb = __ballot(wait);
+        sync();
+        // losers: against the winner's new state
+        const uint32_t st2 = (uint32_t)(w_ld(d) & W::kSM);
+        const bool pend2 = (st2 & W::kP) != 0;
+        const bool skip2 = (st2 & W::kF) != 0 ? sb < (int32_t)(st2 & W::kV)
+                                              : sb - (int32_t)st2 < 3;
+        const bool lose = active && !win;
+        const bool lwait = lose && pend2;
+        const bool carry = lose && !pend2 && !skip2;
+        const uint64_t mj = __ballot(job), mb = __ballot(wait || lwait);
         const uint32_t tso = (uint32_t)so | (sl << 28);
         if (job)
             jobs[(j_head + n_jobs + lane_rank(mj)) & (kJobCap - 1)] =
                 make_uint2(q | ((uint32_t)diff << 24), tso);
-        if (wait) blk[n_blk + lane_rank(mb)] = make_uint2(q, tso);
+        if (wait || lwait) blk[n_blk + lane_rank(mb)] = make_uint2(q, tso);
         n_jobs += (uint32_t)__popcll(mj);
         n_blk += (uint32_t)__popcll(mb);
         sync();
-        return active && !win;
+        return carry;
     }
 
     // Decides a set of hits completely (passes until no hit is left).
@@ -669,79 +684,142 @@ struct WaveScan {
         return n;
     }
 
+    // One 64-position window of a subject: lane i holds the cell {start,
+    // count} of the 3-mer at s0 + i; excl is the exclusive prefix of the
+    // counts (the window's hits in (s, q) order are flat indices 0 .. H-1;
+    // hit f of lane i's cell is entry f + eb).
+    struct Win {
+        uint32_t cnt, excl;
+        int32_t eb;
+        uint32_t H;
+        int32_t s0, s0m;   // first position, and it mod D
+    };
+
+    template <typename SP>
+    __device__ __forceinline__ void load_window(SP S, int32_t slen, int32_t s0, int32_t s0m,
+                                                Win &w) const
+    {
+        const int32_t s = s0 + lane;
+        uint32_t cnt = 0, cst = 0;
+        if (s + 3 <= slen) {
+            const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
+            // every cell is present (count 0 for words without neighbours);
+            // a presence-bit test first saves no load for the queries this
+            // stage sees and costs an LDS read
+            const uint64_t c2 = cells[idx];
+            cst = (uint32_t)c2;
+            cnt = (uint32_t)(c2 >> 32);
+        }
+        const uint32_t incl = wave_incl_add(cnt);
+        w.cnt = cnt;
+        w.excl = incl - cnt;
+        w.eb = (int32_t)cst - (int32_t)w.excl;
+        w.H = lane63(incl);
+        w.s0 = s0;
+        w.s0m = s0m;
+    }
+
     // Scans one subject (slot sl, diagonal offset sbase).  Hits are taken in
-    // (s, q) order, up to 64 per round; a hit that lost its diagonal to a
-    // lower lane in a round is carried to the front lanes of the next round
-    // (ahead of every newer hit, so each diagonal still sees its hits in
-    // subject order), and a round makes a single decision pass.  Returns the
-    // subject's lookup hits.
+    // (s, q) order, 64 per round: a round takes the rest of the current
+    // window and, when that is not enough, the start of the next one, so
+    // rounds are full until the subject's last.  A hit that lost its
+    // diagonal to a lower lane in a round is carried to the front lanes of
+    // the next round (ahead of every newer hit, so each diagonal still sees
+    // its hits in subject order), and a round makes a single decision pass.
+    // Returns the subject's lookup hits.
     template <bool SL, typename SP>
     __device__ __forceinline__ uint32_t subject(SP S, int32_t slen, uint32_t sl, int32_t sbase)
     {
         uint32_t hits = 0;
-        int32_t s0m = 0;  // s0 mod D
-        uint32_t c = 0;   // carried hits, in lanes 0..c-1
+        Win A, B;
+        A.H = 0;
+        A.cnt = A.excl = 0;
+        A.eb = 0;
+        A.s0 = A.s0m = 0;
+        B = A;
+        bool hasB = false;        // B holds the window after A
+        int32_t nxt = 0, nxtm = 0;   // the next window to load, and it mod D
+        uint32_t r = 0;           // hits of A already taken
+        uint32_t c = 0;           // carried hits, in lanes 0..c-1
         uint32_t cq = 0, cd = 0;
         int32_t cs_ = 0;
-        for (int32_t s0 = 0; s0 + 3 <= slen; s0 += HB_WAVE) {
-            const int32_t s = s0 + lane;
-            uint32_t cnt = 0, cst = 0;
-            if (s + 3 <= slen) {
-                const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
-                // every cell is present (count 0 for words without
-                // neighbours); a presence-bit test first saves no load for
-                // the queries this stage sees and costs an LDS read
-                const uint64_t c2 = cells[idx];
-                cst = (uint32_t)c2;
-                cnt = (uint32_t)(c2 >> 32);
-            }
-            const uint32_t incl = wave_incl_add(cnt);
-            const uint32_t excl = incl - cnt;
-            const uint32_t H = lane63(incl);
-            const int32_t ebase = (int32_t)cst - (int32_t)excl;  // entry index = flat index + ebase
-            hits += H;
-            uint32_t carry = 0;
-            for (uint32_t r0 = 0; r0 < H;) {
-                drain_some<SL>(S);
-                // new hits r0 .. r0 + take - 1 go to lanes c .. c + take - 1
-                const uint32_t take = min((uint32_t)HB_WAVE - c, H - r0);
-                if (++rgen == (1u << 25)) {
-                    seg[lane] = 0;
-                    rgen = 1;
+        uint32_t carry = 0;       // source of the last lane of the previous round
+        for (;;) {
+            // A: a window with hits left, unless the subject has no more
+            while (r == A.H && (hasB || nxt + 3 <= slen)) {
+                if (hasB) {
+                    A = B;
+                    hasB = false;
+                } else {
+                    load_window(S, slen, nxt, nxtm, A);
+                    hits += A.H;
+                    nxt += HB_WAVE;
+                    nxtm += HB_WAVE;
+                    if (nxtm >= D) nxtm -= D;
                 }
-                const uint32_t tag = rgen << 7;
-                if (cnt && excl >= r0 && excl < r0 + take) seg[excl - r0 + c] = tag | ((uint32_t)lane + 1);
-                sync();
-                const uint32_t sv = seg[lane];
-                const uint32_t v = wave_incl_max((sv & ~127u) == tag ? (sv & 127u) : 0u);
-                const uint32_t p = min(v ? v - 1 : carry, 63u);
-                carry = (uint32_t)__builtin_amdgcn_readlane((int)p, (int)(c + take - 1));
-                const bool isnew = (uint32_t)lane >= c && (uint32_t)lane < c + take;
-                const uint32_t h = r0 + (uint32_t)lane - c;
-                const int32_t eb = __shfl(ebase, (int)p, HB_WAVE);
-                const uint32_t ei = min((uint32_t)((int32_t)h + eb), n_entries - 1);
-                const uint32_t qn = isnew ? entries[ei] : 0u;
-                int32_t smp = s0m + (int32_t)p;
-                if (smp >= D) smp -= D;
-                int32_t dn = (int32_t)qn - smp;
-                if (dn < 0) dn += D;
-                const bool isc = (uint32_t)lane < c;
-                const uint32_t q = isc ? cq : qn;
-                const int32_t so = isc ? cs_ : s0 + (int32_t)p;
-                const uint32_t d = isc ? cd : (uint32_t)dn;
-                sync();
-                const bool lost = decide_pass(isc || isnew, q, so, d, sbase, sl);
-                c = carry_losers(lost, q, so, d, cq, cs_, cd);
-                r0 += take;
+                r = 0;
             }
-            s0m += HB_WAVE;
-            if (s0m >= D) s0m -= D;
-        }
-        while (c) {   // hits still carried at the end of the subject
+            const uint32_t need = (uint32_t)HB_WAVE - c;
+            if (!hasB && A.H - r < need && nxt + 3 <= slen) {
+                load_window(S, slen, nxt, nxtm, B);
+                hits += B.H;
+                hasB = true;
+                nxt += HB_WAVE;
+                nxtm += HB_WAVE;
+                if (nxtm >= D) nxtm -= D;
+            }
+            const uint32_t take1 = min(need, A.H - r);
+            const uint32_t take2 = hasB ? min(need - take1, B.H) : 0u;
+            if (take1 + take2 + c == 0) break;
             drain_some<SL>(S);
+            // new hits: A's r .. r + take1 - 1 to lanes c .. c + take1 - 1,
+            // then B's 0 .. take2 - 1; segment starts are tagged with their
+            // source position + 1 (B's positions count from 64)
+            if (++rgen == (1u << 24)) {
+                seg[lane] = 0;
+                rgen = 1;
+            }
+            const uint32_t tag = rgen << 8;
+            if (A.cnt && A.excl >= r && A.excl < r + take1)
+                seg[A.excl - r + c] = tag | ((uint32_t)lane + 1);
+            if (take2 && B.cnt && B.excl < take2)
+                seg[B.excl + take1 + c] = tag | ((uint32_t)lane + 65);
+            sync();
+            const uint32_t sv = seg[lane];
+            const uint32_t v = wave_incl_max((sv & ~255u) == tag ? (sv & 255u) : 0u);
+            const uint32_t p = min(v ? v - 1 : carry, 127u);
+            const uint32_t last = c + take1 + take2 - 1;
+            const uint32_t isnew_hi = c + take1 + take2;
+            carry = (uint32_t)__builtin_amdgcn_readlane((int)p, (int)min(last, 63u));
+            const bool isnew = (uint32_t)lane >= c && (uint32_t)lane < isnew_hi;
+            const bool inb = (uint32_t)lane >= c + take1;
+            int32_t eb = __shfl(A.eb, (int)(p & 63u), HB_WAVE);
+            if (take2) {
+                const int32_t ebb = __shfl(B.eb, (int)(p & 63u), HB_WAVE);
+                eb = p >= 64 ? ebb : eb;
+            }
+            const uint32_t flat = inb ? (uint32_t)lane - c - take1 : r + (uint32_t)lane - c;
+            const uint32_t ei = min((uint32_t)((int32_t)flat + eb), n_entries - 1);
+            const uint32_t qn = isnew ? entries[ei] : 0u;
+            int32_t smp = A.s0m + (int32_t)p;
+            if (smp >= D) smp -= D;
+            if (smp >= D) smp -= D;
+            int32_t dn = (int32_t)qn - smp;
+            if (dn < 0) dn += D;
             const bool isc = (uint32_t)lane < c;
-            const bool lost = decide_pass(isc, cq, cs_, cd, sbase, sl);
-            c = carry_losers(lost, cq, cs_, cd, cq, cs_, cd);
+            const uint32_t q = isc ? cq : qn;
+            const int32_t so = isc ? cs_ : A.s0 + (int32_t)p;
+            const uint32_t d = isc ? cd : (uint32_t)dn;
+            sync();
+            const bool lost = decide_pass(isc || isnew, q, so, d, sbase, sl);
+            c = carry_losers(lost, q, so, d, cq, cs_, cd);
+            r += take1;
+            if (take2) {   // the round ended in B: it becomes the current window
+                A = B;
+                hasB = false;
+                r = take2;
+                carry -= 64;
+            }
         }
         return hits;
     }
