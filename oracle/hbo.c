@@ -180,7 +180,12 @@ void hbo_lut_free(hbo_lut *lut)
 /* Left X-drop walk from (s_off, q_off) towards offset 0
  * (H-BLAST/h_blast_kernel.cu:1329-1540): n = min(s_off, q_off), positions
  * i = n..0, strict '>' updates the best, stop once max - score >= dropoff;
- * length = n - best_i + 1 (0 if nothing improved). */
+ * length = n - best_i + 1 (0 if nothing improved).  At n = 0 this walks
+ * position 0, as NCBI's loop (for i = n; i >= 0; i--) does; the reference
+ * kernel guards its walk with if (n > 0) (h_blast_kernel.cu:1368) and
+ * returns length 0 there.  The two-hit path never reaches n < 2 (the
+ * trigger lies at least 3 residues after the first hit on both sequences),
+ * so the difference does not show in any HSP. */
 int32_t hbo_extend_left(const int8_t *m, const uint8_t *s, const uint8_t *q,
                         int32_t s_off, int32_t q_off, int32_t dropoff,
                         int32_t maxscore, int32_t *length)

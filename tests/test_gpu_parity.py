@@ -262,3 +262,19 @@ def test_per_context_xdrop_and_cutoff(ctx):
                             xdrop=[10, 16, 22, 30, 7, 16], cutoff=[35, 41, 50, 30, 41, 60])
     stats, hsps = compare(ctx, spec, res, offs)
     assert len(hsps) > 0
+
+
+def test_smallest_left_walk_crosses_offset_zero(ctx):
+    """The two-hit case at the smallest offsets (the trigger 3 residues after
+    the first hit on both sequences, no improving prefix, so the left walk
+    starts at offset 2 and reaches offset 0), alone and repeated over
+    subjects, equals the oracle."""
+    import test_oracle_pins as pins
+    m, q, s = pins.smallest_left_walk_case()
+    spec = hbseed.QuerySpec(q, [0], [len(q)], [16], [5])
+    res = np.concatenate([s] * 50 + [q[:6], s[:7]])
+    offs = np.zeros(53, np.uint64)
+    offs[1:51] = np.arange(1, 51) * len(s)
+    offs[51] = offs[50] + 6
+    offs[52] = offs[51] + 7
+    compare(ctx, spec, res, offs, matrix=m)

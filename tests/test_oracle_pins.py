@@ -216,6 +216,121 @@ def test_left_walk_closed_form(seed):
         assert (sc, ln) == (max(0, int(seen.max())), exp_len)
 
 
+def test_left_walk_at_offset_zero_follows_ncbi():
+    """With n = min(s_off, q_off) = 0 the oracle walks position 0, as NCBI's
+    loop (for i = n; i >= 0; i--) does; the reference kernel guards its walk
+    with if (n > 0) (h_blast_kernel.cu:1368) and would return length 0."""
+    m, seq = path_matrix([5])
+    assert oracle_lib.extend_left(m, seq, seq, 0, 0, 16, 0) == (5, 1)
+    m, seq = path_matrix([-3])
+    assert oracle_lib.extend_left(m, seq, seq, 0, 0, 16, 0) == (0, 0)
+
+
+def left_walk_starts(spec, res, offs, m, T=11, window=40):
+    """min(s_off, q_off) at which every extension's left walk starts, from
+    the two-hit rule restated per diagonal: the walk starts one residue before
+    the trigger word's best prefix end (q + right_d - 1, s + right_d - 1)."""
+    mi = m.astype(np.int64)
+    seq = spec.seq
+    words = []
+    for o, l in zip(spec.ctx_off, spec.ctx_len):
+        words.extend(range(o, o + l - 2))
+    starts = []
+    for oid in range(len(offs) - 1):
+        s = res[offs[oid]:offs[oid + 1]]
+        hl = []
+        for b in range(len(s) - 2):
+            for a in words:
+                sc = sum(int(mi[seq[a + i], s[b + i]]) for i in range(3))
+                if sc >= T or all(seq[a + i] == s[b + i] for i in range(3)):
+                    hl.append((a, b))
+        by_diag = {}
+        for a, b in hl:
+            by_diag.setdefault(a - b, []).append((a, b))
+        for hits in by_diag.values():
+            last, flag = None, False
+            for a, b in hits:
+                if flag:
+                    if b >= last:
+                        last, flag = b, False
+                    continue
+                if last is None or b - last >= window:
+                    last = b
+                    continue
+                if b - last < 3:
+                    continue
+                sc, best, rd = 0, 0, 0
+                for i in range(3):
+                    sc += int(mi[seq[a + i], s[b + i]])
+                    if sc > best:
+                        best, rd = sc, i + 1
+                starts.append(min(a + rd - 1, b + rd - 1))
+                k = int(np.searchsorted(spec.ctx_off, a, side="right")) - 1
+                r = oracle_lib.extend_two_hit(m, s, seq, last + 3, b, a, int(spec.xdrop[k]))
+                if r["right"]:
+                    last, flag = r["s_last"] - 2, True
+                else:
+                    last = b
+    return starts
+
+
+def test_left_walk_never_starts_below_offset_two():
+    """Every extension is triggered by a hit at least 3 residues after an
+    earlier hit of its diagonal on both sequences, so its left walk starts
+    at offset >= 2 of both: the n = 0 case of the walk (where the reference
+    kernel and NCBI differ) is never reached.  Brute force over repetitive
+    queries and subjects that put words at offset 0 of both."""
+    m = oracle_lib.blosum62()
+    rng = np.random.default_rng(77)
+    starts = []
+    for t in range(6):
+        unit = hbseed.random_protein(700 + t, 4)
+        q = np.tile(unit, 8)[:30].astype(np.uint8)
+        subj = [np.tile(unit, 6)[:20 + 3 * k].astype(np.uint8) for k in range(4)]
+        subj.append(rng.choice(q, 25).astype(np.uint8))
+        res = np.concatenate(subj)
+        offs = np.zeros(len(subj) + 1, np.uint64)
+        offs[1:] = np.cumsum([len(x) for x in subj])
+        starts += left_walk_starts(hbseed.blastp_query(q), res, offs, m)
+    # runs of X (BLOSUM62 X-X = -1: exact words seed with no improving
+    # prefix, so the walk starts one residue before the trigger)
+    X = CODE["X"]
+    q = np.array([X] * 10 + list(hbseed.random_protein(790, 12)), np.uint8)
+    s = np.array([X] * 12 + list(hbseed.random_protein(791, 10)), np.uint8)
+    starts += left_walk_starts(hbseed.blastp_query(q), s, np.array([0, len(s)], np.uint64), m)
+    assert len(starts) > 50
+    assert min(starts) >= 2
+    assert 2 in starts   # the bound is reached
+
+
+def smallest_left_walk_case():
+    """A two-hit case at the smallest offsets: Y B B B B B on both sequences
+    with M[Y][Y] = 7, M[B][B] = -1 (exact words always seed), the rest -2.
+    The trigger (3, 3) has no improving prefix (right_d = 0), so the left
+    walk starts at offset 2 and walks 2, 1, 0: running -1, -2, +5 -> left
+    length 3, score 5; the right walk from (3, 3) stops at once (score <= 0):
+    HSP q_start = s_start = 0, length 3, score 5."""
+    Y, B, A, C = CODE["Y"], CODE["B"], CODE["A"], CODE["C"]
+    m = np.full((28, 28), -2, np.int8)
+    m[0, :] = m[:, 0] = -128
+    m[Y, Y] = 7
+    m[B, B] = -1
+    q = np.array([Y, B, B, B, B, B] + [A] * 30, np.uint8)
+    s = np.array([Y, B, B, B, B, B] + [C] * 30, np.uint8)
+    return m, q, s
+
+
+def test_smallest_left_walk_crosses_offset_zero():
+    m, q, s = smallest_left_walk_case()
+    spec = hbseed.QuerySpec(q, [0], [len(q)], [16], [5])
+    cnt, h = oracle_lib.OracleQuery.from_spec(spec, matrix=m).scan(
+        s, np.array([0, len(s)], np.uint64))
+    first = h[h["q_off"] == 3]
+    assert len(first) == 1
+    r = first[0]
+    assert (r["q_start"], r["s_start"], r["s_off"], r["len"], r["score"]) == (0, 0, 3, 3, 5)
+
+
 def test_right_walk_ties_keep_first_best():
     m, seq = path_matrix([3, -1, 1, 2, -16])  # running 3,2,3,5 -> peak 5 at index 3
     assert oracle_lib.extend_right(m, seq, seq, 0, 0, 16, 0) == (5, 4, 4)
