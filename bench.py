@@ -33,6 +33,7 @@ for _p in (os.path.join(ROOT, "h-blast_amd"), os.path.join(ROOT, "tests")):
 import numpy as np  # noqa: E402
 
 import hbseed  # noqa: E402
+from hbseed import workloads  # noqa: E402
 
 HBM_PEAK = 8.0e12   # bytes/s, MI355X
 
@@ -44,14 +45,11 @@ def log(msg):
 def workload_c3(n_queries):
     """2,048 BLASTP queries, log-normal lengths (median 350, sigma 0.7,
     clipped 50-3,000), residues from background frequencies (seed 1300)."""
-    lens = hbseed.lognormal_lengths(1300, n_queries, 350.0, 0.7, 50, 3000)
-    return [hbseed.random_protein(1300 * 100003 + i, int(L)) for i, L in enumerate(lens)]
+    return workloads.batched_queries(n_queries)
 
 
 def workload_c1(n_subjects):
-    q = hbseed.random_protein(1101, 1000)
-    res, offs = hbseed.make_db(1100, n_subjects, plant_query=q, plant_frac=0.05)
-    return q, res, offs
+    return workloads.plain_db(n_subjects)
 
 
 def algorithmic_bytes(n_res, n_subj, n_hsp, n_entries):
@@ -623,24 +621,7 @@ def workload_c4(n, seed=1400):
     the caller) and the query mix: 80% BLASTP with lengths as the batched
     workload (log-normal median 350, sigma 0.7, clipped 50-3,000), 20%
     BLASTX of U[600, 3000] nt with two planted coding regions."""
-    rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0, n)
-    kinds = rng.random(n) < 0.8
-    lens = hbseed.lognormal_lengths(seed, n, 350.0, 0.7, 50, 3000)
-    nts = rng.integers(600, 3001, n)
-    src = hbseed.random_protein(1101, 1000)
-    specs = []
-    for i in range(n):
-        if kinds[i]:
-            specs.append(("blastp", hbseed.blastp_query(
-                hbseed.random_protein(seed * 100003 + i, int(lens[i])))))
-        else:
-            seg_len = int(min(100, nts[i] // 7))
-            a = int(rng.integers(0, 1000 - seg_len))
-            seg = hbseed.homolog(seed * 7 + i, src[a:a + seg_len], 0.7, 1)
-            nt = hbseed.blastx_nt(seed * 13 + i, int(nts[i]), seg, seg[::-1].copy())
-            specs.append(("blastx", hbseed.blastx_query(nt)))
-    return gaps, specs
+    return workloads.served_stream(n, seed)
 
 
 def bench_c4(args):
@@ -650,9 +631,8 @@ def bench_c4(args):
     queries/s, subject residues/s (queries x database residues / wall) and
     p50/p99 latency from submit to results ready; a sample of the served
     results is checked against the plain per-query path."""
-    q1 = hbseed.random_protein(1101, 1000)
     log(f"generating {args.serve_subjects} subjects")
-    res, offs = hbseed.make_db(1400, args.serve_subjects, plant_query=q1, plant_frac=0.05)
+    res, offs = workloads.served_db(args.serve_subjects)
     n_res = int(offs[-1])
     n_arr = max(1, int(args.qps * args.duration * 1.5) + 20)
     gaps, specs = workload_c4(n_arr)

@@ -499,3 +499,47 @@ int64_t hbo_scan_db(const hbo_query *q, const hbo_lut *lut,
 }
 
 void hbo_free(void *p) { free(p); }
+
+/* Property check of HSP records against the sequences (for results whose
+ * subjects are too many to search with the oracle): each record's subject,
+ * context and extent lie inside their sequences, q_off - s_off == q_start -
+ * s_start, the score recomputed from the matrix equals the record's and
+ * reaches the context's cutoff, and records are in strictly increasing
+ * (oid, s_off, q_off) order.  Returns the number of bad records; *first_bad
+ * = the first one's index (or n). */
+uint64_t hbo_hsp_check(const hbo_query *q, const uint8_t *residues, const uint64_t *offsets,
+                       uint32_t n_subjects, const hbo_hsp *h, uint64_t n, uint64_t *first_bad)
+{
+    const int8_t *m = q->matrix ? q->matrix : hbo_blosum62();
+    uint64_t bad = 0;
+    *first_bad = n;
+    for (uint64_t i = 0; i < n; ++i) {
+        const hbo_hsp *r = &h[i];
+        int ok = r->oid < n_subjects && r->ctx >= 0 && r->ctx < q->n_ctx && r->len > 0 &&
+                 r->q_off - r->s_off == r->q_start - r->s_start;
+        if (ok) {
+            const int64_t slen = (int64_t)(offsets[r->oid + 1] - offsets[r->oid]);
+            const int32_t c0 = q->ctx_off[r->ctx], c1 = c0 + q->ctx_len[r->ctx];
+            ok = r->s_start >= 0 && (int64_t)r->s_start + r->len <= slen && r->q_start >= c0 &&
+                 r->q_start + r->len <= c1 && r->q_off >= c0 && r->q_off + 3 <= c1 &&
+                 r->s_off >= 0 && (int64_t)r->s_off + 3 <= slen;
+        }
+        if (ok) {
+            const uint8_t *s = residues + offsets[r->oid] + r->s_start;
+            const uint8_t *qq = q->seq + r->q_start;
+            int64_t sc = 0;
+            for (int32_t k = 0; k < r->len; ++k) sc += m[qq[k] * HBO_ALPHA + s[k]];
+            ok = sc == r->score && r->score >= q->cutoff[r->ctx];
+        }
+        if (ok && i > 0) {
+            const hbo_hsp *p = &h[i - 1];
+            ok = p->oid < r->oid || (p->oid == r->oid && (p->s_off < r->s_off ||
+                                     (p->s_off == r->s_off && p->q_off < r->q_off)));
+        }
+        if (!ok) {
+            if (!bad) *first_bad = i;
+            ++bad;
+        }
+    }
+    return bad;
+}

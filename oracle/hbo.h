@@ -94,6 +94,11 @@ int64_t hbo_scan_db(const hbo_query *q, const hbo_lut *lut,
                     uint32_t n_subjects, int n_threads, hbo_hsp **out,
                     uint32_t *counters);
 void hbo_free(void *p);
+/* Property check of HSP records against the sequences (extent, context,
+ * diagonal, recomputed score, cutoff, (oid, s_off, q_off) order); returns
+ * the number of bad records, *first_bad the first one's index or n. */
+uint64_t hbo_hsp_check(const hbo_query *q, const uint8_t *residues, const uint64_t *offsets,
+                       uint32_t n_subjects, const hbo_hsp *h, uint64_t n, uint64_t *first_bad);
 
 /* --- preliminary gapped extension (hbo_gap.c) ---------------------------- */
 typedef struct hbo_gap_params {

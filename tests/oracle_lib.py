@@ -112,6 +112,21 @@ class OracleQuery:
         lib().hbo_free(out)
         return counters[:3 * n].reshape(n, 3), hsps
 
+    def check_hsps(self, residues, offsets, hsps):
+        """Property check of device HSP records (hbo_hsp_check): returns
+        (number of bad records, index of the first)."""
+        residues = np.ascontiguousarray(residues, np.uint8)
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        h = np.ascontiguousarray(hsps).view(HSP_DTYPE)
+        first = C.c_uint64()
+        L = lib()
+        L.hbo_hsp_check.restype = C.c_uint64
+        L.hbo_hsp_check.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                    C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]
+        bad = L.hbo_hsp_check(C.addressof(self.q), residues.ctypes.data, offsets.ctypes.data,
+                              len(offsets) - 1, h.ctypes.data, len(h), C.byref(first))
+        return int(bad), int(first.value)
+
     def __del__(self):
         try:
             lib().hbo_lut_free(C.byref(self.lut))

@@ -505,6 +505,35 @@ def test_oracle_multi_context_equals_restatement():
     assert [tuple(int(x) for x in r) for r in h] == eh
 
 
+def test_hsp_property_check_catches_planted_mistakes():
+    """The property check used on results too large for the oracle passes
+    the oracle's own records and flags one planted mistake of each kind: a
+    score off by one, a start shifted along the subject (diagonal broken), a
+    record past its subject's end, a record crossing a context boundary,
+    a score below the cutoff and two records out of order."""
+    q = hbseed.random_protein(41, 300)
+    res, offs = hbseed.make_db(42, 2000, plant_query=q, plant_frac=0.2)
+    spec = hbseed.blastp_query(q)
+    oq = oracle_lib.OracleQuery.from_spec(spec)
+    _, h = oq.scan(res, offs, threads=4)
+    assert len(h) > 20
+    assert oq.check_hsps(res, offs, h) == (0, len(h))
+    slen = (offs[1:] - offs[:-1]).astype(np.int64)
+    cases = []
+    a = h.copy(); a["score"][3] += 1; cases.append((a, 3))
+    a = h.copy(); a["s_start"][4] += 1; cases.append((a, 4))
+    a = h.copy(); a["len"][5] = slen[a["oid"][5]] - a["s_start"][5] + 1; cases.append((a, 5))
+    a = h.copy(); a["q_start"][6] = -1; cases.append((a, 6))
+    a = h.copy(); a[[7, 8]] = a[[8, 7]]; cases.append((a, 8))
+    for bad, idx in cases:
+        n, first = oq.check_hsps(res, offs, bad)
+        assert n >= 1 and first == idx, (idx, n, first)
+    hi = oracle_lib.OracleQuery(spec.seq, spec.ctx_off, spec.ctx_len, spec.xdrop,
+                                [int(h["score"].min()) + 1])
+    n, _ = hi.check_hsps(res, offs, h)
+    assert n == int((h["score"] < int(h["score"].min()) + 1).sum()) > 0
+
+
 def test_hsp_invariants():
     q = hbseed.random_protein(31, 300)
     res, offs = hbseed.make_db(32, 3000, plant_query=q, plant_frac=0.1)
