@@ -37,6 +37,8 @@ struct hb_ctx {
     std::mutex pin_mu;
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
+    // device address of this GPU's hb_scan_noalloc scratch
+    unsigned char *scratch_dev = nullptr;
 };
 
 // Copies device bytes to pageable host memory through the context's pinned

@@ -408,6 +408,32 @@ struct hb_scan {
     int place = HB_PLACE_LDS;      // where this scan keeps the diagonal state
 };
 
+// ------------------------------------------------------------------ no-alloc scan
+// Scratch of hb_scan_noalloc, one copy per GPU, held until the process ends:
+// the scan's HSP cursor and tile counter (the first 256 bytes), then the
+// diagonal words of a query whose state lives in global memory.  Host code
+// uses the device address hipGetSymbolAddress returns (hb_ctx::scratch_dev),
+// never the array's name.
+#define HB_NOALLOC_HEAD 256
+static __device__ __align__(256) unsigned char scratch[HB_NOALLOC_HEAD + HB_NOALLOC_STATE_BYTES];
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_dev[256];   // per device
+
+// The device address of this GPU's scratch (the device is current).
+static int scratch_address(int device, unsigned char **out)
+{
+    if (device < 0 || device >= 256) return HB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        HB_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_dev[device];
+    return HB_OK;
+}
+
+
 extern "C" {
 
 const char *hb_strerror(int err)
@@ -462,8 +488,10 @@ int hb_create(int device, hb_ctx **out)
             hipSuccess && (size_t)lds_optin > c->max_lds)
         c->max_lds = (size_t)lds_optin;
     // one dynamic-LDS limit for every launch of the scan kernels, so that
-    // launches of different sizes on different streams never disagree
-    if (hb_scan_allow_lds(c->max_lds) != hipSuccess) {
+    // launches of different sizes on different streams never disagree; the
+    // no-alloc scan's scratch address of this device
+    if (hb_scan_allow_lds(c->max_lds) != hipSuccess ||
+        scratch_address(device, &c->scratch_dev) != HB_OK) {
         hipStreamDestroy(c->stream);
         delete c;
         return HB_ERR_DEVICE;
@@ -971,6 +999,37 @@ int hb_query_lut_fetch(const hb_query *q, uint32_t *cell_start, uint32_t *entrie
 }
 
 // ------------------------------------------------------------------ scan
+// Kernel arguments of a plain scan of q over db with launch configuration
+// cfg and placement place (outputs, counters and state filled by the caller).
+static int plain_scan_args(const hb_db *db, const hb_query *q, int place,
+                           const hb_launch_cfg &cfg, hb_scan_args *a)
+{
+    memset(a, 0, sizeof(*a));
+    a->res = db->d_scan_res;   // subjects in processing order (a->subj's offsets)
+    a->offs = db->d_offs;
+    a->subj = db->d_subj;
+    a->tiles = db->d_tiles;
+    a->tile_begin = 0;
+    a->tile_end = db->n_tiles;
+    a->groups = q->gs.d_desc;
+    a->n_groups = 1;
+    a->qlen_pad_max = q->gs.qlen_pad_max;
+    a->ctx_pad_max = q->gs.ctx_pad_max;
+    a->diag_len_max = q->gs.diag_len_max;
+    a->wave_lds = (uint32_t)hb_scan_wave_lds(q->gs.diag_len_max, cfg.ring, hb_state_bytes(place));
+    a->ring = (uint32_t)cfg.ring;
+    a->state_limit = hb_state_limit(place, db->max_len);
+    if (q->reset_interval > 0) {
+        // a subject and its successor must fit below the interval
+        if ((uint64_t)q->reset_interval <= 2 * (db->max_len + (uint64_t)q->p.window) + 64)
+            return HB_ERR_UNSUPPORTED;
+        a->state_limit = std::min(a->state_limit, q->reset_interval - (int32_t)db->max_len);
+    }
+    a->mat32 = q->gs.d_mat;
+    a->window = q->p.window;
+    return HB_OK;
+}
+
 static int launch_scan(hb_scan *s)
 {
     hb_ctx *c = s->ctx;
@@ -990,30 +1049,9 @@ static int launch_scan(hb_scan *s)
         if (!s->d_gstate) return HB_ERR_NOMEM;
     }
     hb_scan_args a;
-    memset(&a, 0, sizeof(a));
-    a.res = db->d_scan_res;   // subjects in processing order (a.subj's offsets)
-    a.offs = db->d_offs;
-    a.subj = db->d_subj;
-    a.tiles = db->d_tiles;
-    a.tile_begin = 0;
-    a.tile_end = db->n_tiles;
-    a.groups = q->gs.d_desc;
-    a.n_groups = 1;
-    a.qlen_pad_max = q->gs.qlen_pad_max;
-    a.ctx_pad_max = q->gs.ctx_pad_max;
-    a.diag_len_max = q->gs.diag_len_max;
-    a.wave_lds = (uint32_t)hb_scan_wave_lds(q->gs.diag_len_max, cfg.ring, hb_state_bytes(place));
-    a.ring = (uint32_t)cfg.ring;
+    rc = plain_scan_args(db, q, place, cfg, &a);
+    if (rc) return rc;
     a.gstate = place == HB_PLACE_GLOBAL ? s->d_gstate : nullptr;   // cleared by each wavefront
-    a.state_limit = hb_state_limit(place, db->max_len);
-    if (q->reset_interval > 0) {
-        // a subject and its successor must fit below the interval
-        if ((uint64_t)q->reset_interval <= 2 * (db->max_len + (uint64_t)q->p.window) + 64)
-            return HB_ERR_UNSUPPORTED;
-        a.state_limit = std::min(a.state_limit, q->reset_interval - (int32_t)db->max_len);
-    }
-    a.mat32 = q->gs.d_mat;
-    a.window = q->p.window;
     a.work_counter = s->d_misc + 1;
     a.out = s->d_out;
     a.out_cap = s->cap;
@@ -1024,6 +1062,49 @@ static int launch_scan(hb_scan *s)
     HB_CHECK(hipEventRecord(s->ev0, c->stream));
     if (db->n_tiles) HB_CHECK(hb_launch_scan(a, false, cfg.blocks, cfg.waves * HB_WAVE, cfg.lds, c->stream));
     HB_CHECK(hipEventRecord(s->ev1, c->stream));
+    return HB_OK;
+}
+
+// ------------------------------------------------------------------ no-alloc scan
+int hb_scan_noalloc(hb_ctx *c, hb_db *db, hb_query *q, hb_hsp *d_hsps, uint32_t hsp_cap,
+                    uint32_t *d_counts, uint32_t *d_n_hsp, void *stream)
+{
+    if (!c || !db || !q || db->ctx != c || q->ctx != c || !d_n_hsp || (hsp_cap && !d_hsps))
+        return HB_ERR_ARG;
+    if (!c->scratch_dev) return HB_ERR_STATE;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HB_CHECK(hipSetDevice(c->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const int place = state_place(q, db);
+    if (place < 0) return HB_ERR_UNSUPPORTED;
+    hb_launch_cfg cfg;
+    int rc = hb_scan_config(c, false, q->gs.qlen_pad_max, q->gs.ctx_pad_max, q->gs.diag_len_max,
+                            db->n_tiles, &cfg, place);
+    if (rc) return rc;
+    hb_scan_args a;
+    rc = plain_scan_args(db, q, place, cfg, &a);
+    if (rc) return rc;
+    int blocks = cfg.blocks;
+    if (place == HB_PLACE_GLOBAL) {
+        // the state of every wavefront of the grid must fit the scratch:
+        // fewer workgroups for long queries, none fits -> refused
+        const size_t per_block = HB_GSTATE_WORD * (size_t)q->gs.diag_len_max * (size_t)cfg.waves;
+        if (per_block > HB_NOALLOC_STATE_BYTES) return HB_ERR_UNSUPPORTED;
+        blocks = std::min(blocks, (int)(HB_NOALLOC_STATE_BYTES / per_block));
+        a.gstate = reinterpret_cast<unsigned long long *>(c->scratch_dev + HB_NOALLOC_HEAD);
+    }
+    uint32_t *ctr = reinterpret_cast<uint32_t *>(c->scratch_dev);
+    a.out_count = ctr;
+    a.work_counter = ctr + 1;
+    a.out = d_hsps;
+    a.out_cap = hsp_cap;
+    a.stats = d_counts;
+    a.group_totals = nullptr;
+    // the previous call's counters stay in the scratch: cleared by a kernel
+    HB_CHECK(hb_launch_fill_u32(ctr, 0u, 2, st));
+    if (db->n_tiles)
+        HB_CHECK(hb_launch_scan(a, false, blocks, cfg.waves * HB_WAVE, cfg.lds, st));
+    HB_CHECK(hipMemcpyAsync(d_n_hsp, ctr, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
     return HB_OK;
 }
 

@@ -54,7 +54,7 @@ EXPORTS = [
     "hb_db_residues", "hb_query_prepare", "hb_query_free", "hb_query_lut_info",
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_query_prepare_masked", "hb_query_set_state_mode",
-    "hb_query_set_reset_interval",
+    "hb_query_set_reset_interval", "hb_scan_noalloc",
     "hb_query_state_mode", "hb_scan_state_mode", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
@@ -185,6 +185,7 @@ def _lib():
         L.hb_query_free.argtypes = [vp]
         L.hb_query_set_state_mode.argtypes = [vp, C.c_int]
         L.hb_query_set_reset_interval.argtypes = [vp, i32]
+        L.hb_scan_noalloc.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp]
         L.hb_query_state_mode.argtypes = [vp]
         L.hb_scan_state_mode.argtypes = [vp]
         L.hb_query_lut_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]

@@ -413,6 +413,30 @@ int hb_scan_free(hb_scan *s);
  * (see hb_query_set_state_mode). */
 int hb_scan_state_mode(const hb_scan *s);
 
+/* No counterpart in the reference: a plain scan that allocates nothing, for
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  It launches the same
+ * kernels as hb_scan_start on `stream` (a hipStream_t; NULL: the context's
+ * stream) and
+ * returns at once:
+ *   d_hsps    - device array of hsp_cap records: the scan's HSPs, unordered
+ *               (sort by (oid, s_off, q_off) for hb_scan_results' order);
+ *               records past hsp_cap are counted but not stored;
+ *   d_counts  - device array of 3 * n_subjects uint32, per oid {lookup hits,
+ *               extensions, HSPs}, or NULL;
+ *   d_n_hsp   - device uint32: the number of HSPs found (may exceed hsp_cap:
+ *               then run again with a larger array).
+ * Its counters, and the diagonal state of a query that keeps it in global
+ * memory (HB_NOALLOC_STATE_BYTES at most, with fewer workgroups for long
+ * queries), live in a fixed scratch array in device memory, one per GPU,
+ * held until the process ends; a query whose state does not fit one
+ * workgroup's share returns HB_ERR_UNSUPPORTED.  Every call on a GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (on different streams or
+ * threads) race and must not be issued. */
+#define HB_NOALLOC_STATE_BYTES (4u << 20)
+int hb_scan_noalloc(hb_ctx *ctx, hb_db *db, hb_query *q, hb_hsp *d_hsps, uint32_t hsp_cap,
+                    uint32_t *d_counts, uint32_t *d_n_hsp, void *stream);
+
 /* Convenience: prepare + start + results for one query. */
 int hb_search(hb_ctx *ctx, hb_db *db, const uint8_t *seq, int32_t len,
               int32_t n_ctx, const int32_t *ctx_off, const int32_t *ctx_len,
