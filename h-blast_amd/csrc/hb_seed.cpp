@@ -1065,6 +1065,158 @@ static int launch_scan(hb_scan *s)
     return HB_OK;
 }
 
+// ------------------------------------------------------------------ memory fit
+// Device bytes of a plain scan of q over n subjects with `residues`
+// residues (0: bounded by file_bytes), loaded from a .hDb of file_bytes
+// (0: packed arrays), subjects up to max_len residues:
+//   upload:   the file, the residues, lengths (4 B), offsets (8 B) and
+//             processing-order records (16 B) per subject, tiles;
+//   resident: the residues, their processing-order copy (spans padded to
+//             16 B) and the metadata;
+//   scan:     the HSP reservation (32 B records), counters (12 B per
+//             subject), the diagonal state in global memory if placed there;
+//   results:  per-subject stats (24 B), the ordered records, the layout's
+//             scratch (4 B per subject) and the list of large subjects.
+static uint64_t scan_need_bytes(const hb_ctx *c, const hb_query *q, uint64_t file_bytes,
+                                uint64_t residues, uint32_t n, uint64_t max_len)
+{
+    const uint64_t R = residues ? residues : file_bytes;
+    const uint64_t meta = 8ull * (n + 1) + 16ull * n + 4ull * (n / 64 + 2);
+    const uint64_t upload = file_bytes + (R + 64) + 4ull * n + meta;
+    const uint64_t resident = (R + 64) + (R + 16ull * n + 64) + meta;
+    const uint64_t cap = std::max<uint64_t>(1u << 18, n / 4);
+    uint64_t scan = 32 * cap + 12ull * n + 64;
+    int place = HB_PLACE_LDS;
+    switch (q->mode) {
+    case HB_STATE_GLOBAL: place = HB_PLACE_GLOBAL; break;
+    case HB_STATE_LDS: place = HB_PLACE_LDS; break;
+    default:
+        place = hb_state_choose(c, q->gs.diag_len_max,
+                                hb_scan_fixed_lds(q->gs.qlen_pad_max, q->gs.ctx_pad_max), max_len,
+                                q->p.window);
+    }
+    if (place == HB_PLACE_GLOBAL) scan += hb_gstate_bytes_max(c, q->gs.diag_len_max);
+    const uint64_t results = 24ull * n + 32 * cap + 4ull * (n + n / 4096 + 1) +
+                             4096 * sizeof(hb_fin_big);
+    return std::max(upload, resident + scan + results);
+}
+
+int hb_check_memory(hb_ctx *c, const hb_query *q, uint64_t file_bytes, uint64_t residues,
+                    uint32_t n_seqs, uint64_t max_len, int64_t budget, uint64_t *need,
+                    int32_t *fits)
+{
+    if (!c || !q || q->ctx != c || !need || !fits) return HB_ERR_ARG;
+    *need = scan_need_bytes(c, q, file_bytes, residues, n_seqs, max_len);
+    uint64_t room = (uint64_t)std::max<int64_t>(budget, 0);
+    if (budget < 0) {
+        uint64_t total = 0;
+        int rc = hb_mem_info(c, &room, &total);
+        if (rc) return rc;
+    }
+    *fits = *need <= room ? 1 : 0;
+    return HB_OK;
+}
+
+// ------------------------------------------------------------------ co-scheduled search
+// The library's device steps of the co-scheduled loop (hb_cosched.cpp).
+namespace {
+struct CoschedDevice {
+    hb_ctx *ctx = nullptr;
+    hb_query *q = nullptr;
+    int64_t budget = -1;
+    hb_gpu_results_fn gpu = nullptr;
+    void *user = nullptr;
+    hb_db *db = nullptr;
+    hb_scan *scan = nullptr;
+    std::vector<hb_subject_stats> stats;
+    std::vector<hb_hsp> hsps;
+
+    void drop()
+    {
+        if (scan) hb_scan_free(scan);
+        if (db) hb_db_free(db);
+        scan = nullptr;
+        db = nullptr;
+    }
+    static CoschedDevice *self(void *u) { return static_cast<CoschedDevice *>(u); }
+    static int fits(void *u, int32_t, const hb_hdb_share *sh, int32_t max_len, uint64_t *need,
+                    int32_t *ok)
+    {
+        CoschedDevice *d = self(u);
+        return hb_check_memory(d->ctx, d->q, sh->bytes, 0, (uint32_t)sh->gpu_seqs,
+                               (uint64_t)std::max(max_len, 0), d->budget, need, ok);
+    }
+    static int launch(void *u, int32_t, const uint8_t *hdb, const hb_hdb_share *sh,
+                      int32_t blocksx, int32_t threadsx, const int32_t *group_len)
+    {
+        CoschedDevice *d = self(u);
+        int rc = hb_db_upload_hdb(d->ctx, hdb, sh->bytes, blocksx, threadsx, sh->gpu_seqs,
+                                  group_len, sh->n_groups, &d->db);
+        if (rc) return rc;
+        rc = hb_scan_start(d->ctx, d->db, d->q, &d->scan);
+        if (rc) d->drop();
+        return rc;
+    }
+    static int fetch(void *u, int32_t volume, int64_t oid_base, double *gpu_seconds,
+                     uint64_t *n_hsp)
+    {
+        CoschedDevice *d = self(u);
+        uint64_t n = 0;
+        int rc = hb_scan_results(d->scan, nullptr, nullptr, 0, &n);
+        const uint32_t ns = d->db ? d->db->n : 0;
+        if (!rc) {
+            d->stats.resize(std::max<uint32_t>(ns, 1));
+            d->hsps.resize(std::max<uint64_t>(n, 1));
+            rc = hb_scan_results(d->scan, d->stats.data(), d->hsps.data(), n, &n);
+        }
+        float ms = 0.f;
+        if (!rc) rc = hb_scan_elapsed_ms(d->scan, &ms);
+        d->drop();
+        if (rc) return rc;
+        for (uint64_t i = 0; i < n; ++i) d->hsps[i].oid += (uint32_t)oid_base;
+        *gpu_seconds = ms / 1000.0;
+        *n_hsp = n;
+        if (d->gpu && d->gpu(d->user, volume, oid_base, d->stats.data(), ns, d->hsps.data(), n))
+            return HB_ERR_STATE;
+        return HB_OK;
+    }
+    static void abandon(void *u) { self(u)->drop(); }
+};
+}  // namespace
+
+int hb_cosearch(hb_ctx *c, const char *index_path, const uint8_t *seq, int32_t len,
+                int32_t n_ctx, const int32_t *ctx_off, const int32_t *ctx_len,
+                const int32_t *xdrop, const int32_t *cutoff, const hb_params *params,
+                const uint8_t *lut_mask, const hb_runtime_options *options,
+                int32_t cpu_threads, int32_t gapped, int64_t device_budget,
+                hb_cpu_search_fn cpu, hb_gpu_results_fn gpu, void *user,
+                hb_volume_report *reports, int32_t report_cap, int32_t *n_volumes)
+{
+    if (!c || !index_path || !cpu || !n_volumes) return HB_ERR_ARG;
+    hb_query *q = nullptr;
+    int rc = hb_query_prepare_masked(c, seq, len, n_ctx, ctx_off, ctx_len, xdrop, cutoff, params,
+                                     lut_mask, &q);
+    if (rc) return rc;
+    CoschedDevice dev;
+    dev.ctx = c;
+    dev.q = q;
+    dev.budget = device_budget;
+    dev.gpu = gpu;
+    dev.user = user;
+    hb_cosched_steps steps;
+    steps.user = &dev;
+    steps.fits = &CoschedDevice::fits;
+    steps.launch = &CoschedDevice::launch;
+    steps.fetch = &CoschedDevice::fetch;
+    steps.abandon = &CoschedDevice::abandon;
+    rc = hb_cosched_run(index_path, options, cpu_threads, gapped, len,
+                        (uint32_t)(len / std::max(n_ctx, 1)), &steps, cpu, user, reports,
+                        report_cap, n_volumes);
+    dev.drop();
+    hb_query_free(q);
+    return rc;
+}
+
 // ------------------------------------------------------------------ no-alloc scan
 int hb_scan_noalloc(hb_ctx *c, hb_db *db, hb_query *q, hb_hsp *d_hsps, uint32_t hsp_cap,
                     uint32_t *d_counts, uint32_t *d_n_hsp, void *stream)

@@ -29,6 +29,9 @@
 
 #include "hb_runtime.h"
 
+// Completed requests whose latencies the percentiles cover (the most recent).
+static constexpr size_t kLatencyWindow = 65536;
+
 namespace {
 
 using Clock = std::chrono::steady_clock;
@@ -73,7 +76,12 @@ struct hb_server {
     uint64_t next_ticket = 1;
 
     // statistics (under mu)
+    // latencies of the most recent completed requests (a ring of
+    // kLatencyWindow), for the percentiles; count, sum and max cover all
     std::vector<double> lat_ms;
+    size_t lat_next = 0;
+    uint64_t lat_count = 0;
+    double lat_sum = 0, lat_max = 0;
     hb_serve_stats st;
     Clock::time_point t_start;
 
@@ -115,7 +123,15 @@ static void finish_requests(hb_server *s, InFlight &f, int rc)
         r.state = 2;
         const double ms = std::chrono::duration<double, std::milli>(now - r.t_submit).count();
         if (r.rc == HB_OK) {
-            s->lat_ms.push_back(ms);
+            if (s->lat_ms.size() < kLatencyWindow) {
+                s->lat_ms.push_back(ms);
+            } else {
+                s->lat_ms[s->lat_next] = ms;
+                s->lat_next = (s->lat_next + 1) % kLatencyWindow;
+            }
+            s->lat_count++;
+            s->lat_sum += ms;
+            s->lat_max = std::max(s->lat_max, ms);
             s->st.completed++;
             s->st.query_residues += (uint64_t)r.residues;
         }
@@ -392,10 +408,8 @@ int hb_serve_stats_get(hb_server *s, hb_serve_stats *out)
         };
         out->p50_ms = pct(50);
         out->p99_ms = pct(99);
-        double sum = 0;
-        for (double x : v) sum += x;
-        out->mean_ms = sum / (double)v.size();
-        out->max_ms = v.back();
+        out->mean_ms = s->lat_sum / (double)s->lat_count;
+        out->max_ms = s->lat_max;
     }
     return HB_OK;
 }

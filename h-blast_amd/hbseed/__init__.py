@@ -54,7 +54,8 @@ EXPORTS = [
     "hb_db_residues", "hb_query_prepare", "hb_query_free", "hb_query_lut_info",
     "hb_query_lut_fetch", "hb_blastx_concat_len", "hb_blastx_translate",
     "hb_encode_protein", "hb_query_prepare_masked", "hb_query_set_state_mode",
-    "hb_query_set_reset_interval", "hb_scan_noalloc",
+    "hb_query_set_reset_interval", "hb_scan_noalloc", "hb_check_memory", "hb_cosched_run",
+    "hb_cosearch",
     "hb_query_state_mode", "hb_scan_state_mode", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
@@ -186,6 +187,12 @@ def _lib():
         L.hb_query_set_state_mode.argtypes = [vp, C.c_int]
         L.hb_query_set_reset_interval.argtypes = [vp, i32]
         L.hb_scan_noalloc.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp]
+        L.hb_check_memory.argtypes = [vp, vp, u64, u64, u32, u64, C.c_int64, C.POINTER(u64),
+                                      C.POINTER(i32)]
+        L.hb_cosched_run.argtypes = [C.c_char_p, vp, i32, i32, i32, u32, vp, vp, vp, vp, i32,
+                                     C.POINTER(i32)]
+        L.hb_cosearch.argtypes = [vp, C.c_char_p, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32,
+                                  i32, C.c_int64, vp, vp, vp, vp, i32, C.POINTER(i32)]
         L.hb_query_state_mode.argtypes = [vp]
         L.hb_scan_state_mode.argtypes = [vp]
         L.hb_query_lut_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
@@ -666,99 +673,159 @@ def balance_next_ratio(options, volume, ratio, cpu_threads, volume_bytes, ratio_
     return out.value
 
 
+class _VolumeReport(C.Structure):
+    _fields_ = [("oid_base", C.c_int64), ("volume_seqs", C.c_int32), ("gpu_seqs", C.c_int32),
+                ("ratio", C.c_float), ("on_cpu", C.c_int32), ("need_bytes", C.c_uint64),
+                ("gpu_seconds", C.c_double), ("cpu_seconds", C.c_double),
+                ("n_hsp", C.c_uint64), ("launch_seq", C.c_uint64), ("cpu_seq", C.c_uint64)]
+
+
+_CPU_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int64, C.c_int64)
+_GPU_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_uint32,
+                      C.c_void_p, C.c_uint64)
+_FITS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                       C.POINTER(C.c_uint64), C.POINTER(C.c_int32))
+_LAUNCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                         C.c_int32, C.c_void_p)
+_FETCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double),
+                        C.POINTER(C.c_uint64))
+_ABANDON_FN = C.CFUNCTYPE(None, C.c_void_p)
+
+
+class _CoschedSteps(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("fits", _FITS_FN), ("launch", _LAUNCH_FN),
+                ("fetch", _FETCH_FN), ("abandon", _ABANDON_FN)]
+
+
+def _report_dicts(reports, n):
+    return [{f: getattr(reports[i], f) for f, _ in _VolumeReport._fields_} for i in range(n)]
+
+
+class _Trap:
+    """Runs Python callbacks under C: the first exception is kept (the
+    callback returns 1, which stops the loop) and raised afterwards."""
+
+    def __init__(self):
+        self.exc = None
+
+    def call(self, fn, fail=1, always=False):
+        def wrapped(*a):
+            if self.exc is not None and not always:
+                return fail
+            try:
+                r = fn(*a)
+                return 0 if r is None else r
+            except BaseException as e:   # noqa: BLE001 - re-raised after the C call
+                self.exc = e
+                return fail
+        return wrapped
+
+    def raise_if(self):
+        if self.exc is not None:
+            raise self.exc
+
+
+def check_memory(ctx: Context, q, file_bytes, n_seqs, max_len, residues=0, budget=-1):
+    """Device bytes a plain scan of q over a database (or .hDb share) needs
+    and whether they fit (hb_check_memory): (need, fits)."""
+    need, fits = C.c_uint64(), C.c_int32()
+    _check(_lib().hb_check_memory(ctx.h, q.h, int(file_bytes), int(residues), int(n_seqs),
+                                  int(max_len), int(budget), C.byref(need), C.byref(fits)),
+           "hb_check_memory")
+    return need.value, bool(fits.value)
+
+
+def cosched_run(index_path, device, cpu_search, cpu_threads, options=None, gapped=False,
+                query_length=0, avg_query_length=0):
+    """The co-scheduled volume loop (hb_cosched_run) with Python device
+    steps: device.fits(volume, share, max_len) -> (need, fits);
+    device.launch(volume, data, share, blocksx, threadsx, group_len) -> 0 or
+    an HB_ERR code; device.fetch(volume, oid_base) -> (gpu_seconds, n_hsp);
+    device.abandon().  share = dict(gpu_seqs, n_groups, bytes).  Returns one
+    report dict per volume."""
+    trap = _Trap()
+
+    def share_of(p):
+        sh = C.cast(p, C.POINTER(_HdbShare)).contents
+        return dict(gpu_seqs=sh.gpu_seqs, n_groups=sh.n_groups, bytes=sh.bytes)
+
+    def fits(_u, v, sh, max_len, need, ok):
+        n, f = device.fits(v, share_of(sh), max_len)
+        need[0], ok[0] = int(n), int(bool(f))
+
+    def launch(_u, v, hdb, sh, bx, tx, gl):
+        share = share_of(sh)
+        data = C.string_at(hdb, share["bytes"])
+        glen = np.ctypeslib.as_array(C.cast(gl, C.POINTER(C.c_int32)),
+                                     shape=(max(share["n_groups"], 1),))[:share["n_groups"]].copy()
+        return device.launch(v, data, share, bx, tx, glen)
+
+    def fetch(_u, v, oid_base, secs, nh):
+        g, n = device.fetch(v, oid_base)
+        secs[0], nh[0] = float(g), int(n)
+
+    def abandon(_u):
+        device.abandon()
+
+    steps = _CoschedSteps(None, _FITS_FN(trap.call(fits)), _LAUNCH_FN(trap.call(launch)),
+                          _FETCH_FN(trap.call(fetch)), _ABANDON_FN(trap.call(abandon, None, always=True)))
+    cpu = _CPU_FN(trap.call(lambda _u, v, lo, hi: cpu_search(v, lo, hi)))
+    opts = _RuntimeOptions(**options) if options is not None else None
+    reports = (_VolumeReport * 1024)()
+    n = C.c_int32()
+    rc = _lib().hb_cosched_run(os.fsencode(index_path), None if opts is None else C.byref(opts),
+                               cpu_threads, int(bool(gapped)), int(query_length),
+                               int(avg_query_length), C.byref(steps), cpu, None, reports, 1024,
+                               C.byref(n))
+    trap.raise_if()
+    _check(rc, "hb_cosched_run")
+    return _report_dicts(reports, n.value)
+
+
 def cosearch(ctx: Context, index_path, spec: QuerySpec, cpu_search, cpu_threads, options=None,
-             gapped=False, device_budget=None, **query_kw):
+             gapped=False, device_budget=None, threshold=11, window=40, word_size=3,
+             matrix=None):
     """Searches every volume of an .hDbI on the GPU and with the caller's CPU
-    work side by side, in the order GPU_BLAST_PreliminarySearchEngine
-    (ncbi_blast_files/functions.cpp:1409-1520) pipelines them: the device
-    runs volume v + 1's share while cpu_search(v, lo, hi) searches database
-    oids [lo, hi), the rest of volume v.  Each share is hdb_split's prefix of
-    the volume's .hDb; from the second volume on, an adaptive ratio follows
-    balance_next_ratio from the device time of the last share and the wall
-    time of the last cpu_search.  A share that needs more device memory
-    than device_budget bytes (default: the device's free memory), or whose
-    upload runs out of device memory, is not scanned: cpu_search gets the
-    whole volume, as the reference sends a volume to the CPU when
-    H_BLAST_check_memory fails (functions.cpp:1518-1530, 1584-1585).
-    Returns one dict per volume: oid_base, gpu_seqs, ratio, on_cpu,
-    gpu_seconds, cpu_seconds, stats and hsps (database oids) of the GPU
-    share."""
-    import time
-    with open(index_path, "rb") as f:
-        idx = hdbi_parse(f.read())
-    d = os.path.dirname(index_path)
-    vols, bx, tx = idx["volumes"], idx["blocksx"], idx["threadsx"]
-    opts = runtime_options() if options is None else dict(options)
-    ratio, fixed = opts["gpu_ratio"], opts["fixed_ratio"]
-    ring_bytes, ring_ratio, ring_hsp = [0, 0, 0], [opts["gpu_ratio"], 0.0], [0, 0]
-    seq = np.asarray(spec.seq)
-    qlen, avg_q = int(seq.size), int(seq.size) // spec.n_ctx
-    out = [None] * len(vols)
-    q = Query(ctx, spec, **query_kw)
-    running = None   # (volume, Database, Scan) on the device
-    try:
-        for v in range(-1, len(vols)):
-            if v >= 0:
-                rec = out[v]
-                if running is not None:
-                    _, db, scan = running
-                    stats, hsps = scan.results()
-                    rec["gpu_seconds"] = scan.elapsed_ms() / 1000.0
-                    hsps["oid"] += np.uint32(rec["oid_base"])
-                    rec["stats"], rec["hsps"] = stats, hsps
-                    scan.close()
-                    db.close()
-                    running = None
-                else:
-                    rec["gpu_seconds"] = 0.0
-                    rec["stats"] = np.zeros(0, STATS_DTYPE)
-                    rec["hsps"] = np.zeros(0, HSP_DTYPE)
-                ring_hsp[v & 1] = int(rec["hsps"].size)
-            if v + 1 < len(vols):
-                nv, vol = v + 1, vols[v + 1]
-                ring_bytes[nv % 3] = int(vol["bytes"]) + int(vol["cpu_rest_bytes"])
-                if v > 0 and fixed != 1:
-                    ratio = balance_next_ratio(opts, v, ratio, cpu_threads, ring_bytes, ring_ratio,
-                                               ring_hsp, out[v]["gpu_seconds"],
-                                               out[v - 1]["cpu_seconds"], gapped, qlen, avg_q)
-                share, ratio, fixed = hdb_split(vol["group_len"], vol["gpu_seqs"], vol["bytes"],
-                                                bx, tx, cpu_threads, ratio, fixed)
-                ring_ratio[nv & 1] = ratio
-                rec = dict(oid_base=vol["break_limit"] - vol["volume_seqs"],
-                           volume_seqs=vol["volume_seqs"], gpu_seqs=share["gpu_seqs"],
-                           ratio=ratio, on_cpu=False)
-                out[nv] = rec
-                # the unpacked share, its file bytes in flight, per-subject
-                # metadata and counters, and room for HSPs
-                need = 2 * share["bytes"] + 64 * share["gpu_seqs"]
-                budget = ctx.mem_info()[0] if device_budget is None else device_budget
-                if share["gpu_seqs"] > 0 and need > budget:
-                    rec["gpu_seqs"], rec["on_cpu"] = 0, True
-                if rec["gpu_seqs"] > 0:
-                    with open(os.path.join(d, vol["name"]), "rb") as f:
-                        data = f.read(share["bytes"])
-                    db = None
-                    try:
-                        db = Database.from_hdb(ctx, data, bx, tx, share["gpu_seqs"],
-                                               vol["group_len"][:share["n_groups"]])
-                        running = (nv, db, Scan(ctx, db, q))
-                    except HBError as e:
-                        if db is not None:
-                            db.close()
-                        if e.code != HB_ERR_NOMEM:
-                            raise
-                        rec["gpu_seqs"], rec["on_cpu"] = 0, True
-            if v >= 0:
-                rec = out[v]
-                t0 = time.perf_counter()
-                cpu_search(v, rec["oid_base"] + rec["gpu_seqs"],
-                           rec["oid_base"] + rec["volume_seqs"])
-                rec["cpu_seconds"] = time.perf_counter() - t0
-    finally:
-        if running is not None:
-            running[2].close()
-            running[1].close()
-        q.close()
+    work side by side (hb_cosearch, the order of
+    GPU_BLAST_PreliminarySearchEngine, ncbi_blast_files/functions.cpp:
+    1409-1790): the device runs volume v + 1's share while cpu_search(v, lo,
+    hi) searches database oids [lo, hi), the rest of volume v.  A share over
+    device_budget bytes (default: the device's free memory), or whose
+    upload runs out of device memory, goes to cpu_search whole.  Returns one
+    dict per volume (hb_volume_report fields, plus stats and hsps of the
+    device share with database oids)."""
+    trap = _Trap()
+    got = {}
+
+    def gpu(_u, v, oid_base, st, ns, h, nh):
+        stats = np.zeros(ns, STATS_DTYPE)
+        hsps = np.zeros(nh, HSP_DTYPE)
+        if ns:
+            C.memmove(stats.ctypes.data, st, ns * STATS_DTYPE.itemsize)
+        if nh:
+            C.memmove(hsps.ctypes.data, h, nh * HSP_DTYPE.itemsize)
+        got[v] = (stats, hsps)
+
+    cpu = _CPU_FN(trap.call(lambda _u, v, lo, hi: cpu_search(v, lo, hi)))
+    gfn = _GPU_FN(trap.call(gpu))
+    p, keep = _params_struct(threshold, window, word_size, matrix)
+    opts = _RuntimeOptions(**options) if options is not None else None
+    reports = (_VolumeReport * 1024)()
+    n = C.c_int32()
+    mask = getattr(spec, "lut_mask", None)
+    rc = _lib().hb_cosearch(ctx.h, os.fsencode(index_path), _ptr(spec.seq), len(spec.seq),
+                            spec.n_ctx, _ptr(spec.ctx_off), _ptr(spec.ctx_len), _ptr(spec.xdrop),
+                            _ptr(spec.cutoff), C.byref(p),
+                            None if mask is None else _ptr(mask),
+                            None if opts is None else C.byref(opts), cpu_threads,
+                            int(bool(gapped)), -1 if device_budget is None else int(device_budget),
+                            cpu, gfn, None, reports, 1024, C.byref(n))
+    del keep
+    trap.raise_if()
+    _check(rc, "hb_cosearch")
+    out = _report_dicts(reports, n.value)
+    for v, rec in enumerate(out):
+        rec["stats"], rec["hsps"] = got.get(v, (np.zeros(0, STATS_DTYPE), np.zeros(0, HSP_DTYPE)))
     return out
 
 

@@ -369,6 +369,96 @@ typedef struct hb_balance_in {
 } hb_balance_in;
 int hb_balance_next_ratio(const hb_runtime_options *o, const hb_balance_in *in, float *ratio);
 
+/* --- memory fit and the co-scheduled volume loop -------------------------
+ * Device bytes a plain scan of q over a database (or a .hDb volume share)
+ * needs, and whether they fit: replaces H_BLAST_check_memory
+ * (H-BLAST/h_blast.cu:1280-1349, declared at gpu_cpu_common.h:93-102), which
+ * sums the volume, lookup table, diagonal arrays and result buffers against
+ * the device's memory.  Here: the upload (file_bytes of a .hDb share, then
+ * its residues; file_bytes = 0 for packed arrays) with the per-sequence
+ * metadata, the scan's copy of the residues, the HSP reservation, the
+ * per-subject counters and the result layout, and the diagonal state if q
+ * keeps it in global memory for subjects up to max_len.  residues = 0 takes
+ * file_bytes as its bound.  budget < 0 takes the device's free memory.
+ * *fits = 1 if *need <= the budget.  (The lookup table of q is already on
+ * the device.) */
+int hb_check_memory(hb_ctx *ctx, const hb_query *q, uint64_t file_bytes, uint64_t residues,
+                    uint32_t n_seqs, uint64_t max_len, int64_t budget, uint64_t *need,
+                    int32_t *fits);
+
+/* One volume of the co-scheduled loop (GPU_BLAST_PreliminarySearchEngine,
+ * ncbi_blast_files/functions.cpp:1409-1790). */
+typedef struct hb_volume_report {
+    int64_t oid_base;         /* database oid of the volume's first sequence */
+    int32_t volume_seqs;      /* sequences in the volume */
+    int32_t gpu_seqs;         /* the device's share (a prefix of the .hDb); 0 if none */
+    float ratio;              /* the workload ratio the split used */
+    int32_t on_cpu;           /* 1: the share did not fit the device, the CPU searched it all */
+    uint64_t need_bytes;      /* device bytes the share needed (hb_check_memory) */
+    double gpu_seconds;       /* device time of the share's scan */
+    double cpu_seconds;       /* wall time of the CPU search of the rest */
+    uint64_t n_hsp;           /* HSPs of the device share */
+    uint64_t launch_seq;      /* order of the share's launch among the loop's launches and
+                                 CPU searches (0: not launched) */
+    uint64_t cpu_seq;         /* order of the volume's CPU search */
+} hb_volume_report;
+
+/* The caller's CPU side (NCBI's WordFinder threads, blast_engine.c:555-559,
+ * over functions.cpp:1590-1768): search database oids [oid_lo, oid_hi) of
+ * volume `volume`; nonzero stops the loop. */
+typedef int (*hb_cpu_search_fn)(void *user, int32_t volume, int64_t oid_lo, int64_t oid_hi);
+/* The device share's results of one volume, handed over before that
+ * volume's CPU search: per-subject stats of its n_subjects sequences and its
+ * HSPs, whose oid fields are database oids; nonzero stops the loop. */
+typedef int (*hb_gpu_results_fn)(void *user, int32_t volume, int64_t oid_base,
+                                 const hb_subject_stats *stats, uint32_t n_subjects,
+                                 const hb_hsp *hsps, uint64_t n_hsp);
+/* Device steps of the loop.  hb_cosearch binds the library's own (upload
+ * with hb_db_upload_hdb, hb_scan_start, hb_scan_results); another binding
+ * can drive the same loop (the tests drive it on the CPU).  fits: device
+ * bytes the share needs and whether they fit; launch: upload the share (the
+ * .hDb prefix in hdb) and start its scan, HB_ERR_NOMEM to send the volume to
+ * the CPU; fetch: wait for the started share, hand its results on, return
+ * its device seconds and HSP count; abandon: drop a started share after an
+ * error. */
+typedef struct hb_cosched_steps {
+    void *user;
+    int (*fits)(void *user, int32_t volume, const hb_hdb_share *share, int32_t max_len,
+                uint64_t *need, int32_t *fits);
+    int (*launch)(void *user, int32_t volume, const uint8_t *hdb, const hb_hdb_share *share,
+                  int32_t blocksx, int32_t threadsx, const int32_t *group_len);
+    int (*fetch)(void *user, int32_t volume, int64_t oid_base, double *gpu_seconds,
+                 uint64_t *n_hsp);
+    void (*abandon)(void *user);
+} hb_cosched_steps;
+/* Runs the loop over the volumes of the .hDbI at index_path (volume files
+ * beside it): for v = 0, 1, ...: fetch volume v's device results, update the
+ * ratio (hb_balance_next_ratio, from the second volume on unless fixed),
+ * split volume v + 1 (hb_hdb_split), check it fits (else the CPU takes it
+ * whole) and launch it, then run the CPU search of volume v's rest.
+ * options NULL = the defaults; reports receives one entry per volume
+ * (report_cap >= the volume count, else HB_ERR_CAPACITY with *n_volumes
+ * set).  A callback's nonzero return stops the loop with HB_ERR_STATE. */
+int hb_cosched_run(const char *index_path, const hb_runtime_options *options,
+                   int32_t cpu_threads, int32_t gapped, int32_t query_length,
+                   uint32_t avg_query_length, const hb_cosched_steps *steps,
+                   hb_cpu_search_fn cpu, void *cpu_user, hb_volume_report *reports,
+                   int32_t report_cap, int32_t *n_volumes);
+/* The co-scheduled search of one query (as hb_query_prepare_masked takes it) over
+ * the volumes of an .hDbI: the device scans each volume's share (uploaded
+ * with hb_db_upload_hdb) while cpu(user, v, lo, hi) searches the rest of
+ * volume v - 1, as H-BLAST pipelines them; gpu(user, ...) receives each
+ * share's results.  device_budget: bytes a share may use on the device (< 0:
+ * the free memory at each share); a share over it, or whose upload runs out
+ * of memory, goes to the CPU whole. */
+int hb_cosearch(hb_ctx *ctx, const char *index_path, const uint8_t *seq, int32_t len,
+                int32_t n_ctx, const int32_t *ctx_off, const int32_t *ctx_len,
+                const int32_t *xdrop, const int32_t *cutoff, const hb_params *params,
+                const uint8_t *lut_mask, const hb_runtime_options *options,
+                int32_t cpu_threads, int32_t gapped, int64_t device_budget,
+                hb_cpu_search_fn cpu, hb_gpu_results_fn gpu, void *user,
+                hb_volume_report *reports, int32_t report_cap, int32_t *n_volumes);
+
 /* --- SEG low-complexity masking of a query ------------------------------
  * NCBI BLAST+ masks BLASTX queries (and BLASTP queries with -seg) with SEG
  * before the lookup table is built (-seg 12 2.2 2.5,
@@ -595,7 +685,9 @@ typedef struct hb_serve_stats {
     uint64_t peak_bytes;         /* largest admitted device footprint */
     int32_t peak_inflight;       /* most batches on the device at once */
     int32_t reruns;              /* batches rerun because their output overflowed */
-    double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> results ready, completed queries */
+    double p50_ms, p99_ms;       /* submit -> results ready, over the most recent 65,536
+                                    completed queries */
+    double mean_ms, max_ms;      /* ... over all completed queries */
     double gpu_ms;               /* sum of batch kernel times */
     double wall_ms;              /* since start */
 } hb_serve_stats;

@@ -1,9 +1,11 @@
-"""Host logic of the co-scheduled volume loop (hbseed.cosearch), no GPU: the
-device-side classes are replaced by fakes that unpack a share with the
-oracle's .hDb reader and scan it with the oracle, and report a device time
-the test chooses.  Checks the order of the loop (volume v + 1's share is
-launched before volume v's CPU search), the shares and ratios against a
-replay through the oracle's balancer, and the device-budget route.
+"""Host logic of the co-scheduled volume loop (hb_cosched_run, the C++ loop
+behind hb_cosearch), no GPU: the device steps are fakes that unpack a share
+with the oracle's .hDb reader, scan it with the oracle, report a device
+time the test chooses and a stand-in memory need.  Checks the order of the
+loop (volume v + 1's share is launched before volume v's CPU search, from
+the callbacks and from the reports' sequence numbers), the shares and
+ratios against a replay through the oracle's balancer, the device-memory
+route and that a callback's error stops the loop.
 """
 import numpy as np
 import pytest
@@ -16,58 +18,45 @@ SIZES = [3000, 2000, 2500, 2500]
 BX, TX = 4, 64
 
 
-class FakeCtx:
-    def __init__(self, free):
-        self.free = free
+class FakeDevice:
+    def __init__(self, spec, gpu_ms, events, budget=1 << 60, fail_launch=None):
+        self.oq = O.OracleQuery.from_spec(spec)
+        self.gpu_ms, self.events, self.budget = gpu_ms, events, budget
+        self.fail_launch = fail_launch
+        self.share = None
+        self.results = {}
 
-    def mem_info(self):
-        return self.free, self.free
+    @staticmethod
+    def need(share):
+        return 2 * share["bytes"] + 64 * share["gpu_seqs"]
 
+    def fits(self, v, share, max_len):
+        n = self.need(share)
+        return n, n <= self.budget
 
-def fakes(monkeypatch, events, gpu_ms):
-    class DB:
-        def __init__(self, r, o, n):
-            self.r, self.o, self.n = r, o, n
+    def launch(self, v, data, share, bx, tx, gl):
+        assert (bx, tx) == (BX, TX) and len(data) == share["bytes"]
+        if self.fail_launch == v:
+            return -2   # HB_ERR_NOMEM: the volume goes to the CPU
+        self.events.append(("launch", v))
+        r, o = O.hdb_unpack(np.frombuffer(data, np.uint8), share["gpu_seqs"], bx, tx,
+                            np.asarray(gl))
+        self.share = (v, r, o, share["gpu_seqs"])
+        return 0
 
-        @classmethod
-        def from_hdb(cls, ctx, data, bx, tx, n, gl):
-            r, o = O.hdb_unpack(np.frombuffer(data, np.uint8), n, bx, tx, np.asarray(gl))
-            return cls(r, o, n)
+    def fetch(self, v, oid_base):
+        sv, r, o, n = self.share
+        assert sv == v
+        self.events.append(("fetch", v))
+        cnt, h = self.oq.scan(r, o, threads=2)
+        h["oid"] += np.uint32(oid_base)
+        self.results[v] = (cnt, h)
+        self.share = None
+        return self.gpu_ms[v] / 1000.0, len(h)
 
-        def close(self):
-            pass
-
-    class Query:
-        def __init__(self, ctx, spec, **kw):
-            self.oq = O.OracleQuery.from_spec(spec)
-
-        def close(self):
-            pass
-
-    class Scan:
-        count = 0
-
-        def __init__(self, ctx, db, q):
-            self.db, self.q = db, q
-            self.k = Scan.count
-            Scan.count += 1
-            events.append(("launch", self.k))
-
-        def results(self):
-            cnt, h = self.q.oq.scan(self.db.r, self.db.o, threads=2)
-            st = np.zeros(self.db.n, hbseed.STATS_DTYPE)
-            st["lookup_hits"], st["extensions"], st["n_hsp"] = cnt[:, 0], cnt[:, 1], cnt[:, 2]
-            return st, h
-
-        def elapsed_ms(self):
-            return gpu_ms[self.k]
-
-        def close(self):
-            pass
-
-    monkeypatch.setattr(hbseed, "Database", DB)
-    monkeypatch.setattr(hbseed, "Query", Query)
-    monkeypatch.setattr(hbseed, "Scan", Scan)
+    def abandon(self):
+        self.events.append(("abandon", -1))
+        self.share = None
 
 
 @pytest.fixture(scope="module")
@@ -89,10 +78,10 @@ def volumes(tmp_path_factory):
     return dict(res=res, offs=offs, index=prefix + ".hDbI", idx=idx, spec=spec, ref=ref)
 
 
-def run(v, monkeypatch, gpu_ms, cpu_delay, free=1 << 40, **kw):
+def run(v, gpu_ms, cpu_delay, budget=1 << 60, fail_launch=None, **kw):
     import time
     events, parts = [], {}
-    fakes(monkeypatch, events, gpu_ms)
+    dev = FakeDevice(v["spec"], gpu_ms, events, budget, fail_launch)
     res, offs = v["res"], v["offs"]
     oq = O.OracleQuery.from_spec(v["spec"])
 
@@ -103,21 +92,32 @@ def run(v, monkeypatch, gpu_ms, cpu_delay, free=1 << 40, **kw):
         parts[k] = (lo, hi, h)
         time.sleep(cpu_delay[k])
 
-    out = hbseed.cosearch(FakeCtx(free), v["index"], v["spec"], cpu_search, 4, **kw)
-    got = np.concatenate(sum(([r["hsps"], parts[k][2]] for k, r in enumerate(out)), []))
+    n = int(np.asarray(v["spec"].seq).size)
+    out = hbseed.cosched_run(v["index"], dev, cpu_search, 4, query_length=n,
+                             avg_query_length=n, **kw)
+    pieces = []
+    for k, r in enumerate(out):
+        if r["gpu_seqs"]:
+            pieces.append(dev.results[k][1])
+        lo, hi, h = parts[k]
+        assert (lo, hi) == (r["oid_base"] + r["gpu_seqs"], r["oid_base"] + r["volume_seqs"])
+        pieces.append(h)
+    got = np.concatenate(pieces)
     assert np.array_equal(got.view(np.int32), v["ref"].view(np.int32))
     return out, events
 
 
-def test_loop_order_and_balancer_replay(volumes, monkeypatch):
+def test_loop_order_and_balancer_replay(volumes):
     # device 1 ms per share; the CPU 40 ms on volume 0 and 1 ms later: the
     # ratio rises after the slow CPU volume and falls back after fast ones
-    out, events = run(volumes, monkeypatch, [1.0] * 4, [0.04, 0.001, 0.001, 0.001])
-    launches = [k for e, k in events if e == "launch"]
-    assert launches == [0, 1, 2, 3]
-    # share v + 1 is launched before the CPU search of volume v
-    assert events.index(("launch", 1)) < events.index(("cpu", 0))
-    assert events.index(("launch", 3)) < events.index(("cpu", 2))
+    out, events = run(volumes, [1.0] * 4, [0.04, 0.001, 0.001, 0.001])
+    assert [k for e, k in events if e == "launch"] == [0, 1, 2, 3]
+    # share v + 1 is launched before the CPU search of volume v, and volume
+    # v's results are fetched before it
+    for v in range(3):
+        assert events.index(("launch", v + 1)) < events.index(("cpu", v))
+        assert events.index(("fetch", v)) < events.index(("launch", v + 1))
+        assert out[v + 1]["launch_seq"] < out[v]["cpu_seq"]
     o = O.runtime_options()
     ratio, fixed = o["gpu_ratio"], o["fixed_ratio"]
     rb, rr, rh = [0, 0, 0], [o["gpu_ratio"], 0.0], [0, 0]
@@ -125,7 +125,7 @@ def test_loop_order_and_balancer_replay(volumes, monkeypatch):
     n = int(np.asarray(volumes["spec"].seq).size)
     for v in range(-1, len(vols)):
         if v >= 0:
-            rh[v & 1] = int(out[v]["hsps"].size)
+            rh[v & 1] = int(out[v]["n_hsp"])
         if v + 1 < len(vols):
             nv, vol = v + 1, vols[v + 1]
             rb[nv % 3] = int(vol["bytes"]) + int(vol["cpu_rest_bytes"])
@@ -140,17 +140,32 @@ def test_loop_order_and_balancer_replay(volumes, monkeypatch):
     assert out[2]["ratio"] > out[1]["ratio"]                 # the slow CPU volume moved it up
 
 
-def test_device_budget_routes_whole_volumes_to_the_cpu(volumes, monkeypatch):
+def test_device_budget_routes_whole_volumes_to_the_cpu(volumes):
     vols = volumes["idx"]["volumes"]
     opts = dict(hbseed.runtime_options(), gpu_ratio=1.0, fixed_ratio=1)
-    need = [2 * v["bytes"] + 64 * v["gpu_seqs"] for v in vols]
+    need = [FakeDevice.need(dict(bytes=v["bytes"], gpu_seqs=v["gpu_seqs"])) for v in vols]
     limit = sorted(need)[1]          # the two smallest shares fit
-    out, events = run(volumes, monkeypatch, [1.0] * 4, [0.0] * 4, options=opts,
-                      device_budget=limit)
+    out, events = run(volumes, [1.0] * 4, [0.0] * 4, budget=limit, options=opts)
     for rec, nd, vol in zip(out, need, vols):
         assert rec["on_cpu"] == (nd > limit)
         assert rec["gpu_seqs"] == (0 if nd > limit else vol["gpu_seqs"])
+        assert rec["need_bytes"] == nd
     assert len([e for e in events if e[0] == "launch"]) == 2
-    # the default budget is the device's free memory
-    out, _ = run(volumes, monkeypatch, [1.0] * 4, [0.0] * 4, free=0, options=opts)
+    out, _ = run(volumes, [1.0] * 4, [0.0] * 4, budget=0, options=opts)
     assert all(r["on_cpu"] for r in out)
+    # an upload that runs out of memory sends its volume to the CPU too
+    out, events = run(volumes, [1.0] * 4, [0.0] * 4, options=opts, fail_launch=2)
+    assert out[2]["on_cpu"] and out[2]["gpu_seqs"] == 0
+    assert [k for e, k in events if e == "launch"] == [0, 1, 3]
+
+
+def test_a_failing_callback_stops_the_loop(volumes):
+    dev = FakeDevice(volumes["spec"], [1.0] * 4, [])
+
+    def cpu_search(k, lo, hi):
+        if k == 1:
+            raise KeyError("stop here")
+
+    with pytest.raises(KeyError):
+        hbseed.cosched_run(volumes["index"], dev, cpu_search, 4)
+    assert ("abandon", -1) in dev.events   # the share started for volume 2 was dropped

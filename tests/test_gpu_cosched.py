@@ -1,4 +1,4 @@
-"""GPU parity of the co-scheduled volume loop (hbseed.cosearch, the order of
+"""GPU parity of the co-scheduled volume loop (hb_cosearch, the order of
 GPU_BLAST_PreliminarySearchEngine, ncbi_blast_files/functions.cpp:1409-1520):
 each volume's GPU share, a prefix of its .hDb cut by hb_hdb_split, is scanned
 on the device while the rest of the volume is searched on the host -- here
@@ -150,12 +150,23 @@ def test_volume_over_the_device_budget_goes_to_the_cpu(setup):
     s = setup
     vols = s["idx"]["volumes"]
     opts = dict(hbseed.runtime_options(), gpu_ratio=1.0, fixed_ratio=1)
-    need = [2 * v["bytes"] + 64 * v["gpu_seqs"] for v in vols]
+    # each whole share's need as the library computes it (hb_check_memory)
+    q = hbseed.Query(s["ctx"], s["spec"])
+    try:
+        need = [hbseed.check_memory(s["ctx"], q, v["bytes"], v["gpu_seqs"],
+                                    int(v["group_len"][-1]))[0] for v in vols]
+        # the estimate bounds what a load and scan of the share hold: file,
+        # residues twice, 24 B of metadata and 12 B of counters per sequence
+        for v, nd in zip(vols, need):
+            assert nd >= 2 * v["bytes"] + 36 * v["gpu_seqs"]
+    finally:
+        q.close()
     big = int(np.argmax(need))
     assert sorted(need)[-2] < need[big]
     out, cpu_parts = run(s, 16, options=opts, device_budget=need[big] - 1)
     check_union(s, out, cpu_parts)
     for k, (rec, vol) in enumerate(zip(out, vols)):
+        assert rec["need_bytes"] == need[k]
         assert rec["on_cpu"] == (k == big)
         assert rec["gpu_seqs"] == (0 if k == big else vol["gpu_seqs"])
     # and with no room at all, every volume is searched on the CPU
