@@ -9,14 +9,25 @@ resident in HBM).  Word 3, T 11, window 40, X-drop 16, cutoff 41.
 
 One step = one pass of the stage over the whole database for the query
 (hb_scan_start + hb_scan_wait: lookup, two-hit tracking, extensions, HSP
-records and per-subject counters in HBM).  The same line carries the
-preliminary gapped extension of the last pass's HSPs ("gapped": device
-kernel time, extensions, saved HSPs, the oracle on a bounded sample;
---no-gapped skips it) and a compact batched-vs-plain and served measurement
-on the same database ("forms"; --no-forms skips it), and the database
-loaded from an H-BLAST .hDb volume ("hdb_load"; --no-hdb skips it).
---config c3 / c4 run the full batched (2,048 queries) and served
-(20,000,000 subjects) workloads instead.  Prints one
+records and per-subject counters in HBM).  `value` = subject residues/s of
+those passes.  The same line carries:
+* dropin: prepare + scan + results per query (the H_BLAST_pre +
+  H_BLAST_get_data pair), queries/s;
+* blastx: the 1,500-nt six-frame BLASTX query against the same database,
+  with its own roofline;
+* gapped: the preliminary gapped extension of the last pass's HSPs (device
+  kernel time, extensions, saved HSPs, the oracle on a bounded sample);
+* hdb_load / db_upload: the database loaded from an H-BLAST .hDb volume and
+  from packed arrays;
+* forms.batched: all 2,048 batched queries in one autotuned batched scan
+  (the knobs it chose) against the plain per-query loop;
+* forms.served: an offered-load sweep of the served stream (80% BLASTP,
+  20% BLASTX) over a 20,000,000-subject database resident in HBM, up to
+  saturation: completed queries/s, p50 and p99 per point;
+* roofline.traffic and valu_issue: profile-derived figures (rocprofv3 PMC
+  runs of the same kernel, files named in the line).
+--no-forms / --no-gapped / --no-hdb / --no-cpu skip parts; --config c3 / c4
+/ qlen run the batched, served and long-query workloads alone.  Prints one
 JSON line.
 """
 import argparse
@@ -252,6 +263,12 @@ def measure_batched(ctx, db, qs, steps, warmup, plain_sample, group_residues=0):
             "ms_per_step": wall / steps * 1e3, "kernel_ms": float(np.mean(kms)),
             "first_run_ms": first_ms, "build_ms": info["build_ms"],
             "group_residues": info["group_residues"], "groups": info["n_groups"],
+            "autotuned": {"group_residues": info["group_residues"],
+                          "waves_per_workgroup": info["waves_per_workgroup"],
+                          "ring_bytes": info["ring_bytes"],
+                          "items_per_workgroup": info["items_per_workgroup"],
+                          "candidates_timed": info["tune_candidates"]} if group_residues == 0
+            else None,
             "launches": info["n_classes"], "hsps": info["hsps"],
             "lookup_hits": info["lookup_hits"], "extensions": info["extensions"]}
 
@@ -321,33 +338,108 @@ def measure_served(ctx, db, specs, arrivals, max_batch_residues, max_inflight, c
             "hsps": int(sum(n_hsp)), "sample_equal_plain": ok}
 
 
-def forms_on_c1(ctx, db, n_batch=256, plain_sample=32, served_s=10.0):
-    """Batched vs plain on the first 256 queries of the batched workload
-    (768-residue groups, the size the autotuner picks for it), then a
-    served stream (Poisson, 80% BLASTP / 20% BLASTX) at 60% of the batched
-    rate in query residues for served_s seconds, against the
-    2,000,000-subject database."""
-    qs = workload_c3(n_batch)
-    bat = measure_batched(ctx, db, qs, steps=2, warmup=1, plain_sample=plain_sample,
-                          group_residues=768)
-    bat["workload"] = (f"batched BLASTP: first {n_batch} queries of the 2,048-query workload "
-                       "vs the same 2,000,000 subjects")
-    # offered load: 60% of the batched rate in query residues, for the served
-    # mix's mean query length (six-frame BLASTX queries are longer)
-    gaps, specs = workload_c4(4000)
-    mean_res = float(np.mean([len(sp.seq) for _, sp in specs]))
-    qps = 0.6 * bat["queries_per_s"] * (bat["query_residues"] / bat["queries"]) / mean_res
-    n_arr = min(len(specs), int(qps * served_s * 1.5) + 20)
-    gaps, specs = gaps[:n_arr], specs[:n_arr]
-    arrivals = np.cumsum(gaps / qps)
-    n_sub = max(1, int(np.searchsorted(arrivals, served_s)))
-    srv = measure_served(ctx, db, specs[:n_sub], arrivals[:n_sub], max_batch_residues=16384,
-                         max_inflight=2)
-    srv["workload"] = (f"served stream: Poisson arrivals at {qps:.1f} q/s (60% of the batched "
-                       f"rate in query residues) for {served_s:.0f} s, 80% BLASTP / 20% BLASTX "
-                       "(600-3000 nt) vs the same 2,000,000 subjects")
-    srv["offered_qps"] = qps
-    return {"batched": bat, "served": srv}
+def profile_figures():
+    """Figures read from the committed rocprofv3 summaries of the scan
+    kernel on the plain workload (profile-derived, with their source): HBM
+    traffic per launch (FETCH_SIZE + WRITE_SIZE, scaled by a known-size copy)
+    and the VALU issue fraction (SQ_INSTS_VALU x 4 cycles / (SIMDs x clock x
+    kernel time))."""
+    out = {}
+    for key, name in (("traffic", "hb_scan_kernel_c1_traffic.json"),
+                      ("issue", "hb_scan_kernel_c1_sq_counters.json")):
+        path = os.path.join(ROOT, "profiles", name)
+        if os.path.exists(path):
+            with open(path) as f:
+                out[key] = dict(json.load(f), source=f"profiles/{name}")
+    return out
+
+
+def measure_dropin(ctx, db, spec, reps=10):
+    """The drop-in per query as H-BLAST's H_BLAST_pre + H_BLAST_get_data
+    pair does it: prepare (lookup table on the device), scan, and results
+    (per-subject stats and ordered HSPs copied to the host), after one
+    warm-up; mean wall time per query."""
+    def one():
+        q = hbseed.Query(ctx, spec)
+        s = hbseed.Scan(ctx, db, q)
+        st, h = s.results()
+        s.close()
+        q.close()
+        return len(h)
+    one()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"queries_per_s": 1.0 / dt, "ms_per_query": dt * 1e3, "reps": reps,
+            "steps": "prepare (lookup table) + scan + results (stats and HSPs to the host)"}
+
+
+def measure_blastx(ctx, db, res, offs, reps=5):
+    """The plain BLASTX workload: the 1,500-nt six-frame query against the
+    same database; device time of the scan and its roofline."""
+    spec = workloads.blastx_c2(res, offs)
+    q = hbseed.Query(ctx, spec)
+    n_entries = q.lut()[1].size
+    ms = []
+    for k in range(reps + 1):
+        s = hbseed.Scan(ctx, db, q)
+        s.wait()
+        if k:
+            ms.append(s.elapsed_ms())
+        if k < reps:
+            s.close()
+    st, h = s.results()
+    s.close()
+    q.close()
+    n_res, n_subj = int(offs[-1]), len(offs) - 1
+    kms = float(np.mean(ms))
+    abytes = algorithmic_bytes(n_res, n_subj, len(h), n_entries)
+    return {"workload": "plain BLASTX: 1,500-nt query (six frames, two planted coding regions) "
+                        "vs the same 2,000,000 subjects",
+            "query_residues": int(len(spec.seq)), "contexts": int(spec.n_ctx),
+            "kernel_ms": kms, "subject_residues_per_s": n_res / (kms * 1e-3),
+            "hsps": int(len(h)), "frames_with_hsps": int(len(np.unique(h["context"]))),
+            "lookup_hits": int(st["lookup_hits"].sum()),
+            "extensions": int(st["extensions"].sum()),
+            "roofline": {"bound": "hbm", "achieved": abytes / (kms * 1e-3) / 1e9,
+                         "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": abytes / (kms * 1e-3) / HBM_PEAK, "traffic": None}}
+
+
+def measure_served_sweep(ctx, db, n_res, n_subj, start_qps=2.0, factor=1.6, points=6,
+                         seconds=8.0):
+    """Offered-load sweep of the served form: Poisson arrivals of the served
+    mix at rising rates, seconds of arrivals each, until the completed rate
+    falls below 85% of the offered one (saturation) or p99 passes 4 s."""
+    gaps, specs = workloads.served_stream(4000)
+    rows = []
+    qps = start_qps
+    for k in range(points):
+        n_arr = min(len(specs), int(qps * seconds) + 1)
+        arrivals = np.cumsum(gaps[:n_arr] / qps)
+        log(f"served sweep: {qps:.2f} q/s offered, {n_arr} arrivals")
+        m = measure_served(ctx, db, specs[:n_arr], arrivals, 16384, 2, check=2 if k == 0 else 0)
+        row = {"offered_qps": qps, "completed_qps": m["queries_per_s"],
+               "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "mean_ms": m["mean_ms"],
+               "arrivals": m["arrivals"], "blastx": m["blastx"], "batches": m["batches"],
+               "peak_inflight": m["peak_inflight"], "gpu_busy": m["gpu_ms"] / 1e3 / m["wall_s"],
+               "effective_subject_residues_per_s": m["completed"] * n_res / m["wall_s"]}
+        if k == 0:
+            row["sample_equal_plain"] = m["sample_equal_plain"]
+        rows.append(row)
+        if m["queries_per_s"] < 0.85 * qps or m["p99_ms"] > 4000:
+            break
+        qps *= factor
+    best = max(rows, key=lambda r: r["completed_qps"])
+    return {"workload": f"served stream: Poisson arrivals, 80% BLASTP (log-normal median 350) / "
+                        f"20% BLASTX (600-3000 nt) vs {n_subj} subjects ({n_res} residues) "
+                        "resident",
+            "subjects": n_subj, "residues": n_res, "points": rows,
+            "max_completed_qps": best["completed_qps"],
+            "p50_ms_at_max": best["p50_ms"], "p99_ms_at_max": best["p99_ms"]}
 
 
 def main():
@@ -379,7 +471,7 @@ def main():
                     help="c4: query residues per served batch")
     ap.add_argument("--max-inflight", type=int, default=2, help="c4: batches in flight")
     ap.add_argument("--no-forms", action="store_true",
-                    help="c1: skip the compact batched and served measurements")
+                    help="c1: skip the batched (all queries) and served (sweep) measurements")
     ap.add_argument("--no-gapped", action="store_true",
                     help="c1: skip the gapped-extension measurement")
     ap.add_argument("--no-hdb", action="store_true",
@@ -458,7 +550,7 @@ def main():
     achieved = abytes / (mean_kernel_ms * 1e-3)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "hb_scan_kernel_c1_traffic.json")
-    if os.path.exists(tf):
+    if os.path.exists(tf):   # profile-derived: rocprofv3 PMC runs of this kernel
         with open(tf) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
 
@@ -471,13 +563,36 @@ def main():
         log("database upload from pageable and page-locked memory")
         upload = measure_upload(ctx, res, offs)
 
-    # compact batched and served measurements on the same resident database
+    log("drop-in per query (prepare + scan + results)")
+    dropin = measure_dropin(ctx, db, hbseed.blastp_query(q))
+    log("plain BLASTX")
+    blastx = measure_blastx(ctx, db, res, offs)
+    prof = profile_figures()
+
+    # the batched workload (all queries) on the same resident database, and
+    # the served stream on its own 20,000,000-subject database
     forms = None
     if not args.no_forms:
+        forms = {}
         try:
-            forms = forms_on_c1(ctx, db)
+            log(f"batched: {args.queries} queries")
+            bat = measure_batched(ctx, db, workload_c3(args.queries), steps=1, warmup=1,
+                                  plain_sample=args.plain_sample)
+            bat["workload"] = (f"batched BLASTP: {args.queries} queries (log-normal median 350, "
+                               "sigma 0.7, clipped 50-3000) vs the same 2,000,000 subjects")
+            forms["batched"] = bat
         except Exception as e:   # reported in the line; the plain figure stands
-            forms = {"error": repr(e)}
+            forms["batched"] = {"error": repr(e)}
+        try:
+            log(f"served: {args.serve_subjects} subjects")
+            sres, soffs = workloads.served_db(args.serve_subjects)
+            sdb = hbseed.Database(ctx, sres, soffs)
+            n_sres, n_ssubj = int(soffs[-1]), len(soffs) - 1
+            del sres, soffs
+            forms["served"] = measure_served_sweep(ctx, sdb, n_sres, n_ssubj)
+            sdb.close()
+        except Exception as e:
+            forms["served"] = {"error": repr(e)}
 
     out = {
         "metric": "subject residues/s",
@@ -506,9 +621,18 @@ def main():
         "lookup_hits": int(stats["lookup_hits"].sum()),
         "extensions": int(stats["extensions"].sum()),
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": prof.get("traffic", {}).get("source")},
         "cpu_baseline": cpu,
+        "dropin": dropin,
+        "blastx": blastx,
     }
+    if "issue" in prof:
+        iss = prof["issue"]
+        out["valu_issue"] = {k: iss.get(k) for k in ("valu_issue_frac", "valu_per_hit",
+                                                      "salu_per_hit", "lds_per_hit",
+                                                      "lds_bank_conflict_frac", "kernel_ms",
+                                                      "source")}
     if copy:
         out["copy_calibration"] = copy
     if gapped:

@@ -37,6 +37,9 @@ struct hb_batch {
     bool built = false;
     int32_t group_res = 0;
     float build_ms = 0;
+    // launch geometry: 0 = the configuration's own choice
+    int32_t g_waves = 0, g_ring = 0, g_items = 64;
+    int32_t tune_candidates = 0;
     // groups
     hb_group_set gs;
     std::vector<Class> classes;
@@ -158,7 +161,7 @@ static void size_items(hb_batch *b, uint32_t n_tiles)
 {
     for (Class &c : b->classes) {
         const uint32_t n_groups = (uint32_t)(c.g_end - c.g_begin);
-        const uint64_t want = (uint64_t)c.cfg.blocks * 64;
+        const uint64_t want = (uint64_t)c.cfg.blocks * (uint64_t)std::max(b->g_items, 1);
         const uint64_t per_group = std::max<uint64_t>(1, want / std::max<uint32_t>(n_groups, 1));
         c.tiles_per_item = (uint32_t)std::max<uint64_t>(1, (n_tiles + per_group - 1) / per_group);
     }
@@ -203,7 +206,7 @@ static int build(hb_batch *b, int32_t target)
                                   hb_scan_fixed_lds(c.qlen_pad_max, c.ctx_pad_max),
                                   b->db->max_len, b->p.window);
         rc = hb_scan_config(b->ctx, true, c.qlen_pad_max, c.ctx_pad_max, c.diag_len_max,
-                            n_groups * b->db->n_tiles, &c.cfg, c.place);
+                            n_groups * b->db->n_tiles, &c.cfg, c.place, b->g_ring, b->g_waves);
         if (rc) return rc;
         if (c.place == HB_PLACE_GLOBAL) {
             c.gstate_bytes =
@@ -337,6 +340,34 @@ static int run_pass(hb_batch *b, uint32_t t_begin, uint32_t t_end, bool use_grap
     }
 }
 
+// Times one candidate configuration on the slice [t0, t0 + span) of the
+// database tiles (a warm pass, then a timed one); *ms = its kernel time, or
+// a negative value when the configuration does not fit.
+static int time_candidate(hb_batch *b, int32_t target, uint32_t t0, uint32_t span, float *ms)
+{
+    *ms = -1.f;
+    int rc = build(b, target);
+    if (rc == HB_ERR_UNSUPPORTED) return HB_OK;
+    if (rc) return rc;
+    // items sized for the slice, so that the timing is not a tail effect
+    size_items(b, span);
+    uint32_t n = 0;
+    rc = run_pass(b, t0, t0 + span, false, &n);   // warm
+    if (!rc) rc = run_pass(b, t0, t0 + span, false, &n);
+    if (rc == HB_ERR_UNSUPPORTED) return HB_OK;
+    if (rc) return rc;
+    *ms = b->kernel_ms;
+    b->tune_candidates++;
+    return HB_OK;
+}
+
+// Autotuner: times candidates on a median-length slice of the database
+// (4% of the tiles) and keeps the fastest, one knob at a time: the group
+// size (query residues per group: more groups rescan the database, larger
+// ones starve LDS), then wavefronts per workgroup, the staging ring (which
+// trades LDS for the number of subjects whose extensions batch together)
+// and the work items per workgroup (subjects per queue pop: balance against
+// queue traffic).
 static int autotune(hb_batch *b, int32_t *best)
 {
     const uint32_t nt = b->db->n_tiles;
@@ -345,27 +376,43 @@ static int autotune(hb_batch *b, int32_t *best)
     const uint32_t t0 = (nt - span) / 2;   // subjects of median length
     int32_t longest = 0;
     for (const hb_concat &q : b->queries) longest = std::max(longest, (int32_t)q.seq.size());
-    float best_ms = 0;
+    float best_ms = 0, ms = 0;
     *best = 0;
+    b->tune_candidates = 0;
     for (int32_t cand : {512, 768, 1024, 1536, 2048, 3072, 4096}) {
-        if (cand < 64) continue;
-        int rc = build(b, cand);
-        if (rc == HB_ERR_UNSUPPORTED) continue;
+        int rc = time_candidate(b, cand, t0, span, &ms);
         if (rc) return rc;
-        // items sized for the slice, so that the timing is not a tail effect
-        size_items(b, span);
-        uint32_t n = 0;
-        rc = run_pass(b, t0, t0 + span, false, &n);   // warm
-        if (!rc) rc = run_pass(b, t0, t0 + span, false, &n);
-        if (rc == HB_ERR_UNSUPPORTED) continue;
-        if (rc) return rc;
-        if (*best == 0 || b->kernel_ms < best_ms) {
-            best_ms = b->kernel_ms;
+        if (ms >= 0 && (*best == 0 || ms < best_ms)) {
+            best_ms = ms;
             *best = cand;
         }
         if (cand >= longest + b->p.window && cand > 2048) break;   // larger groups change nothing
     }
-    return *best ? HB_OK : HB_ERR_UNSUPPORTED;
+    if (!*best) return HB_ERR_UNSUPPORTED;
+    // launch geometry at the chosen group size, one knob at a time
+    struct Knob {
+        int32_t *field;
+        std::vector<int32_t> values;
+        bool user;
+    } knobs[] = {{&b->g_waves, {2, 1}, b->o.waves_per_workgroup > 0},
+                 {&b->g_ring, {1024, 1536}, b->o.ring_bytes > 0},
+                 {&b->g_items, {16, 256}, b->o.items_per_workgroup > 0}};
+    for (Knob &k : knobs) {
+        if (k.user) continue;
+        const int32_t keep = *k.field;
+        int32_t pick = keep;
+        for (int32_t v : k.values) {
+            *k.field = v;
+            int rc = time_candidate(b, *best, t0, span, &ms);
+            if (rc) return rc;
+            if (ms >= 0 && ms < best_ms * 0.98f) {   // a clear gain only
+                best_ms = ms;
+                pick = v;
+            }
+        }
+        *k.field = pick;
+    }
+    return HB_OK;
 }
 
 // Fetches the n records and per-group counters of the last pass, maps
@@ -439,6 +486,17 @@ int hb_batch_create_on(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb
     b->db = db;
     hb_batch_opts_default(&b->o);
     if (opts) b->o = *opts;
+    const hb_batch_opts &o = b->o;
+    if ((o.waves_per_workgroup != 0 && o.waves_per_workgroup != 1 &&
+         o.waves_per_workgroup != 2 && o.waves_per_workgroup != 4) ||
+        (o.ring_bytes != 0 && (o.ring_bytes < 256 || o.ring_bytes % 64)) ||
+        o.items_per_workgroup < 0 || o.group_residues < 0) {
+        delete b;
+        return HB_ERR_ARG;
+    }
+    b->g_waves = o.waves_per_workgroup;
+    b->g_ring = o.ring_bytes;
+    b->g_items = o.items_per_workgroup > 0 ? o.items_per_workgroup : 64;
     int rc = hb_resolve_params(params, &b->p);
     if (rc) {
         delete b;
@@ -530,6 +588,9 @@ void hb_batch_opts_default(hb_batch_opts *o)
     o->group_residues = 0;
     o->use_graph = 1;
     o->tune_tiles = 0;
+    o->waves_per_workgroup = 0;
+    o->ring_bytes = 0;
+    o->items_per_workgroup = 0;
 }
 
 int hb_batch_create(hb_ctx *ctx, hb_db *db, const hb_params *params, const hb_batch_opts *opts,
@@ -619,6 +680,15 @@ int hb_batch_info_get(hb_batch *b, hb_batch_info *info)
     info->lookup_hits = b->tot[0];
     info->extensions = b->tot[1];
     info->hsps = b->tot[2];
+    info->waves_per_workgroup = 0;
+    info->ring_bytes = 0;
+    for (const Class &c : b->classes)   // the largest class's configuration
+        if (c.g_end - c.g_begin >= 0 && c.cfg.lds) {
+            info->waves_per_workgroup = c.cfg.waves;
+            info->ring_bytes = c.cfg.ring;
+        }
+    info->items_per_workgroup = b->g_items;
+    info->tune_candidates = b->tune_candidates;
     return HB_OK;
 }
 

@@ -143,9 +143,11 @@ constexpr size_t HB_GSTATE_WORD = 8;
 // Bytes of that state for diagonal arrays of D entries (upper bound).
 size_t hb_gstate_bytes_max(const hb_ctx *c, int32_t D);
 // place: where the diagonal state goes (LDS bytes per diagonal follow).
+// ring_cap > 0 caps the staging ring, waves > 0 fixes the wavefronts per
+// workgroup (autotuner constraints; HB_ERR_UNSUPPORTED if nothing fits).
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
                    int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg,
-                   int place = HB_PLACE_LDS);
+                   int place = HB_PLACE_LDS, int32_t ring_cap = 0, int32_t waves = 0);
 
 // Asynchronous steps of a batch (hb_batch.cpp), used by the serving loop:
 // the batch runs on the given stream (nullptr: the context's); launch

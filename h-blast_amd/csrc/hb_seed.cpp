@@ -312,15 +312,18 @@ int hb_group_set_build(hb_ctx *c, const std::vector<hb_concat> &qs,
 // form stages at most 2048 residues per wave: its largest class of C3
 // queries lost a workgroup per CU to larger staging.
 int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_max,
-                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg, int place)
+                   int32_t diag_len_max, uint32_t work_items, hb_launch_cfg *cfg, int place,
+                   int32_t ring_cap, int32_t force_waves)
 {
     const size_t fixed = hb_scan_fixed_lds(qlen_pad_max, ctx_pad_max);
     const bool gstate = place == HB_PLACE_GLOBAL;
     const int sb = hb_state_bytes(place);
+    int32_t ring_top = multi ? std::min(HB_RING_MAX, 2048) : HB_RING_MAX;
+    if (ring_cap > 0) ring_top = std::min(ring_top, ring_cap & ~63);
     int best_wpc = 0;
     for (int waves : {4, 2, 1}) {
-        for (int32_t ring = multi ? std::min(HB_RING_MAX, 2048) : HB_RING_MAX; ring >= 256;
-             ring -= 64) {
+        if (force_waves > 0 && waves != force_waves) continue;
+        for (int32_t ring = ring_top; ring >= 256; ring -= 64) {
             const size_t lds = fixed + waves * hb_scan_wave_lds(diag_len_max, ring, sb);
             if (lds > c->max_lds) continue;
             int per_cu = 0;

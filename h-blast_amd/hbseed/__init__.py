@@ -80,13 +80,16 @@ class HBError(RuntimeError):
 
 class _BatchOpts(C.Structure):
     _fields_ = [("group_residues", C.c_int32), ("use_graph", C.c_int32),
-                ("tune_tiles", C.c_uint32)]
+                ("tune_tiles", C.c_uint32), ("waves_per_workgroup", C.c_int32),
+                ("ring_bytes", C.c_int32), ("items_per_workgroup", C.c_int32)]
 
 
 class _BatchInfo(C.Structure):
     _fields_ = [("n_queries", C.c_int32), ("n_groups", C.c_int32), ("n_classes", C.c_int32),
                 ("group_residues", C.c_int32), ("kernel_ms", C.c_float), ("build_ms", C.c_float),
-                ("lookup_hits", C.c_uint64), ("extensions", C.c_uint64), ("hsps", C.c_uint64)]
+                ("lookup_hits", C.c_uint64), ("extensions", C.c_uint64), ("hsps", C.c_uint64),
+                ("waves_per_workgroup", C.c_int32), ("ring_bytes", C.c_int32),
+                ("items_per_workgroup", C.c_int32), ("tune_candidates", C.c_int32)]
 
 
 class _ServeOpts(C.Structure):
@@ -947,10 +950,12 @@ class Batch:
     """Many queries in one batched scan (hb_batch_*)."""
 
     def __init__(self, ctx: Context, db: Database, group_residues=0, use_graph=True,
-                 tune_tiles=0, threshold=11, window=40, word_size=3, matrix=None):
+                 tune_tiles=0, threshold=11, window=40, word_size=3, matrix=None,
+                 waves_per_workgroup=0, ring_bytes=0, items_per_workgroup=0):
         self.ctx, self.db = ctx, db
         p, self._keep = _params_struct(threshold, window, word_size, matrix)
-        o = _BatchOpts(group_residues, 1 if use_graph else 0, tune_tiles)
+        o = _BatchOpts(group_residues, 1 if use_graph else 0, tune_tiles, waves_per_workgroup,
+                       ring_bytes, items_per_workgroup)
         h = C.c_void_p()
         _check(_lib().hb_batch_create(ctx.h, db.h, C.byref(p), C.byref(o), C.byref(h)),
                "hb_batch_create")

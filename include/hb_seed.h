@@ -618,9 +618,18 @@ int hb_gapped_search(hb_ctx *ctx, hb_db *db, const uint8_t *seq, int32_t len, in
  * fit on-chip memory. */
 typedef struct hb_batch hb_batch;
 typedef struct hb_batch_opts {
-    int32_t group_residues;  /* target concatenated residues per group; 0 = autotune */
+    int32_t group_residues;  /* target concatenated residues per group; 0 = autotune this
+                                and the launch geometry below */
     int32_t use_graph;       /* 1: capture the launches in a hipGraph and replay it */
     uint32_t tune_tiles;     /* database tiles timed per autotune candidate (0 = default) */
+    /* Launch geometry (the reference exposes its grid as -gpu_blocks /
+     * -gpu_threads, cmdline_flags.cpp:213-230): 0 = autotuned with the group
+     * size, or the default when group_residues is given. */
+    int32_t waves_per_workgroup;   /* 4, 2 or 1 (default: the most wavefronts per CU) */
+    int32_t ring_bytes;            /* staging ring per wavefront (multiple of 64, >= 256;
+                                      default: the largest that keeps the occupancy) */
+    int32_t items_per_workgroup;   /* (group, tile chunk) work items per workgroup, which
+                                      sets the subjects per item (default 64) */
 } hb_batch_opts;
 
 typedef struct hb_batch_info {
@@ -633,6 +642,10 @@ typedef struct hb_batch_info {
     uint64_t lookup_hits;    /* last run, all queries */
     uint64_t extensions;
     uint64_t hsps;
+    int32_t waves_per_workgroup;   /* chosen launch geometry (of the largest class) */
+    int32_t ring_bytes;
+    int32_t items_per_workgroup;
+    int32_t tune_candidates;       /* autotune candidates timed (0: none) */
 } hb_batch_info;
 
 void hb_batch_opts_default(hb_batch_opts *o);

@@ -110,3 +110,30 @@ def test_batch_and_serve_with_long_subjects():
     finally:
         db.close()
         ctx.close()
+
+
+@pytest.mark.parametrize("waves,ring,items", [(1, 256, 1), (2, 1024, 16), (4, 2048, 256)])
+def test_launch_geometry_keeps_results(setup, waves, ring, items):
+    """The knobs the autotuner sets (wavefronts per workgroup, staging ring,
+    work items per workgroup) change only speed: every query equals the
+    oracle under each geometry, and the autotuned run reports its choice."""
+    ctx, db, res, offs, specs = setup
+    b = hbseed.Batch(ctx, db, group_residues=1024, waves_per_workgroup=waves, ring_bytes=ring,
+                     items_per_workgroup=items)
+    ids = [b.add(s) for s in specs]
+    b.run()
+    info = b.info()
+    assert info["waves_per_workgroup"] == waves and info["items_per_workgroup"] == items
+    assert 256 <= info["ring_bytes"] <= ring
+    for qid, spec in zip(ids, specs):
+        assert np.array_equal(b.results(qid).view(np.int32).reshape(-1, 8),
+                              oracle(spec, res, offs)), (waves, ring, items, qid)
+    b.close()
+    b = hbseed.Batch(ctx, db)   # autotuned
+    for s in specs:
+        b.add(s)
+    b.run()
+    info = b.info()
+    assert info["tune_candidates"] >= 3
+    assert info["waves_per_workgroup"] in (1, 2, 4) and info["items_per_workgroup"] in (16, 64, 256)
+    b.close()
