@@ -21,8 +21,9 @@
 // between resets (hb_scan_args::state_limit), so the longest subject and
 // its successor must fit: 2 * (longest + window) + 64 < 2^23.
 #define HB_LDS_POS_LIMIT (1 << 23)
-// ... in global memory: one 64-bit word per diagonal, 30-bit positions.
-#define HB_GLOBAL_POS_LIMIT (1 << 29)
+// ... in global memory: one 64-bit word per diagonal, 29-bit positions
+// (subjects up to 2^28 residues).
+#define HB_GLOBAL_POS_LIMIT (1 << 28)
 
 // One query group: a concatenated query (one or more queries / frames as
 // contexts) with its lookup table.  The plain path scans one group; the

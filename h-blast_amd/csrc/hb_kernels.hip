@@ -324,17 +324,18 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 //              the ordering marker of the decision pass under way (0
 //              between passes).  With the state in global memory (long
 //              queries) the word is a uint64 in the wavefront's own region:
-//              30-bit position, bit 30 pending, bit 31 flag, marker in the
+//              29-bit position, bit 29 pending, bit 30 flag, marker in the
 //              high half.
 //   seg[64]    uint32 tagged segment starts of the current hit round
 //   jobs[kJobCap] uint2 queued two-hit extensions (a ring, oldest first):
 //              (q | (s - last) << 24, s | slot << 28)
 //   blk[kBlkCap]  uint2 hits waiting on a pending diagonal (q, s | slot << 28)
-//   slot[HB_SLOTS] per staged subject whose extensions may still be queued:
-//              {oid, length, diagonal offset, ring offset} and {lookup hits,
-//              extensions, HSPs, 0}
-//   ring[ring] residues of those subjects (16-byte aligned), filled from
-//              the front and emptied whole when every queued extension ran
+//   slot[HB_SLOTS] per staged subject whose extensions may still be queued
+//              (a ring, oldest first): {oid, length, diagonal offset, ring
+//              offset} and {lookup hits, extensions, HSPs, 0}
+//   ring[ring] residues of those subjects (16-byte aligned, circular: a
+//              subject that does not fit before the end starts at 0); the
+//              oldest subject leaves once none of its extensions is queued
 // Per-workgroup LDS: score matrix, context tables (stride ctx_pad_max) and
 // query of the group being scanned.
 constexpr uint32_t kJobCap = HB_JOBCAP;
@@ -349,18 +350,24 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Diagonal word formats: marker shift, flag, pending, position mask.
+// Diagonal word formats: marker shift, flag, pending, position mask.  The
+// state is a position p < kP (idle, last hit p), kP (an extension pending)
+// or kF | p (extended to the right up to p); kF = 2 kP and 3 kP < 2^31, so
+// for a hit at offset sb < kP the signed difference sb - state is sb - p
+// when idle, in [-kP, 0) when pending and sb - p - kF when flagged.
 template <bool G> struct DiagWord;
 template <> struct DiagWord<false> {
     using T = uint32_t;
     static constexpr int kMS = 25;
-    static constexpr uint32_t kF = 1u << 24, kP = 1u << 23, kV = (1u << 23) - 1, kSM = (1u << 25) - 1;
+    static constexpr uint32_t kF = 1u << 24, kP = 1u << 23, kSM = (1u << 25) - 1;
 };
 template <> struct DiagWord<true> {
     using T = unsigned long long;
     static constexpr int kMS = 32;
-    static constexpr uint32_t kF = 1u << 31, kP = 1u << 30, kV = (1u << 30) - 1, kSM = 0xffffffffu;
+    static constexpr uint32_t kF = 1u << 30, kP = 1u << 29, kSM = 0xffffffffu;
 };
+static_assert(HB_LDS_POS_LIMIT <= (1 << 23) && HB_GLOBAL_POS_LIMIT <= (1 << 28),
+              "positions (< limit + longest subject) must stay below kP");
 
 template <bool G>
 struct WaveScan {
@@ -399,8 +406,10 @@ struct WaveScan {
     uint32_t j_head;   // oldest queued extension (wave-uniform)
     uint32_t n_jobs;   // wave-uniform
     uint32_t n_blk;    // wave-uniform
-    uint32_t n_live;   // staged subjects not yet retired (slots 0 .. n_live - 1)
-    uint32_t ring_head;
+    uint32_t n_live;   // staged subjects not yet retired (slots s_tail ..)
+    uint32_t s_tail;   // slot of the oldest staged subject
+    uint32_t ring_head;  // ring offset after the newest staged subject
+    bool rwrap;        // the staged residues wrap: newer subjects start at 0
     uint64_t tot_hits, tot_ext, tot_hsp;  // wave-uniform running totals
 
     __device__ __forceinline__ uint32_t mod_d(uint32_t x) const
@@ -491,31 +500,32 @@ struct WaveScan {
         const uint32_t mk = (uint32_t)(HB_WAVE - lane);
         word_t old = 0;
         if (active) old = w_max(d, (word_t)mk << W::kMS);
-        const uint32_t ost = (uint32_t)(old & W::kSM);
-        if (active && (uint32_t)(old >> W::kMS) == 0 && ost != 0) w_or(d, (word_t)ost);
+        // the first lane to reach the word finds no marker: it ORs the state
+        // back (nothing to do when the state is 0)
+        if (active && (word_t)(old - 1) < ((word_t)1 << W::kMS) - 1) w_or(d, old);
         sync();
         const word_t wv = w_ld(d);
         const bool win = active && (uint32_t)(wv >> W::kMS) == mk;
-        const uint32_t st = (uint32_t)(wv & W::kSM);
-        const bool pend = (st & W::kP) != 0;
-        const bool flag = (st & W::kF) != 0;
+        const uint32_t st = (uint32_t)wv & W::kSM;
         const int32_t sb = so + sbase;
-        const int32_t diff = sb - (int32_t)st;   // s - last if unflagged and not pending
-        const bool job = win && !pend && !flag && diff >= 3 && diff < window;
-        const bool wait = win && pend;
-        const uint32_t fst = flag ? (sb >= (int32_t)(st & W::kV) ? (uint32_t)sb : st)
-                                  : (diff >= window ? (uint32_t)sb : st);
-        const uint32_t nst = job ? W::kP : fst;
+        // s - last on an idle diagonal; negative (below -kP + sb) when the
+        // diagonal is pending or flagged
+        const int32_t diff = (int32_t)((uint32_t)sb - st);
+        const bool job = win && diff >= 3 && diff < window;
+        const bool wait = win && st == W::kP;
+        // the hit becomes last (and clears the flag) on an idle diagonal whose
+        // last hit is a window or more behind, or on a flagged one it reached
+        const bool set = diff >= window || (uint32_t)(diff + (int32_t)W::kF) < W::kP;
+        const uint32_t nst = job ? W::kP : (set ? (uint32_t)sb : st);
         if (win) w_st(d, (word_t)nst);
         sync();
-        // losers: against the winner's new state
-        const uint32_t st2 = (uint32_t)(w_ld(d) & W::kSM);
-        const bool pend2 = (st2 & W::kP) != 0;
-        const bool skip2 = (st2 & W::kF) != 0 ? sb < (int32_t)(st2 & W::kV)
-                                              : sb - (int32_t)st2 < 3;
+        // losers against the winner's new state: wait if it is pending, done
+        // if they would skip, else carried
+        const uint32_t st2 = (uint32_t)w_ld(d) & W::kSM;
+        const int32_t diff2 = (int32_t)((uint32_t)sb - st2);
         const bool lose = active && !win;
-        const bool lwait = lose && pend2;
-        const bool carry = lose && !pend2 && !skip2;
+        const bool lwait = lose && st2 == W::kP;
+        const bool carry = lose && (diff2 >= 3 || (uint32_t)(diff2 + (int32_t)W::kF) < W::kP);
         const uint64_t mj = __ballot(job), mb = __ballot(wait || lwait);
         const uint32_t tso = (uint32_t)so | (sl << 28);
         if (job)
@@ -640,14 +650,14 @@ struct WaveScan {
         retire_all();
     }
 
-    // Writes the counters of the subjects in slots 0 .. n_live - 1 and
-    // empties the ring.
+    // Writes the counters of every staged subject and empties the ring.
     __device__ __forceinline__ void retire_all()
     {
         if (n_live == 0) return;
         uint32_t h = 0, e = 0, hs = 0;
         if ((uint32_t)lane < n_live) {
-            const uint4 A = sla[lane], B = slb[lane];
+            const uint32_t sl = (s_tail + (uint32_t)lane) & (HB_SLOTS - 1);
+            const uint4 A = sla[sl], B = slb[sl];
             h = B.x;
             e = B.y;
             hs = B.z;
@@ -661,8 +671,78 @@ struct WaveScan {
         tot_ext += wave_sum(e);
         tot_hsp += wave_sum(hs);
         n_live = 0;
+        s_tail = 0;
         ring_head = 0;
+        rwrap = false;
         sync();
+    }
+
+    // True when no queued extension belongs to the oldest staged subject.
+    // Checked after a replay, none of its hits is then blocked either: a
+    // blocked hit waits on a diagonal that a queued extension of its own or
+    // an older subject holds, and the oldest subject has no older one.
+    __device__ __forceinline__ bool tail_done() const
+    {
+        bool mine = false;
+        for (uint32_t i = (uint32_t)lane; i < n_jobs; i += HB_WAVE)
+            mine |= (jobs[(j_head + i) & (kJobCap - 1)].y >> 28) == s_tail;
+        return __ballot(mine) == 0;
+    }
+
+    // Writes the counters of the oldest staged subject and frees its slot
+    // and ring space.
+    __device__ __forceinline__ void retire_tail()
+    {
+        const uint32_t sl = s_tail;
+        const uint4 A = sla[sl], B = slb[sl];
+        // wave-uniform, so that the ring state stays uniform
+        const uint32_t t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)A.w);
+        if (lane == 0 && a->stats) {
+            a->stats[3 * (size_t)A.x + 0] = B.x;
+            a->stats[3 * (size_t)A.x + 1] = B.y;
+            a->stats[3 * (size_t)A.x + 2] = B.z;
+        }
+        tot_hits += (uint32_t)__builtin_amdgcn_readfirstlane((int)B.x);
+        tot_ext += (uint32_t)__builtin_amdgcn_readfirstlane((int)B.y);
+        tot_hsp += (uint32_t)__builtin_amdgcn_readfirstlane((int)B.z);
+        s_tail = (sl + 1) & (HB_SLOTS - 1);
+        n_live -= 1;
+        if (n_live == 0) {   // an empty ring starts over at slot 0, offset 0
+            s_tail = 0;
+            ring_head = 0;
+            rwrap = false;
+        } else if (rwrap) {
+            // the first subject past the wrap starts at 0, below every
+            // offset before it
+            const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)sla[s_tail].w);
+            if (t < t0) rwrap = false;
+        }
+    }
+
+    // Returns the ring offset for a new subject of span bytes (span <= the
+    // ring size) once a slot and the space are free: retires the oldest
+    // subjects whose extensions all ran, and runs the oldest queued
+    // extensions while the oldest subject still has some.
+    __device__ __forceinline__ uint32_t make_room(uint32_t span)
+    {
+        for (;;) {
+            while (n_live && tail_done()) retire_tail();
+            if (n_live == 0) return 0;
+            if (n_live < HB_SLOTS) {
+                const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)sla[s_tail].w);
+                if (!rwrap) {
+                    if (ring_head + span <= a->ring) return ring_head;
+                    if (span <= t) {
+                        rwrap = true;
+                        return 0;
+                    }
+                } else if (ring_head + span <= t) {
+                    return ring_head;
+                }
+            }
+            run_batch<true>(ring);
+            replay();
+        }
     }
 
     // Moves the hits that lost a pass (lost = true) to lanes 0..n-1 in lane
@@ -691,6 +771,7 @@ struct WaveScan {
     struct Win {
         uint32_t cnt, excl;
         int32_t eb;
+        int32_t sm;        // (s0 + lane) mod D
         uint32_t H;
         int32_t s0, s0m;   // first position, and it mod D
     };
@@ -717,6 +798,8 @@ struct WaveScan {
         w.H = lane63(incl);
         w.s0 = s0;
         w.s0m = s0m;
+        const int32_t sm = s0m + lane;
+        w.sm = sm >= D ? sm - D : sm;
     }
 
     // Scans one subject (slot sl, diagonal offset sbase).  Hits are taken in
@@ -734,7 +817,7 @@ struct WaveScan {
         Win A, B;
         A.H = 0;
         A.cnt = A.excl = 0;
-        A.eb = 0;
+        A.eb = A.sm = 0;
         A.s0 = A.s0m = 0;
         B = A;
         bool hasB = false;        // B holds the window after A
@@ -743,7 +826,9 @@ struct WaveScan {
         uint32_t c = 0;           // carried hits, in lanes 0..c-1
         uint32_t cq = 0, cd = 0;
         int32_t cs_ = 0;
-        uint32_t carry = 0;       // source of the last lane of the previous round
+        // source of the last lane of the previous round, no larger than any
+        // segment start of the next round (the inclusive max-scan relies on it)
+        uint32_t carry = 0;
         for (;;) {
             // A: a window with hits left, unless the subject has no more
             while (r == A.H && (hasB || nxt + 3 <= slen)) {
@@ -758,6 +843,10 @@ struct WaveScan {
                     if (nxtm >= D) nxtm -= D;
                 }
                 r = 0;
+                // the previous round's last segment ended with the old
+                // window: the new window's first hit starts a segment, and
+                // the carried lanes before it take no source
+                carry = 0;
             }
             const uint32_t need = (uint32_t)HB_WAVE - c;
             if (!hasB && A.H - r < need && nxt + 3 <= slen) {
@@ -780,30 +869,37 @@ struct WaveScan {
                 rgen = 1;
             }
             const uint32_t tag = rgen << 8;
-            if (A.cnt && A.excl >= r && A.excl < r + take1)
-                seg[A.excl - r + c] = tag | ((uint32_t)lane + 1);
+            const uint32_t ea = A.excl - r;
+            if (A.cnt && ea < take1) seg[ea + c] = tag | ((uint32_t)lane + 1);
             if (take2 && B.cnt && B.excl < take2)
                 seg[B.excl + take1 + c] = tag | ((uint32_t)lane + 65);
             sync();
-            const uint32_t sv = seg[lane];
-            const uint32_t v = wave_incl_max((sv & ~255u) == tag ? (sv & 255u) : 0u);
-            const uint32_t p = min(v ? v - 1 : carry, 127u);
-            const uint32_t last = c + take1 + take2 - 1;
+            const uint32_t sv = seg[lane] - tag;
+            // lanes before the first segment start continue the previous
+            // round's last segment (carry); the source of each lane is the
+            // largest start at or before it
+            uint32_t v = sv < 256u ? sv : 0u;
+            if (lane == 0 && v == 0) v = carry + 1;
+            v = wave_incl_max(v);
+            const uint32_t p = v - 1;
             const uint32_t isnew_hi = c + take1 + take2;
-            carry = (uint32_t)__builtin_amdgcn_readlane((int)p, (int)min(last, 63u));
+            carry = (uint32_t)__builtin_amdgcn_readlane((int)p, (int)(isnew_hi - 1));
             const bool isnew = (uint32_t)lane >= c && (uint32_t)lane < isnew_hi;
             const bool inb = (uint32_t)lane >= c + take1;
-            int32_t eb = __shfl(A.eb, (int)(p & 63u), HB_WAVE);
-            if (take2) {
-                const int32_t ebb = __shfl(B.eb, (int)(p & 63u), HB_WAVE);
+            const int p4 = (int)((p & 63u) << 2);
+            int32_t eb = __builtin_amdgcn_ds_bpermute(p4, A.eb);
+            int32_t smp = __builtin_amdgcn_ds_bpermute(p4, A.sm);
+            if (take2) {   // B's position p - 64 is A's lane p - 64 plus 64
+                const int32_t ebb = __builtin_amdgcn_ds_bpermute(p4, B.eb);
+                int32_t smb = smp + HB_WAVE;
+                if (smb >= D) smb -= D;
                 eb = p >= 64 ? ebb : eb;
+                smp = p >= 64 ? smb : smp;
             }
-            const uint32_t flat = inb ? (uint32_t)lane - c - take1 : r + (uint32_t)lane - c;
-            const uint32_t ei = min((uint32_t)((int32_t)flat + eb), n_entries - 1);
-            const uint32_t qn = isnew ? entries[ei] : 0u;
-            int32_t smp = A.s0m + (int32_t)p;
-            if (smp >= D) smp -= D;
-            if (smp >= D) smp -= D;
+            const uint32_t lc = (uint32_t)lane - c;
+            const uint32_t flat = inb ? lc - take1 : lc + r;
+            uint32_t qn = 0;
+            if (isnew) qn = entries[(uint32_t)((int32_t)flat + eb)];
             int32_t dn = (int32_t)qn - smp;
             if (dn < 0) dn += D;
             const bool isc = (uint32_t)lane < c;
@@ -826,9 +922,10 @@ struct WaveScan {
 
     // Scans the subject of processing-order record m.  A subject that fits
     // the ring is staged there (16-byte loads and stores) and its queued
-    // extensions wait until 64 are queued or the ring or the slots run out,
-    // so that extension batches stay full; a longer one is scanned from
-    // global memory after every staged subject is complete.
+    // extensions wait until 64 are queued, or until its ring space or slot
+    // is needed while it is the oldest, so that extension batches stay full;
+    // a longer one is scanned from global memory after every staged subject
+    // is complete.
     __device__ __forceinline__ void scan_subject(const uint4 m)
     {
         const uint32_t oid = m.x;
@@ -836,8 +933,8 @@ struct WaveScan {
         const uint8_t *S = a->res + ((uint64_t)m.z | ((uint64_t)m.w << 32));
         const uint32_t span = ((uint32_t)slen + 15u) & ~15u;
         if ((uint32_t)slen <= a->ring) {
-            if (n_live == HB_SLOTS || ring_head + span > a->ring) finish<true>(ring);
-            const uint32_t sl = n_live, buf = ring_head;
+            const uint32_t buf = make_room(span);
+            const uint32_t sl = (s_tail + n_live) & (HB_SLOTS - 1);
             const uint4 *src = reinterpret_cast<const uint4 *>(S);
             uint4 *dst = reinterpret_cast<uint4 *>(ring + buf);
             for (uint32_t i = (uint32_t)lane; i < span / 16; i += HB_WAVE) dst[i] = src[i];
@@ -846,7 +943,7 @@ struct WaveScan {
                 slb[sl] = make_uint4(0, 0, 0, 0);
             }
             sync();
-            ring_head += span;
+            ring_head = buf + span;
             n_live += 1;
             const uint32_t hits = subject<true>(ring + buf, slen, sl, base);
             if (lane == 0) slb[sl].x = hits;
@@ -913,15 +1010,30 @@ __device__ __forceinline__ void bind_group(WaveScan<GS> &w, const hb_group_desc 
 // MULTI = true: many groups; the workgroup pops (group, tile) items and its
 // wavefronts share the tile's subjects, so all of them use one group's LDS
 // tables at a time.
-// Register budget for five wavefronts per SIMD (no spills at this limit):
-// the scan is latency-bound, and short queries or small groups leave LDS
-// for that many.
 // GSTATE: the diagonal words live in global memory (a.gstate, diag_len_max
 // per wavefront of the grid) instead of LDS, for queries (groups) whose
 // state does not fit LDS or would leave it too few wavefronts, and for
 // databases whose subjects are too long for the LDS word's positions.
+// The LDS-state kernels (hb_scan_kernel) keep a register budget for five
+// wavefronts per SIMD (no spills at this limit; short queries or small
+// groups leave LDS for that many); the global-state ones (hb_scan_kernel_g)
+// four, which they need to run without spills.
+template <bool MULTI, bool GSTATE>
+__device__ __forceinline__ void scan_body(const hb_scan_args &a);
+
 template <bool MULTI, bool GSTATE>
 __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void hb_scan_kernel(hb_scan_args a)
+{
+    scan_body<MULTI, GSTATE>(a);
+}
+template <bool MULTI>
+__global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void hb_scan_kernel_g(hb_scan_args a)
+{
+    scan_body<MULTI, true>(a);
+}
+
+template <bool MULTI, bool GSTATE>
+__device__ __forceinline__ void scan_body(const hb_scan_args &a)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t s_item, s_next;
@@ -961,7 +1073,9 @@ __global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) v
     w.n_jobs = 0;
     w.n_blk = 0;
     w.n_live = 0;
+    w.s_tail = 0;
     w.ring_head = 0;
+    w.rwrap = false;
     w.tot_hits = w.tot_ext = w.tot_hsp = 0;
     w.seg[lane] = 0;
     const uint32_t n_tiles = a.tile_end - a.tile_begin;
@@ -1311,8 +1425,8 @@ hipError_t hb_launch_lut(const hb_lut_job *jobs_dev, int n_jobs, int32_t qlen_ma
 static const void *scan_fn(bool multi, bool gstate)
 {
     if (gstate)
-        return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, true>)
-                     : reinterpret_cast<const void *>(hb_scan_kernel<false, true>);
+        return multi ? reinterpret_cast<const void *>(hb_scan_kernel_g<true>)
+                     : reinterpret_cast<const void *>(hb_scan_kernel_g<false>);
     return multi ? reinterpret_cast<const void *>(hb_scan_kernel<true, false>)
                  : reinterpret_cast<const void *>(hb_scan_kernel<false, false>);
 }
@@ -1325,15 +1439,13 @@ hipError_t hb_launch_scan(const hb_scan_args &a, bool multi, int n_blocks, int b
     if (a.ring % 16 || a.wave_lds % 16 || a.state_limit <= 0) return hipErrorInvalidValue;
     const bool g = a.gstate != nullptr;
     const dim3 grid(n_blocks), block(block_threads);
-#define HB_SCAN_LAUNCH(M, GS) hipLaunchKernelGGL((hb_scan_kernel<M, GS>), grid, block, lds_bytes, st, a)
     if (multi) {
-        if (g) HB_SCAN_LAUNCH(true, true);
-        else HB_SCAN_LAUNCH(true, false);
+        if (g) hipLaunchKernelGGL(hb_scan_kernel_g<true>, grid, block, lds_bytes, st, a);
+        else hipLaunchKernelGGL((hb_scan_kernel<true, false>), grid, block, lds_bytes, st, a);
     } else {
-        if (g) HB_SCAN_LAUNCH(false, true);
-        else HB_SCAN_LAUNCH(false, false);
+        if (g) hipLaunchKernelGGL(hb_scan_kernel_g<false>, grid, block, lds_bytes, st, a);
+        else hipLaunchKernelGGL((hb_scan_kernel<false, false>), grid, block, lds_bytes, st, a);
     }
-#undef HB_SCAN_LAUNCH
     return hipGetLastError();
 }
 
