@@ -766,156 +766,183 @@ struct WaveScan {
 
     // One 64-position window of a subject: lane i holds the cell {start,
     // count} of the 3-mer at s0 + i; excl is the exclusive prefix of the
-    // counts (the window's hits in (s, q) order are flat indices 0 .. H-1;
-    // hit f of lane i's cell is entry f + eb).
+    // counts, or ~0 where the count is 0 (the window's hits in (s, q) order
+    // are flat indices 0 .. H-1; hit f of lane i's cell is entry f + eb).
     struct Win {
-        uint32_t cnt, excl;
+        uint32_t excl;
         int32_t eb;
-        int32_t sm;        // (s0 + lane) mod D
         uint32_t H;
         int32_t s0, s0m;   // first position, and it mod D
     };
 
+    // The {start, count} cell of the word at position s0 + lane (0 past the
+    // subject's last word).  Every cell is present (count 0 for words
+    // without neighbours): a presence-bit test first saves no load for the
+    // queries this stage sees and costs an LDS read.
     template <typename SP>
-    __device__ __forceinline__ void load_window(SP S, int32_t slen, int32_t s0, int32_t s0m,
-                                                Win &w) const
+    __device__ __forceinline__ uint64_t cell_at(SP S, int32_t slen, int32_t s0) const
     {
         const int32_t s = s0 + lane;
-        uint32_t cnt = 0, cst = 0;
+        uint64_t c2 = 0;
         if (s + 3 <= slen) {
             const uint32_t idx = ((uint32_t)S[s] << 10) | ((uint32_t)S[s + 1] << 5) | S[s + 2];
-            // every cell is present (count 0 for words without neighbours);
-            // a presence-bit test first saves no load for the queries this
-            // stage sees and costs an LDS read
-            const uint64_t c2 = cells[idx];
-            cst = (uint32_t)c2;
-            cnt = (uint32_t)(c2 >> 32);
+            c2 = cells[idx];
         }
+        return c2;
+    }
+
+    // The window at s0 from its cells (loaded one window ahead, so that the
+    // load's latency passes while the previous window's hits are decided).
+    __device__ __forceinline__ void make_window(uint64_t c2, int32_t s0, int32_t s0m, Win &w) const
+    {
+        const uint32_t cst = (uint32_t)c2, cnt = (uint32_t)(c2 >> 32);
         const uint32_t incl = wave_incl_add(cnt);
-        w.cnt = cnt;
-        w.excl = incl - cnt;
-        w.eb = (int32_t)cst - (int32_t)w.excl;
+        const uint32_t excl = incl - cnt;
+        w.excl = cnt ? excl : ~0u;
+        w.eb = (int32_t)cst - (int32_t)excl;
         w.H = lane63(incl);
         w.s0 = s0;
         w.s0m = s0m;
-        const int32_t sm = s0m + lane;
-        w.sm = sm >= D ? sm - D : sm;
     }
 
     // Scans one subject (slot sl, diagonal offset sbase).  Hits are taken in
-    // (s, q) order, 64 per round: a round takes the rest of the current
-    // window and, when that is not enough, the start of the next one, so
-    // rounds are full until the subject's last.  A hit that lost its
-    // diagonal to a lower lane in a round is carried to the front lanes of
-    // the next round (ahead of every newer hit, so each diagonal still sees
-    // its hits in subject order), and a round makes a single decision pass.
-    // Returns the subject's lookup hits.
+    // (s, q) order, 64 per round.  The next hits are staged one round ahead
+    // in hit order (lane j holds the j-th hit after the last one taken:
+    // the rest of the current window and, when that is not enough, the
+    // start of the next one), so that their lookup-entry loads run while
+    // the current round decides; a round moves the staged hits it takes to
+    // its lanes c, c + 1, ... behind the hits it carries.  A hit that lost
+    // its diagonal to a lower lane in a round is carried to the front lanes
+    // of the next round (ahead of every newer hit, so each diagonal still
+    // sees its hits in subject order), and a round makes a single decision
+    // pass.  Returns the subject's lookup hits.
     template <bool SL, typename SP>
     __device__ __forceinline__ uint32_t subject(SP S, int32_t slen, uint32_t sl, int32_t sbase)
     {
         uint32_t hits = 0;
         Win A, B;
         A.H = 0;
-        A.cnt = A.excl = 0;
-        A.eb = A.sm = 0;
+        A.excl = ~0u;
+        A.eb = 0;
         A.s0 = A.s0m = 0;
         B = A;
         bool hasB = false;        // B holds the window after A
         int32_t nxt = 0, nxtm = 0;   // the next window to load, and it mod D
+        uint64_t pf = cell_at(S, slen, 0);   // its cells (in flight)
         uint32_t r = 0;           // hits of A already taken
         uint32_t c = 0;           // carried hits, in lanes 0..c-1
         uint32_t cq = 0, cd = 0;
         int32_t cs_ = 0;
-        // source of the last lane of the previous round, no larger than any
-        // segment start of the next round (the inclusive max-scan relies on it)
+        // source of the last hit taken, no larger than any segment start of
+        // the next staging (the inclusive max-scan relies on it)
         uint32_t carry = 0;
-        for (;;) {
+        // staged hits: n_st of them (n_a from A), lane j: entry (load in
+        // flight), position, position mod D
+        uint32_t n_st = 0, n_a = 0, eq = 0;
+        int32_t eso = 0, esm = 0;
+        auto stage = [&]() {
             // A: a window with hits left, unless the subject has no more
             while (r == A.H && (hasB || nxt + 3 <= slen)) {
                 if (hasB) {
                     A = B;
                     hasB = false;
                 } else {
-                    load_window(S, slen, nxt, nxtm, A);
+                    make_window(pf, nxt, nxtm, A);
+                    pf = cell_at(S, slen, nxt + HB_WAVE);
                     hits += A.H;
                     nxt += HB_WAVE;
                     nxtm += HB_WAVE;
                     if (nxtm >= D) nxtm -= D;
                 }
                 r = 0;
-                // the previous round's last segment ended with the old
-                // window: the new window's first hit starts a segment, and
-                // the carried lanes before it take no source
+                // the last hit taken was in the old window: the new window's
+                // first hit starts a segment
                 carry = 0;
             }
-            const uint32_t need = (uint32_t)HB_WAVE - c;
-            if (!hasB && A.H - r < need && nxt + 3 <= slen) {
-                load_window(S, slen, nxt, nxtm, B);
+            if (!hasB && A.H - r < (uint32_t)HB_WAVE && nxt + 3 <= slen) {
+                make_window(pf, nxt, nxtm, B);
+                pf = cell_at(S, slen, nxt + HB_WAVE);
                 hits += B.H;
                 hasB = true;
                 nxt += HB_WAVE;
                 nxtm += HB_WAVE;
                 if (nxtm >= D) nxtm -= D;
             }
-            const uint32_t take1 = min(need, A.H - r);
-            const uint32_t take2 = hasB ? min(need - take1, B.H) : 0u;
-            if (take1 + take2 + c == 0) break;
-            drain_some<SL>(S);
-            // new hits: A's r .. r + take1 - 1 to lanes c .. c + take1 - 1,
-            // then B's 0 .. take2 - 1; segment starts are tagged with their
-            // source position + 1 (B's positions count from 64)
+            const uint32_t t1 = min((uint32_t)HB_WAVE, A.H - r);
+            const uint32_t t2 = hasB ? min((uint32_t)HB_WAVE - t1, B.H) : 0u;
+            n_a = t1;
+            n_st = t1 + t2;
+            if (n_st == 0) return;
+            // A's hits r .. r + t1 - 1 to lanes 0 .. t1 - 1, then B's 0 ..
+            // t2 - 1; segment starts are tagged with their source position
+            // + 1 (B's positions count from 64)
             if (++rgen == (1u << 24)) {
                 seg[lane] = 0;
                 rgen = 1;
             }
             const uint32_t tag = rgen << 8;
             const uint32_t ea = A.excl - r;
-            if (A.cnt && ea < take1) seg[ea + c] = tag | ((uint32_t)lane + 1);
-            if (take2 && B.cnt && B.excl < take2)
-                seg[B.excl + take1 + c] = tag | ((uint32_t)lane + 65);
+            if (ea < t1) seg[ea] = tag | ((uint32_t)lane + 1);
+            if (t2 && B.excl < t2) seg[B.excl + t1] = tag | ((uint32_t)lane + 65);
             sync();
             const uint32_t sv = seg[lane] - tag;
-            // lanes before the first segment start continue the previous
-            // round's last segment (carry); the source of each lane is the
-            // largest start at or before it
+            // lanes before the first segment start continue the last taken
+            // hit's segment (carry); the source of each lane is the largest
+            // start at or before it
             uint32_t v = sv < 256u ? sv : 0u;
             if (lane == 0 && v == 0) v = carry + 1;
             v = wave_incl_max(v);
             const uint32_t p = v - 1;
-            const uint32_t isnew_hi = c + take1 + take2;
-            carry = (uint32_t)__builtin_amdgcn_readlane((int)p, (int)(isnew_hi - 1));
-            const bool isnew = (uint32_t)lane >= c && (uint32_t)lane < isnew_hi;
-            const bool inb = (uint32_t)lane >= c + take1;
             const int p4 = (int)((p & 63u) << 2);
             int32_t eb = __builtin_amdgcn_ds_bpermute(p4, A.eb);
-            int32_t smp = __builtin_amdgcn_ds_bpermute(p4, A.sm);
-            if (take2) {   // B's position p - 64 is A's lane p - 64 plus 64
+            if (t2) {   // B's position p - 64 (its lane p - 64)
                 const int32_t ebb = __builtin_amdgcn_ds_bpermute(p4, B.eb);
-                int32_t smb = smp + HB_WAVE;
-                if (smb >= D) smb -= D;
                 eb = p >= 64 ? ebb : eb;
-                smp = p >= 64 ? smb : smp;
             }
-            const uint32_t lc = (uint32_t)lane - c;
-            const uint32_t flat = inb ? lc - take1 : lc + r;
-            uint32_t qn = 0;
-            if (isnew) qn = entries[(uint32_t)((int32_t)flat + eb)];
+            // (s0 + p) mod D, p < 128 <= 2 D
+            int32_t smp = A.s0m + (int32_t)p;
+            if (smp >= D) smp -= D;
+            if (smp >= D) smp -= D;
+            const uint32_t flat = (uint32_t)lane >= t1 ? (uint32_t)lane - t1 : (uint32_t)lane + r;
+            eq = 0;
+            if ((uint32_t)lane < n_st) eq = entries[(uint32_t)((int32_t)flat + eb)];
+            eso = A.s0 + (int32_t)p;
+            esm = smp;
+            sync();
+        };
+        stage();
+        for (;;) {
+            if (n_st + c == 0) break;
+            drain_some<SL>(S);
+            // this round takes the first m staged hits into lanes c .. c + m - 1
+            const uint32_t m = min(n_st, (uint32_t)HB_WAVE - c);
+            const int src4 = (int)((((uint32_t)lane - c) & 63u) << 2);
+            const uint32_t qn = (uint32_t)__builtin_amdgcn_ds_bpermute(src4, (int)eq);
+            const int32_t son = __builtin_amdgcn_ds_bpermute(src4, eso);
+            const int32_t smp = __builtin_amdgcn_ds_bpermute(src4, esm);
+            if (m) {
+                // the source of the last hit taken, in A's positions (B's
+                // count from 64), then the cursor moves past it
+                carry = (uint32_t)(__builtin_amdgcn_readlane(eso, (int)(m - 1)) - A.s0);
+                if (m <= n_a) {
+                    r += m;
+                } else {
+                    A = B;
+                    hasB = false;
+                    r = m - n_a;
+                    carry -= 64;
+                }
+            }
+            stage();
             int32_t dn = (int32_t)qn - smp;
             if (dn < 0) dn += D;
             const bool isc = (uint32_t)lane < c;
+            const bool isnew = (uint32_t)lane >= c && (uint32_t)lane < c + m;
             const uint32_t q = isc ? cq : qn;
-            const int32_t so = isc ? cs_ : A.s0 + (int32_t)p;
+            const int32_t so = isc ? cs_ : son;
             const uint32_t d = isc ? cd : (uint32_t)dn;
-            sync();
             const bool lost = decide_pass(isc || isnew, q, so, d, sbase, sl);
             c = carry_losers(lost, q, so, d, cq, cs_, cd);
-            r += take1;
-            if (take2) {   // the round ended in B: it becomes the current window
-                A = B;
-                hasB = false;
-                r = take2;
-                carry -= 64;
-            }
         }
         return hits;
     }
