@@ -357,24 +357,35 @@ def profile_figures():
 def measure_dropin(ctx, db, spec, reps=10):
     """The drop-in per query as H-BLAST's H_BLAST_pre + H_BLAST_get_data
     pair does it: prepare (lookup table on the device), scan, and results
-    (per-subject stats and ordered HSPs copied to the host), after one
-    warm-up; mean wall time per query."""
-    def one():
+    (per-subject stats of every subject and the ordered HSPs copied to the
+    host), after one warm-up; mean wall time per query.  The same loop with
+    the compact results (entries of the subjects with HSPs, totals) is
+    reported beside it."""
+    def one(compact):
         q = hbseed.Query(ctx, spec)
         s = hbseed.Scan(ctx, db, q)
-        st, h = s.results()
+        r = s.results_compact() if compact else s.results()
         s.close()
         q.close()
-        return len(h)
-    one()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        one()
-    ctx.synchronize()
-    dt = (time.perf_counter() - t0) / reps
-    return {"queries_per_s": 1.0 / dt, "ms_per_query": dt * 1e3, "reps": reps,
-            "steps": "prepare (lookup table) + scan + results (stats and HSPs to the host)"}
+        return len(r[1])
+    out = {}
+    for compact in (False, True):
+        one(compact)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            one(compact)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        if compact:
+            out["compact"] = {"queries_per_s": 1.0 / dt, "ms_per_query": dt * 1e3,
+                              "steps": "prepare + scan + compact results (subjects with "
+                                       "HSPs, totals, HSPs to the host)"}
+        else:
+            out.update({"queries_per_s": 1.0 / dt, "ms_per_query": dt * 1e3, "reps": reps,
+                        "steps": "prepare (lookup table) + scan + results (stats of every "
+                                 "subject and HSPs to the host)"})
+    return out
 
 
 def measure_blastx(ctx, db, res, offs, reps=5):
@@ -531,6 +542,11 @@ def main():
     tr0 = time.perf_counter()
     stats, hsps = last.results()
     results_ms = (time.perf_counter() - tr0) * 1e3
+    # the compact form: entries of the subjects with HSPs and the totals
+    last.results_compact()
+    tr0 = time.perf_counter()
+    subj_c, _, _ = last.results_compact()
+    results_compact_ms = (time.perf_counter() - tr0) * 1e3
     gapped = None
     if not args.no_gapped:
         log("gapped extension of the last pass's HSPs")
@@ -620,6 +636,8 @@ def main():
         "kernel_ms": mean_kernel_ms,
         "results_fetch_ms": results_ms,
         "results_fetch_first_ms": results_first_ms,
+        "results_compact_ms": results_compact_ms,
+        "subjects_with_hsps": int(len(subj_c)),
         "hsps": int(len(hsps)),
         "lookup_hits": int(stats["lookup_hits"].sum()),
         "extensions": int(stats["extensions"].sum()),

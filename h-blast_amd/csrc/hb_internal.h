@@ -155,6 +155,15 @@ struct hb_fin_big {
 hipError_t hb_launch_finalize(const uint32_t *st3, uint32_t ns, const hb_hsp *recs, uint32_t n,
                               hb_subject_stats *stats_out, hb_hsp *recs_out, uint32_t *scratch,
                               hb_fin_big *big, uint32_t big_cap, uint32_t *n_big, hipStream_t st);
+// The compact layout: entries of the subjects with HSPs in oid order
+// (subj_out, at most min(ns, n)), the records grouped and ordered as above,
+// tot[0..1] += the lookup hits and extensions of all subjects (tot cleared
+// by the caller).  With nb = (ns + 4095) / 4096, scratch holds 2 nb + 1 + ns
+// uint32, and the number of subjects with HSPs ends up in scratch[2 nb].
+hipError_t hb_launch_finalize_compact(const uint32_t *st3, uint32_t ns, const hb_hsp *recs,
+                                      uint32_t n, hb_subject_hits *subj_out, hb_hsp *recs_out,
+                                      uint32_t *scratch, unsigned long long *tot, hb_fin_big *big,
+                                      uint32_t big_cap, uint32_t *n_big, hipStream_t st);
 // First-record indices of per-subject counts st3[3i + 2] (hb_kernels.hip):
 // first[i] = sum of the counts of subjects < i.  scratch: ns / 4096 + 1.
 hipError_t hb_launch_first_index(const uint32_t *st3, uint32_t ns, uint32_t *first,

@@ -1216,8 +1216,10 @@ __global__ __launch_bounds__(kFinThreads) void fin_sum_kernel(const uint32_t *st
     if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-// One block: exclusive scan of the nb block totals in place.
-__global__ __launch_bounds__(kFinThreads) void fin_part_kernel(uint32_t *part, uint32_t nb)
+// One block: exclusive scan of the nb block totals in place; the grand
+// total goes to *total_out when it is given.
+__global__ __launch_bounds__(kFinThreads) void fin_part_kernel(uint32_t *part, uint32_t nb,
+                                                               uint32_t *total_out)
 {
     __shared__ uint32_t sm[kFinThreads / HB_WAVE + 1];
     uint32_t carry = 0;
@@ -1228,6 +1230,83 @@ __global__ __launch_bounds__(kFinThreads) void fin_part_kernel(uint32_t *part, u
         const uint32_t e = block_excl_scan(v, sm, &total);
         if (i < nb) part[i] = carry + e;
         carry += total;
+    }
+    if (total_out && threadIdx.x == 0) *total_out = carry;
+}
+
+// Compact layout, first pass: per block, the HSP count and the number of
+// subjects with HSPs; the launch's lookup hits and extensions are summed
+// into tot[0], tot[1] (one 64-bit atomic per wavefront).
+__global__ __launch_bounds__(kFinThreads) void fin_sum2_kernel(const uint32_t *st3, uint32_t ns,
+                                                               uint32_t *part, uint32_t *part_s,
+                                                               unsigned long long *tot)
+{
+    __shared__ uint32_t sm[kFinThreads / HB_WAVE + 1];
+    uint32_t v = 0, nz = 0;
+    unsigned long long lh = 0, ex = 0;
+    for (int k = 0; k < kFinPer; ++k) {
+        const uint32_t i = blockIdx.x * kFinSpan + (uint32_t)k * kFinThreads + threadIdx.x;
+        if (i < ns) {
+            const uint32_t c = st3[3 * (size_t)i + 2];
+            v += c;
+            nz += c != 0;
+            lh += st3[3 * (size_t)i];
+            ex += st3[3 * (size_t)i + 1];
+        }
+    }
+    uint32_t total = 0;
+    block_excl_scan(v, sm, &total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+    block_excl_scan(nz, sm, &total);
+    if (threadIdx.x == 0) part_s[blockIdx.x] = total;
+    for (int o = HB_WAVE / 2; o > 0; o >>= 1) {
+        lh += __shfl_xor(lh, o, HB_WAVE);
+        ex += __shfl_xor(ex, o, HB_WAVE);
+    }
+    if ((threadIdx.x & (HB_WAVE - 1)) == 0) {
+        atomicAdd(&tot[0], lh);
+        atomicAdd(&tot[1], ex);
+    }
+}
+
+// Compact layout, second pass: every subject with HSPs gets its entry at
+// its rank among them (so in oid order), and every subject its first
+// record index in cursor[] for the scatter.
+__global__ __launch_bounds__(kFinThreads) void fin_compact_kernel(const uint32_t *st3, uint32_t ns,
+                                                                  const uint32_t *part,
+                                                                  const uint32_t *part_s,
+                                                                  hb_subject_hits *out,
+                                                                  uint32_t *cursor)
+{
+    __shared__ uint32_t sm[kFinThreads / HB_WAVE + 1];
+    const uint32_t i0 = blockIdx.x * kFinSpan + (uint32_t)threadIdx.x * kFinPer;
+    uint32_t cnt[kFinPer];
+    uint32_t v = 0, nz = 0;
+#pragma unroll
+    for (int k = 0; k < kFinPer; ++k) {
+        cnt[k] = i0 + k < ns ? st3[3 * (size_t)(i0 + k) + 2] : 0u;
+        v += cnt[k];
+        nz += cnt[k] != 0;
+    }
+    uint32_t total = 0;
+    uint32_t first = part[blockIdx.x] + block_excl_scan(v, sm, &total);
+    uint32_t rank = part_s[blockIdx.x] + block_excl_scan(nz, sm, &total);
+#pragma unroll
+    for (int k = 0; k < kFinPer; ++k) {
+        const uint32_t i = i0 + k;
+        if (i < ns) {
+            cursor[i] = first;
+            if (cnt[k]) {
+                hb_subject_hits o;
+                o.oid = i;
+                o.n_hsp = cnt[k];
+                o.lookup_hits = st3[3 * (size_t)i];
+                o.extensions = st3[3 * (size_t)i + 1];
+                o.first = first;
+                out[rank++] = o;
+            }
+        }
+        first += cnt[k];
     }
 }
 
@@ -1295,19 +1374,15 @@ __global__ void fin_scatter_kernel(const hb_hsp *in, uint32_t n, uint32_t *curso
     }
 }
 
-// One thread per subject: insertion sort of its records; subjects with more
-// than HB_FIN_SORT_MAX records are listed for the host.
-__global__ void fin_sort_kernel(const hb_subject_stats *st, uint32_t ns, hb_hsp *recs,
-                                hb_fin_big *big, uint32_t big_cap, uint32_t *n_big)
+// Insertion sort of one subject's n records at recs + f by (s_off, q_off);
+// a subject with more than HB_FIN_SORT_MAX records is listed for the host.
+__device__ void sort_subject(uint32_t oid, uint32_t n, uint64_t f, hb_hsp *recs, hb_fin_big *big,
+                             uint32_t big_cap, uint32_t *n_big)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    const uint32_t n = st[i].n_hsp;
     if (n < 2) return;
-    const uint64_t f = (uint64_t)st[i].first;
     if (n > HB_FIN_SORT_MAX) {
         const uint32_t k = atomicAdd(n_big, 1u);
-        if (k < big_cap) big[k] = hb_fin_big{i, n, f};
+        if (k < big_cap) big[k] = hb_fin_big{oid, n, f};
         return;
     }
     hb_hsp *r = recs + f;
@@ -1321,6 +1396,23 @@ __global__ void fin_sort_kernel(const hb_subject_stats *st, uint32_t ns, hb_hsp 
         }
         r[b] = x;
     }
+}
+
+// One thread per subject (dense layout).
+__global__ void fin_sort_kernel(const hb_subject_stats *st, uint32_t ns, hb_hsp *recs,
+                                hb_fin_big *big, uint32_t big_cap, uint32_t *n_big)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ns) sort_subject(i, st[i].n_hsp, (uint64_t)st[i].first, recs, big, big_cap, n_big);
+}
+
+// One thread per subject with HSPs (compact layout; their number is *n_subj).
+__global__ void fin_sort_compact_kernel(const hb_subject_hits *sh, const uint32_t *n_subj,
+                                        hb_hsp *recs, hb_fin_big *big, uint32_t big_cap,
+                                        uint32_t *n_big)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *n_subj) sort_subject(sh[i].oid, sh[i].n_hsp, sh[i].first, recs, big, big_cap, n_big);
 }
 
 // Plain 16-byte-per-lane copy: the known-size reference kernel for
@@ -1360,7 +1452,7 @@ hipError_t hb_launch_finalize(const uint32_t *st3, uint32_t ns, const hb_hsp *re
     uint32_t *part = scratch;        // nb
     uint32_t *cursor = scratch + nb; // ns
     hipLaunchKernelGGL(fin_sum_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, part);
-    hipLaunchKernelGGL(fin_part_kernel, dim3(1), dim3(kFinThreads), 0, st, part, nb);
+    hipLaunchKernelGGL(fin_part_kernel, dim3(1), dim3(kFinThreads), 0, st, part, nb, nullptr);
     hipLaunchKernelGGL(fin_write_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, part,
                        stats_out, cursor);
     if (n) {
@@ -1372,13 +1464,40 @@ hipError_t hb_launch_finalize(const uint32_t *st3, uint32_t ns, const hb_hsp *re
     return hipGetLastError();
 }
 
+hipError_t hb_launch_finalize_compact(const uint32_t *st3, uint32_t ns, const hb_hsp *recs,
+                                      uint32_t n, hb_subject_hits *subj_out, hb_hsp *recs_out,
+                                      uint32_t *scratch, unsigned long long *tot, hb_fin_big *big,
+                                      uint32_t big_cap, uint32_t *n_big, hipStream_t st)
+{
+    if (ns == 0) return hipSuccess;
+    const uint32_t nb = (ns + kFinSpan - 1) / kFinSpan;
+    uint32_t *part = scratch;              // nb
+    uint32_t *part_s = scratch + nb;       // nb
+    uint32_t *n_subj = scratch + 2 * nb;   // 1
+    uint32_t *cursor = n_subj + 1;         // ns
+    hipLaunchKernelGGL(fin_sum2_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, part, part_s,
+                       tot);
+    hipLaunchKernelGGL(fin_part_kernel, dim3(1), dim3(kFinThreads), 0, st, part, nb, nullptr);
+    hipLaunchKernelGGL(fin_part_kernel, dim3(1), dim3(kFinThreads), 0, st, part_s, nb, n_subj);
+    hipLaunchKernelGGL(fin_compact_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, part,
+                       part_s, subj_out, cursor);
+    if (n) {
+        hipLaunchKernelGGL(fin_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, st, recs, n,
+                           cursor, recs_out);
+        const uint32_t grid = ((ns < n ? ns : n) + 255) / 256;
+        hipLaunchKernelGGL(fin_sort_compact_kernel, dim3(grid), dim3(256), 0, st, subj_out, n_subj,
+                           recs_out, big, big_cap, n_big);
+    }
+    return hipGetLastError();
+}
+
 hipError_t hb_launch_first_index(const uint32_t *st3, uint32_t ns, uint32_t *first,
                                  uint32_t *scratch, hipStream_t st)
 {
     if (ns == 0) return hipSuccess;
     const uint32_t nb = (ns + kFinSpan - 1) / kFinSpan;
     hipLaunchKernelGGL(fin_sum_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, scratch);
-    hipLaunchKernelGGL(fin_part_kernel, dim3(1), dim3(kFinThreads), 0, st, scratch, nb);
+    hipLaunchKernelGGL(fin_part_kernel, dim3(1), dim3(kFinThreads), 0, st, scratch, nb, nullptr);
     hipLaunchKernelGGL(fin_first_kernel, dim3(nb), dim3(kFinThreads), 0, st, st3, ns, scratch,
                        first);
     return hipGetLastError();

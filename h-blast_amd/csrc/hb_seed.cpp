@@ -1438,6 +1438,104 @@ int hb_scan_results(hb_scan *s, hb_subject_stats *stats, hb_hsp *hsps, uint64_t 
     return HB_OK;
 }
 
+int hb_scan_results_compact(hb_scan *s, hb_subject_hits *subj, uint64_t subj_cap, hb_hsp *hsps,
+                            uint64_t hsp_cap, hb_scan_totals *totals)
+{
+    if (!s || !totals) return HB_ERR_ARG;
+    int rc = hb_scan_wait(s);
+    if (rc) return rc;
+    std::memset(totals, 0, sizeof(*totals));
+    const uint64_t n = s->n_out;
+    totals->n_hsp = n;
+    const uint32_t ns = s->db->n;
+    if (ns == 0) return HB_OK;
+    hb_ctx *c = s->ctx;
+    HB_CHECK(hipSetDevice(c->device));
+    // the same device layout as hb_scan_results, with the per-subject
+    // entries compacted to the subjects with HSPs
+    constexpr uint32_t kBig = 4096;
+    const uint32_t nb = (ns + 4095) / 4096;
+    const uint64_t max_subj = std::min<uint64_t>(ns, n);
+    const size_t b_sub = (size_t)std::max<uint64_t>(max_subj, 1) * sizeof(hb_subject_hits);
+    const size_t b_rec = (size_t)std::max<uint64_t>(n, 1) * sizeof(hb_hsp);
+    const size_t b_scr = (2 * (size_t)nb + 1 + ns) * sizeof(uint32_t);
+    const size_t b_big = kBig * sizeof(hb_fin_big);
+    const size_t b_small = 32;   // two 64-bit totals, the big-subject count
+    auto *d_sub = static_cast<hb_subject_hits *>(hb_pool_get(c, b_sub));
+    auto *d_rec = static_cast<hb_hsp *>(hb_pool_get(c, b_rec));
+    auto *d_scr = static_cast<uint32_t *>(hb_pool_get(c, b_scr));
+    auto *d_big = static_cast<hb_fin_big *>(hb_pool_get(c, b_big));
+    auto *d_small = static_cast<unsigned long long *>(hb_pool_get(c, b_small));
+    auto release = [&]() {
+        hb_pool_put(c, d_sub, b_sub);
+        hb_pool_put(c, d_rec, b_rec);
+        hb_pool_put(c, d_scr, b_scr);
+        hb_pool_put(c, d_big, b_big);
+        hb_pool_put(c, d_small, b_small);
+    };
+    if (!d_sub || !d_rec || !d_scr || !d_big || !d_small) {
+        release();
+        return HB_ERR_NOMEM;
+    }
+    uint32_t *d_nbig = reinterpret_cast<uint32_t *>(d_small + 2);
+    hipStream_t st = c->stream;
+    unsigned long long small[3] = {0, 0, 0};
+    uint32_t n_subj = 0;
+    hipError_t e = hb_launch_fill_u32(reinterpret_cast<uint32_t *>(d_small), 0u, 6, st);
+    if (e == hipSuccess)
+        e = hb_launch_finalize_compact(s->d_stats, ns, s->d_out, (uint32_t)n, d_sub, d_rec, d_scr,
+                                       d_small, d_big, kBig, d_nbig, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(small, d_small, sizeof(small), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&n_subj, d_scr + 2 * nb, sizeof(n_subj), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const uint32_t nbig = (uint32_t)small[2];
+    if (e == hipSuccess) {
+        totals->lookup_hits = small[0];
+        totals->extensions = small[1];
+        totals->n_subjects = n_subj;
+        if ((subj && subj_cap < n_subj) || (hsps && hsp_cap < n)) {
+            release();
+            return HB_ERR_CAPACITY;
+        }
+    }
+    std::vector<hb_subject_hits> host_sub;
+    hb_subject_hits *sp = subj;
+    if (e == hipSuccess && !sp && hsps && nbig > kBig) {   // the host needs the entries below
+        host_sub.resize(n_subj);
+        sp = host_sub.data();
+    }
+    if (e == hipSuccess && sp && n_subj &&
+        hb_copy_out(c, sp, d_sub, (size_t)n_subj * sizeof(hb_subject_hits), st) != HB_OK)
+        e = hipErrorUnknown;
+    if (e == hipSuccess && hsps && n &&
+        hb_copy_out(c, hsps, d_rec, (size_t)n * sizeof(hb_hsp), st) != HB_OK)
+        e = hipErrorUnknown;
+    // subjects with many records are ordered here
+    std::vector<hb_fin_big> big;
+    if (e == hipSuccess && hsps && nbig) {
+        if (nbig <= kBig) {
+            big.resize(nbig);
+            e = hipMemcpy(big.data(), d_big, nbig * sizeof(hb_fin_big), hipMemcpyDeviceToHost);
+        } else {
+            for (uint32_t i = 0; i < n_subj; ++i)
+                if (sp[i].n_hsp > HB_FIN_SORT_MAX)
+                    big.push_back(hb_fin_big{sp[i].oid, sp[i].n_hsp, sp[i].first});
+        }
+    }
+    release();
+    if (e != hipSuccess) {
+        hb_note_hip_error(e, "hb_scan_results_compact", __LINE__);
+        return HB_ERR_DEVICE;
+    }
+    for (const hb_fin_big &b : big)
+        std::sort(hsps + b.first, hsps + b.first + b.n, [](const hb_hsp &x, const hb_hsp &y) {
+            if (x.s_off != y.s_off) return x.s_off < y.s_off;
+            return x.q_off < y.q_off;
+        });
+    return HB_OK;
+}
+
 void hb_gap_params_default(hb_gap_params *p)
 {
     if (!p) return;

@@ -43,6 +43,9 @@ GHSP_DTYPE = np.dtype([("oid", "<u4"), ("context", "<i4"), ("q_start", "<i4"),
                        ("q_gstart", "<i4"), ("s_gstart", "<i4"), ("score", "<i4"),
                        ("pad", "<i4")])
 GSTATS_DTYPE = np.dtype([("extensions", "<u4"), ("n_hsp", "<u4"), ("first", "<i8")])
+# hb_subject_hits: one entry per subject with HSPs (hb_scan_results_compact)
+SUBJ_HITS_DTYPE = np.dtype([("oid", "<u4"), ("n_hsp", "<u4"), ("lookup_hits", "<u4"),
+                            ("extensions", "<u4"), ("first", "<u8")])
 DEFAULT_GAP_OPEN = 11
 DEFAULT_GAP_EXTEND = 1
 DEFAULT_GAP_XDROP = 38
@@ -56,7 +59,7 @@ EXPORTS = [
     "hb_encode_protein", "hb_query_prepare_masked", "hb_query_set_state_mode",
     "hb_query_set_reset_interval", "hb_scan_noalloc", "hb_check_memory", "hb_cosched_run",
     "hb_cosearch",
-    "hb_query_state_mode", "hb_scan_state_mode", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results",
+    "hb_query_state_mode", "hb_scan_state_mode", "hb_batch_add_masked", "hb_scan_start", "hb_scan_wait", "hb_scan_results", "hb_scan_results_compact",
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
     "hb_fasta_parse", "hb_hdb_gpu_seqs", "hb_hdb_layout", "hb_hdb_pack", "hb_hdbi_format",
@@ -207,6 +210,7 @@ def _lib():
         L.hb_scan_start.argtypes = [vp, vp, vp, pp]
         L.hb_scan_wait.argtypes = [vp]
         L.hb_scan_results.argtypes = [vp, vp, vp, u64, C.POINTER(u64)]
+        L.hb_scan_results_compact.argtypes = [vp, vp, u64, vp, u64, vp]
         L.hb_scan_elapsed_ms.argtypes = [vp, C.POINTER(C.c_float)]
         L.hb_scan_free.argtypes = [vp]
         L.hb_diag_copy.argtypes = [vp, u64, C.c_int, C.POINTER(C.c_float)]
@@ -916,6 +920,19 @@ class Scan:
         _check(_lib().hb_scan_results(self.h, _ptr(stats), _ptr(hsps), n.value, C.byref(n)),
                "hb_scan_results")
         return stats, hsps
+
+    def results_compact(self):
+        """-> (subjects[n_subjects with HSPs], hsps[n], totals dict): the
+        compact results (hb_scan_results_compact)."""
+        n = C.c_uint64()
+        _check(_lib().hb_scan_results(self.h, None, None, 0, C.byref(n)), "hb_scan_results")
+        subj = np.zeros(max(n.value, 1), SUBJ_HITS_DTYPE)
+        hsps = np.zeros(n.value, HSP_DTYPE)
+        tot = (C.c_uint64 * 4)()
+        _check(_lib().hb_scan_results_compact(self.h, _ptr(subj), len(subj), _ptr(hsps),
+                                              n.value, tot), "hb_scan_results_compact")
+        totals = dict(zip(("lookup_hits", "extensions", "n_hsp", "n_subjects"), list(tot)))
+        return subj[:totals["n_subjects"]].copy(), hsps, totals
 
     def gapped(self, cutoffs, gap_open=DEFAULT_GAP_OPEN, gap_extend=DEFAULT_GAP_EXTEND,
                x_dropoff=DEFAULT_GAP_XDROP):

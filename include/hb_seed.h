@@ -494,6 +494,33 @@ int hb_scan_start(hb_ctx *ctx, hb_db *db, hb_query *q, hb_scan **out);
 int hb_scan_wait(hb_scan *s);
 int hb_scan_results(hb_scan *s, hb_subject_stats *stats, hb_hsp *hsps,
                     uint64_t cap, uint64_t *n_hsp);
+
+/* The compact form of the same results, for a consumer that, like
+ * blast_engine.c:540-586, needs per-subject data only for the subjects with
+ * HSPs and the ungapped statistics only as sums (Blast_UngappedStatsUpdate
+ * accumulates lookup hits and extensions).  The reference's get_data copies
+ * 16 B per database subject (h_Hits, gpu_cpu_common.h:56-60); this copies
+ * 24 B per subject with HSPs.  subj gets one entry per subject with HSPs in
+ * oid order (first: index of its first record in hsps), hsps every HSP as
+ * hb_scan_results orders them, totals the sums over all subjects.  There
+ * are at most n_hsp subjects with HSPs (hb_scan_results with NULL arrays
+ * gives n_hsp), so arrays of n_hsp entries always suffice; with smaller
+ * capacities totals is still filled and HB_ERR_CAPACITY returned. */
+typedef struct hb_subject_hits {
+    uint32_t oid;
+    uint32_t n_hsp;
+    uint32_t lookup_hits;
+    uint32_t extensions;
+    uint64_t first;
+} hb_subject_hits;
+typedef struct hb_scan_totals {
+    uint64_t lookup_hits;    /* word hits over all subjects */
+    uint64_t extensions;     /* two-hit extensions over all subjects */
+    uint64_t n_hsp;          /* HSPs */
+    uint64_t n_subjects;     /* subjects with HSPs */
+} hb_scan_totals;
+int hb_scan_results_compact(hb_scan *s, hb_subject_hits *subj, uint64_t subj_cap,
+                            hb_hsp *hsps, uint64_t hsp_cap, hb_scan_totals *totals);
 /* Device time of the scan kernel(s) in milliseconds (hipEvent pair on the
  * stream the kernels run on); replaces get_gpu_walltime
  * (gpu_cpu_common.h:71). */

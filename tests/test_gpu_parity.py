@@ -278,3 +278,51 @@ def test_smallest_left_walk_crosses_offset_zero(ctx):
     offs[51] = offs[50] + 6
     offs[52] = offs[51] + 7
     compare(ctx, spec, res, offs, matrix=m)
+
+
+def _compact_equals_dense(ctx, spec, res, offs):
+    db = hbseed.Database(ctx, res, offs)
+    q = hbseed.Query(ctx, spec)
+    try:
+        s = hbseed.Scan(ctx, db, q)
+        stats, hsps = s.results()
+        subj, hsps_c, tot = s.results_compact()
+        s.close()
+    finally:
+        q.close()
+        db.close()
+    assert np.array_equal(hsps_c.view(np.int32).reshape(-1, 8), hsps.view(np.int32).reshape(-1, 8))
+    with_hsps = np.nonzero(stats["n_hsp"])[0]
+    assert np.array_equal(subj["oid"], with_hsps)
+    for f in ("n_hsp", "lookup_hits", "extensions"):
+        assert np.array_equal(subj[f], stats[f][with_hsps]), f
+    assert np.array_equal(subj["first"].astype(np.int64), stats["first"][with_hsps])
+    assert tot == {"lookup_hits": int(stats["lookup_hits"].sum(dtype=np.uint64)),
+                   "extensions": int(stats["extensions"].sum(dtype=np.uint64)),
+                   "n_hsp": len(hsps), "n_subjects": len(with_hsps)}
+    return subj, tot
+
+
+def test_compact_results_equal_dense(ctx):
+    """hb_scan_results_compact: the entries of the subjects with HSPs (in oid
+    order) equal hb_scan_results' per-subject stats of those subjects, the
+    totals equal the sums over every subject, and the records are the same;
+    also with subjects of more than 256 records (host ordering)."""
+    q = hbseed.random_protein(7501, 500)
+    res, offs = hbseed.make_db(7500, 30000, plant_query=q, plant_frac=0.05)
+    subj, tot = _compact_equals_dense(ctx, hbseed.blastp_query(q), res, offs)
+    assert 0 < tot["n_subjects"] < len(offs) - 1
+    q = hbseed.random_protein(7101, 400)
+    parts = []
+    for k in range(420):
+        a = (37 * k) % 340
+        parts.append(hbseed.homolog(7200 + k, q[a:a + 60], 0.9, 0))
+        parts.append(hbseed.random_protein(7300 + k, 7))
+    big = np.concatenate(parts).astype(np.uint8)
+    small = hbseed.random_protein(7400, 300)
+    res = np.concatenate([small, big, small, big]).astype(np.uint8)
+    lens = [len(small), len(big), len(small), len(big)]
+    offs = np.zeros(len(lens) + 1, np.uint64)
+    np.cumsum(lens, out=offs[1:])
+    subj, tot = _compact_equals_dense(ctx, hbseed.blastp_query(q), res, offs)
+    assert subj["n_hsp"].max() > 256
