@@ -475,10 +475,11 @@ struct WaveScan {
     // lane; active lanes are in subject order by lane).  Among the hits of
     // the set that share a diagonal only the lowest lane decides: each hit
     // raises the marker bits of its diagonal's word to 64 - lane with an
-    // atomic max (the max clears the state bits, and the lane that finds the
-    // marker still 0 ORs the state back), the hit that finds its own marker
-    // after the pass's sync decides, and its store of the new state clears
-    // the marker.  A winner whose diagonal waits on a pending extension joins
+    // atomic max (which returns the state to the first hit to arrive and
+    // clears the state bits), the hit that finds its own marker after the
+    // pass's sync decides, and its store of the new state clears the
+    // marker.  The winner is usually the first to arrive; when it is not,
+    // the first hit ORs the state back into the word for it.  A winner whose diagonal waits on a pending extension joins
     // the blocked list.  The losers then read the winner's new state: a
     // loser whose diagonal is now pending joins the blocked list behind the
     // winner, and a loser that would skip against that state is done (a skip
@@ -500,13 +501,19 @@ struct WaveScan {
         const uint32_t mk = (uint32_t)(HB_WAVE - lane);
         word_t old = 0;
         if (active) old = w_max(d, (word_t)mk << W::kMS);
-        // the first lane to reach the word finds no marker: it ORs the state
-        // back (nothing to do when the state is 0)
-        if (active && (word_t)(old - 1) < ((word_t)1 << W::kMS) - 1) w_or(d, old);
         sync();
         const word_t wv = w_ld(d);
         const bool win = active && (uint32_t)(wv >> W::kMS) == mk;
-        const uint32_t st = (uint32_t)wv & W::kSM;
+        // the first hit to reach the word found no marker and holds the state
+        // (the max cleared the state bits for the hits after it); a winner
+        // that came later gets it from that hit through the word
+        const bool first = active && (uint32_t)(old >> W::kMS) == 0;
+        uint32_t st = (uint32_t)old & W::kSM;
+        if (__ballot(win && !first)) {
+            if (first && !win) w_or(d, old);
+            sync();
+            if (win && !first) st = (uint32_t)w_ld(d) & W::kSM;
+        }
         const int32_t sb = so + sbase;
         // s - last on an idle diagonal; negative (below -kP + sb) when the
         // diagonal is pending or flagged
