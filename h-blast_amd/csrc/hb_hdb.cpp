@@ -19,6 +19,7 @@
 // that does not exist, :1058-1066).
 #include <algorithm>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -266,5 +267,177 @@ extern "C" int hb_hdbi_parse(const char *text, uint64_t len, hb_hdb_index *index
     while (L.next(b, n))
         if (n) return fail();
     *n_group_len = ng_total;
+    return HB_OK;
+}
+
+// ------------------------------------------------------------------ sorted volumes
+// H-BLAST's makeblastdb with -sort_volumes (CWriteDB_Volume::WriteSequenceSort,
+// ncbi_blast_files/writedb_volume.cpp:255-513) fills a volume with the input
+// sequences in input order until the next one would overfill its files, then
+// sorts the volume with std::sort by descending length (compare, :159-164) and
+// writes it from the back, so shortest first.  The sequence file's rule
+// (CWriteDB_SequenceFile::CanFit and UpdateSize,
+// ncbi_blast_files/writedb_files.hpp:435-494): the file starts at offset 1,
+// each sequence adds its length + 1 (a NUL after it), a volume's first
+// sequence always fits, and a later one fits while offset + length <
+// max_file_sz and, with a letter limit, letters + length <= the limit.
+extern "C" int hb_db_sort_volumes(const uint64_t *offsets, int32_t n_seqs,
+                                  uint64_t max_file_bytes, uint64_t max_letters, int32_t *order,
+                                  int32_t *volume_seqs, int32_t volume_cap, int32_t *n_volumes)
+{
+    if (!offsets || n_seqs < 0 || !n_volumes || max_file_bytes < 2 || (n_seqs > 0 && !order) ||
+        volume_cap < 0)
+        return HB_ERR_ARG;
+    std::vector<int32_t> cut;
+    uint64_t off = 1, letters = 0;
+    int32_t cnt = 0;
+    for (int32_t i = 0; i < n_seqs; ++i) {
+        if (offsets[i + 1] < offsets[i]) return HB_ERR_ARG;
+        const uint64_t len = offsets[i + 1] - offsets[i];
+        if (cnt > 0 && (off + len >= max_file_bytes || (max_letters && letters + len > max_letters))) {
+            cut.push_back(cnt);
+            off = 1;
+            letters = 0;
+            cnt = 0;
+        }
+        off += len + 1;
+        letters += len;
+        ++cnt;
+    }
+    if (cnt > 0) cut.push_back(cnt);
+    *n_volumes = (int32_t)cut.size();
+    if (volume_seqs) {
+        if (volume_cap < (int32_t)cut.size()) return HB_ERR_CAPACITY;
+        std::copy(cut.begin(), cut.end(), volume_seqs);
+    }
+    // the reference's sort on the lengths alone: std::sort moves elements by
+    // the comparisons' outcomes only, so the permutation is the one it
+    // applies to its sequence objects
+    struct Item {
+        uint64_t len;
+        int32_t idx;
+    };
+    std::vector<Item> v;
+    int32_t first = 0;
+    for (int32_t c : cut) {
+        v.clear();
+        for (int32_t i = first; i < first + c; ++i) v.push_back(Item{offsets[i + 1] - offsets[i], i});
+        std::sort(v.begin(), v.end(), [](const Item &a, const Item &b) { return a.len > b.len; });
+        for (int32_t k = 0; k < c; ++k) order[first + k] = v[(size_t)(c - 1 - k)].idx;
+        first += c;
+    }
+    return HB_OK;
+}
+
+// Writes an H-BLAST GPU database from length-sorted volumes: the database's
+// k-th sequence is input sequence order[k] (order NULL: k), volume_seqs
+// counts the volumes' sequences; <prefix>.hDb for one volume, else
+// <prefix>.NN.hDb per volume (CreateH_BLAST_database's names,
+// functions.cpp:1089-1127), and the index <prefix>.hDbI.
+extern "C" int hb_hdb_write(const char *prefix, const uint8_t *residues, const uint64_t *offsets,
+                            int32_t n_seqs, const int32_t *order, const int32_t *volume_seqs,
+                            int32_t n_volumes, int32_t blocksx, int32_t threadsx)
+{
+    if (!prefix || !offsets || !volume_seqs || n_volumes < 1 || n_seqs < 0 ||
+        (n_seqs > 0 && !residues) || !grid_ok(blocksx, threadsx))
+        return HB_ERR_ARG;
+    if (order) {   // a permutation of 0 .. n_seqs - 1
+        std::vector<uint8_t> seen((size_t)n_seqs, 0);
+        for (int32_t k = 0; k < n_seqs; ++k) {
+            if (order[k] < 0 || order[k] >= n_seqs || seen[(size_t)order[k]]) return HB_ERR_ARG;
+            seen[(size_t)order[k]] = 1;
+        }
+    }
+    for (int32_t i = 0; i < n_seqs; ++i)
+        if (offsets[i + 1] < offsets[i]) return HB_ERR_ARG;
+    int64_t total = 0;
+    for (int32_t k = 0; k < n_volumes; ++k) {
+        if (volume_seqs[k] < 0) return HB_ERR_ARG;
+        total += volume_seqs[k];
+    }
+    if (total != n_seqs) return HB_ERR_ARG;
+    const std::string pre(prefix);
+    const size_t slash = pre.find_last_of('/');
+    const std::string stem = slash == std::string::npos ? pre : pre.substr(slash + 1);
+    const std::string dir = slash == std::string::npos ? std::string() : pre.substr(0, slash + 1);
+    std::vector<std::string> names((size_t)n_volumes);
+    std::vector<std::vector<int32_t>> groups((size_t)n_volumes);
+    std::vector<hb_hdb_volume> vols((size_t)n_volumes);
+    std::vector<uint8_t> buf, gathered;
+    std::vector<uint64_t> o;
+    int32_t first = 0;
+    for (int32_t k = 0; k < n_volumes; ++k) {
+        const int32_t vs = volume_seqs[k];
+        o.assign((size_t)vs + 1, 0);
+        const uint8_t *r = nullptr;
+        if (order) {   // the volume's sequences gathered in database order
+            for (int32_t i = 0; i < vs; ++i) {
+                const int32_t src = order[first + i];
+                o[(size_t)i + 1] = o[(size_t)i] + (offsets[src + 1] - offsets[src]);
+            }
+            gathered.resize(o[(size_t)vs]);
+            for (int32_t i = 0; i < vs; ++i) {
+                const int32_t src = order[first + i];
+                if (o[(size_t)i + 1] > o[(size_t)i])
+                    std::memcpy(gathered.data() + o[(size_t)i], residues + offsets[src],
+                                o[(size_t)i + 1] - o[(size_t)i]);
+            }
+            r = gathered.data();
+        } else {
+            for (int32_t i = 0; i <= vs; ++i) o[(size_t)i] = offsets[first + i] - offsets[first];
+            r = residues ? residues + offsets[first] : nullptr;
+        }
+        const int32_t gs = hb_hdb_gpu_seqs(vs);
+        int32_t ng = 0;
+        uint64_t bytes = 0;
+        int rc = hb_hdb_layout(o.data(), gs, blocksx, threadsx, nullptr, 0, &ng, &bytes);
+        if (rc) return rc;
+        groups[(size_t)k].resize((size_t)std::max(ng, 1));
+        rc = hb_hdb_layout(o.data(), gs, blocksx, threadsx, groups[(size_t)k].data(), ng, &ng,
+                           &bytes);
+        if (rc) return rc;
+        buf.assign(bytes, 0);
+        if (bytes) {
+            rc = hb_hdb_pack(r, o.data(), gs, blocksx, threadsx, groups[(size_t)k].data(), ng,
+                             buf.data(), bytes);
+            if (rc) return rc;
+        }
+        char digits[16];
+        std::snprintf(digits, sizeof(digits), ".%02d", k);
+        names[(size_t)k] = stem + (n_volumes > 1 ? std::string(digits) : std::string()) + ".hDb";
+        FILE *f = std::fopen((dir + names[(size_t)k]).c_str(), "wb");
+        if (!f) return HB_ERR_ARG;
+        const size_t put = bytes ? std::fwrite(buf.data(), 1, bytes, f) : 0;
+        if (std::fclose(f) != 0 || put != bytes) return HB_ERR_ARG;
+        hb_hdb_volume &v = vols[(size_t)k];
+        std::memset(&v, 0, sizeof(v));
+        v.name = names[(size_t)k].c_str();
+        v.name_len = (uint32_t)names[(size_t)k].size();
+        v.bytes = bytes;
+        v.volume_seqs = vs;
+        v.gpu_seqs = gs;
+        v.store_limit = first + gs;
+        v.break_limit = first + vs;
+        v.n_groups = ng;
+        v.group_len = groups[(size_t)k].data();
+        v.cpu_rest_bytes = o[(size_t)vs] - o[(size_t)gs];
+        first += vs;
+    }
+    hb_hdb_index idx;
+    std::memset(&idx, 0, sizeof(idx));
+    idx.blocksx = blocksx;
+    idx.threadsx = threadsx;
+    idx.percentage = 99;
+    idx.n_volumes = n_volumes;
+    uint64_t len = 0;
+    int rc = hb_hdbi_format(&idx, vols.data(), nullptr, 0, &len);
+    if (rc) return rc;
+    std::string text(len, '\0');
+    rc = hb_hdbi_format(&idx, vols.data(), &text[0], len, &len);
+    if (rc) return rc;
+    FILE *f = std::fopen((pre + ".hDbI").c_str(), "wb");
+    if (!f) return HB_ERR_ARG;
+    const size_t put = std::fwrite(text.data(), 1, len, f);
+    if (std::fclose(f) != 0 || put != len) return HB_ERR_ARG;
     return HB_OK;
 }

@@ -63,7 +63,7 @@ EXPORTS = [
     "hb_scan_elapsed_ms", "hb_scan_free", "hb_search", "hb_diag_copy",
     "hb_gap_params_default", "hb_scan_gapped", "hb_scan_gapped_info", "hb_gapped_search",
     "hb_fasta_parse", "hb_hdb_gpu_seqs", "hb_hdb_layout", "hb_hdb_pack", "hb_hdbi_format",
-    "hb_hdbi_parse", "hb_db_upload_hdb", "hb_db_hdb_info", "hb_db_fetch",
+    "hb_hdbi_parse", "hb_db_sort_volumes", "hb_hdb_write", "hb_db_upload_hdb", "hb_db_hdb_info", "hb_db_fetch",
     "hb_runtime_options_default", "hb_runtime_options_parse", "hb_hdb_split",
     "hb_balance_next_ratio", "hb_seg_params_default", "hb_seg_mask",
     "hb_batch_opts_default", "hb_batch_create", "hb_batch_add", "hb_batch_run",
@@ -226,6 +226,8 @@ def _lib():
         L.hb_hdb_gpu_seqs.argtypes = [i32]
         L.hb_hdb_layout.argtypes = [vp, i32, i32, i32, vp, i32, C.POINTER(i32), C.POINTER(u64)]
         L.hb_hdb_pack.argtypes = [vp, vp, i32, i32, i32, vp, i32, vp, u64]
+        L.hb_db_sort_volumes.argtypes = [vp, i32, u64, u64, vp, vp, i32, C.POINTER(i32)]
+        L.hb_hdb_write.argtypes = [C.c_char_p, vp, vp, i32, vp, vp, i32, i32, i32]
         L.hb_hdbi_format.argtypes = [vp, vp, vp, u64, C.POINTER(u64)]
         L.hb_hdbi_parse.argtypes = [C.c_char_p, u64, vp, vp, i32, vp, u64, C.POINTER(u64),
                                     C.POINTER(u64)]
@@ -587,36 +589,58 @@ def hdbi_parse(text):
                 volumes=out)
 
 
-def write_hdb(prefix, residues, offsets, volume_seqs=None, blocksx=512, threadsx=192):
-    """Writes an H-BLAST GPU database as CreateH_BLAST_database lays it out:
-    volume files <name>.hDb (one volume) or <name>.NN.hDb, and the index
-    <prefix>.hDbI.  Sequences must ascend in length inside each volume;
-    volume_seqs lists the volumes' sizes (default: one volume).  Returns the
-    index dict (as hdbi_parse gives it)."""
+def sort_volumes(offsets, max_file_bytes=1_000_000_000, max_letters=0):
+    """Volume cut and length order of H-BLAST's makeblastdb -sort_volumes
+    (hb_db_sort_volumes) -> (order, volume_seqs): the database's k-th
+    sequence is input sequence order[k]."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    n = len(offsets) - 1
+    order = np.zeros(max(n, 1), np.int32)
+    nv = C.c_int32()
+    _check(_lib().hb_db_sort_volumes(_ptr(offsets), n, max_file_bytes, max_letters, _ptr(order),
+                                     None, 0, C.byref(nv)), "hb_db_sort_volumes")
+    vs = np.zeros(max(nv.value, 1), np.int32)
+    _check(_lib().hb_db_sort_volumes(_ptr(offsets), n, max_file_bytes, max_letters, _ptr(order),
+                                     _ptr(vs), len(vs), C.byref(nv)), "hb_db_sort_volumes")
+    return order[:n].copy(), vs[:nv.value].copy()
+
+
+def write_hdb(prefix, residues, offsets, volume_seqs=None, blocksx=512, threadsx=192,
+              order=None):
+    """Writes an H-BLAST GPU database as CreateH_BLAST_database lays it out
+    (hb_hdb_write): volume files <name>.hDb (one volume) or <name>.NN.hDb,
+    and the index <prefix>.hDbI.  The database's k-th sequence is input
+    sequence order[k] (default: the input order); sequences must ascend in
+    length inside each volume; volume_seqs lists the volumes' sizes
+    (default: one volume).  Returns the index dict (as hdbi_parse gives
+    it)."""
     residues = np.ascontiguousarray(residues, dtype=np.uint8)
     offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
     n = len(offsets) - 1
-    volume_seqs = [n] if volume_seqs is None else list(volume_seqs)
-    if sum(volume_seqs) != n:
+    vs = np.ascontiguousarray([n] if volume_seqs is None else list(volume_seqs), np.int32)
+    if int(vs.sum()) != n:
         raise ValueError("volume sizes must add up to the number of sequences")
-    d, stem = os.path.split(prefix)
-    vols, first = [], 0
-    for k, vs in enumerate(volume_seqs):
-        o = offsets[first:first + vs + 1] - offsets[first]
-        r = residues[int(offsets[first]):int(offsets[first + vs])]
-        gs = hdb_gpu_seqs(vs)
-        data, gl = hdb_pack(r, o, gs, blocksx, threadsx)
-        name = stem + (f".{k:02d}" if len(volume_seqs) > 1 else "") + ".hDb"
-        with open(os.path.join(d, name), "wb") as f:
-            f.write(data.tobytes())
-        vols.append(dict(name=name, bytes=int(data.size), volume_seqs=vs, gpu_seqs=gs,
-                         store_limit=first + gs, break_limit=first + vs, group_len=gl,
-                         cpu_rest_bytes=int(o[vs] - o[gs])))
-        first += vs
-    text = hdbi_format(blocksx, threadsx, vols)
-    with open(prefix + ".hDbI", "w") as f:
-        f.write(text)
-    return hdbi_parse(text)
+    ordp = None
+    if order is not None:
+        order = np.ascontiguousarray(order, np.int32)
+        if len(order) != n:
+            raise ValueError("order must list every sequence once")
+        ordp = _ptr(order)
+    _check(_lib().hb_hdb_write(os.fsencode(prefix), _ptr(residues), _ptr(offsets), n, ordp,
+                               _ptr(vs), len(vs), blocksx, threadsx), "hb_hdb_write")
+    with open(prefix + ".hDbI", "rb") as f:
+        return hdbi_parse(f.read())
+
+
+def build_hdb(prefix, residues, offsets, max_file_bytes=1_000_000_000, max_letters=0,
+              blocksx=512, threadsx=192):
+    """From packed sequences in input order (FASTA order) to an H-BLAST GPU
+    database, as makeblastdb -sort_volumes followed by
+    CreateH_BLAST_database builds one: volumes cut and length-sorted
+    (sort_volumes), then written (write_hdb).  Returns (index dict, order)."""
+    order, vs = sort_volumes(offsets, max_file_bytes, max_letters)
+    idx = write_hdb(prefix, residues, offsets, vs, blocksx, threadsx, order=order)
+    return idx, order
 
 
 def open_hdb(ctx: Context, index_path):

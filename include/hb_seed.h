@@ -102,8 +102,9 @@ const int8_t *hb_blosum62(void);  /* 28x28 row-major */
  * set_first_gpu_id (ncbi_blast_files/gpu_cpu_common.h:99,111,79). */
 int hb_create(int device, hb_ctx **out);
 int hb_destroy(hb_ctx *ctx);
-/* Free / total device memory in bytes (admission control uses this;
- * replaces H_BLAST_check_memory, gpu_cpu_common.h:93-102). */
+/* Free / total device memory in bytes (the served form's admission and
+ * hb_check_memory's default budget use it; hb_check_memory is the
+ * counterpart of H_BLAST_check_memory, gpu_cpu_common.h:93-102). */
 int hb_mem_info(hb_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 int hb_synchronize(hb_ctx *ctx);
 /* Page-locked host memory (hipHostMalloc) for the arrays a caller hands to
@@ -279,6 +280,33 @@ int hb_hdbi_format(const hb_hdb_index *index, const hb_hdb_volume *vols, char *t
 int hb_hdbi_parse(const char *text, uint64_t len, hb_hdb_index *index, hb_hdb_volume *vols,
                   int32_t vol_cap, int32_t *group_len, uint64_t group_cap,
                   uint64_t *n_group_len, uint64_t *err_off);
+/* Volume cut and length order of H-BLAST's makeblastdb -sort_volumes
+ * (CWriteDB_Volume::WriteSequenceSort, ncbi_blast_files/
+ * writedb_volume.cpp:255-513; makeblastdb.cpp:1054-1076): the n_seqs input
+ * sequences (offsets, input order) fill volumes in order while their
+ * sequence file stays within max_file_bytes (-max_file_sz; the file starts
+ * at offset 1 and each sequence adds its length + 1, a volume's first
+ * sequence always fits, a later one while offset + length < max_file_bytes,
+ * writedb_files.hpp:435-494) and, when max_letters > 0, their letters
+ * within max_letters; each volume is then ordered by ascending length as
+ * the reference writes it (std::sort by descending length, written from the
+ * back).  order[k] = the input index of the database's k-th sequence;
+ * volume_seqs gets *n_volumes counts (NULL, or HB_ERR_CAPACITY past
+ * volume_cap, to size).  The header and index files' limits depend on
+ * deflines this library does not hold and are not applied. */
+int hb_db_sort_volumes(const uint64_t *offsets, int32_t n_seqs, uint64_t max_file_bytes,
+                       uint64_t max_letters, int32_t *order, int32_t *volume_seqs,
+                       int32_t volume_cap, int32_t *n_volumes);
+/* Writes an H-BLAST GPU database from length-sorted volumes, as
+ * CreateH_BLAST_database does from NCBI's (functions.cpp:785-1165): the
+ * database's k-th sequence is input sequence order[k] (a permutation, as
+ * hb_db_sort_volumes gives it; NULL: the input order), volume_seqs[n_volumes]
+ * counts the volumes' sequences (each volume must ascend in length up to its
+ * .hDb share); files <prefix>.hDb (one volume) or <prefix>.NN.hDb and the
+ * index <prefix>.hDbI. */
+int hb_hdb_write(const char *prefix, const uint8_t *residues, const uint64_t *offsets,
+                 int32_t n_seqs, const int32_t *order, const int32_t *volume_seqs,
+                 int32_t n_volumes, int32_t blocksx, int32_t threadsx);
 /* Uploads one .hDb volume (host bytes) and unpacks it on the device into a
  * resident database: subject i is the volume's i-th sequence (database oid
  * = the volume's first oid + i).  Replaces Read_H_BLAST_GPU_DB plus the

@@ -316,9 +316,151 @@ static void seg(std::mt19937_64 &rng, int rounds)
     }
 }
 
+// Volume cut and length order of random sequence sets (permutation, each
+// volume shortest first, cut sizes as the rule gives them), written out
+// with hb_hdb_write and read back through the index.
+static void volumes(std::mt19937_64 &rng, int rounds, const std::string &dir)
+{
+    for (int it = 0; it < rounds; ++it) {
+        const int32_t n = (int32_t)(rng() % 400);
+        std::vector<uint64_t> offs(n + 1, 0);
+        for (int32_t i = 0; i < n; ++i) offs[i + 1] = offs[i] + rng() % 300;
+        const uint64_t maxb = 2 + rng() % 20000, maxl = rng() % 2 ? 0 : 1 + rng() % 10000;
+        std::vector<int32_t> order(std::max(n, 1)), vs(n + 1);
+        int32_t nv = 0;
+        CHECK(hb_db_sort_volumes(offs.data(), n, maxb, maxl, order.data(), vs.data(), n + 1,
+                                 &nv) == HB_OK);
+        std::vector<char> seen(n, 0);
+        int32_t first = 0;
+        for (int32_t v = 0; v < nv; ++v) {
+            CHECK(vs[v] >= 1);
+            for (int32_t k = first; k < first + vs[v]; ++k) {
+                CHECK(order[k] >= first && order[k] < first + vs[v] && !seen[order[k]]);
+                seen[order[k]] = 1;
+                if (k > first)
+                    CHECK(offs[order[k] + 1] - offs[order[k]] >=
+                          offs[order[k - 1] + 1] - offs[order[k - 1]]);
+            }
+            first += vs[v];
+        }
+        CHECK(first == n);
+        if (n == 0 || it % 10) continue;
+        std::vector<uint8_t> res(offs[n]);
+        for (auto &r : res) r = (uint8_t)(1 + rng() % 24);
+        const std::string pre = dir + "/san_vol";
+        CHECK(hb_hdb_write(pre.c_str(), res.data(), offs.data(), n, order.data(), vs.data(), nv,
+                           1 + (int32_t)(rng() % 3), 1 + (int32_t)(rng() % 64)) == HB_OK);
+        FILE *f = std::fopen((pre + ".hDbI").c_str(), "rb");
+        CHECK(f != nullptr);
+        if (!f) continue;
+        std::string t;
+        char b[4096];
+        size_t got;
+        while ((got = std::fread(b, 1, sizeof(b), f)) > 0) t.append(b, got);
+        std::fclose(f);
+        hb_hdb_index ix;
+        uint64_t ngl = 0, eo = 0;
+        CHECK(hb_hdbi_parse(t.data(), t.size(), &ix, nullptr, 0, nullptr, 0, &ngl, &eo) == HB_OK);
+        CHECK(ix.n_volumes == nv);
+        // a bad order is refused
+        if (n > 1) {
+            std::vector<int32_t> bad(order.begin(), order.begin() + n);
+            bad[1] = bad[0];
+            CHECK(hb_hdb_write(pre.c_str(), res.data(), offs.data(), n, bad.data(), vs.data(), nv,
+                               1, 32) == HB_ERR_ARG);
+        }
+    }
+}
+
+// The C++ co-scheduled loop over a written database with fake device steps:
+// random fits, out-of-memory launches, device and CPU times, a failing CPU
+// search; the order invariants hold and every started share is fetched or
+// abandoned.
+struct FakeDev {
+    std::mt19937_64 *rng;
+    int launched = 0, fetched = 0, abandoned = 0, fail_cpu_at = -1;
+};
+static int fake_fits(void *u, int32_t, const hb_hdb_share *sh, int32_t, uint64_t *need,
+                     int32_t *fits)
+{
+    FakeDev *d = static_cast<FakeDev *>(u);
+    *need = sh->bytes * 3 + 64;
+    *fits = (*d->rng)() % 5 != 0;
+    return HB_OK;
+}
+static int fake_launch(void *u, int32_t, const uint8_t *hdb, const hb_hdb_share *sh, int32_t,
+                       int32_t, const int32_t *)
+{
+    FakeDev *d = static_cast<FakeDev *>(u);
+    volatile uint8_t x = sh->bytes ? hdb[sh->bytes - 1] : 0;   // the share was read whole
+    (void)x;
+    if ((*d->rng)() % 7 == 0) return HB_ERR_NOMEM;
+    ++d->launched;
+    return HB_OK;
+}
+static int fake_fetch(void *u, int32_t, int64_t, double *secs, uint64_t *n_hsp)
+{
+    FakeDev *d = static_cast<FakeDev *>(u);
+    ++d->fetched;
+    *secs = 0.001 * (double)((*d->rng)() % 100);
+    *n_hsp = (*d->rng)() % 1000;
+    return HB_OK;
+}
+static void fake_abandon(void *u) { ++static_cast<FakeDev *>(u)->abandoned; }
+static int fake_cpu(void *u, int32_t v, int64_t lo, int64_t hi)
+{
+    FakeDev *d = static_cast<FakeDev *>(u);
+    CHECK(lo <= hi);
+    return v == d->fail_cpu_at ? 1 : 0;
+}
+
+static void cosched(std::mt19937_64 &rng, int rounds, const std::string &dir)
+{
+    for (int it = 0; it < rounds; ++it) {
+        const int32_t n = 200 + (int32_t)(rng() % 3000);
+        std::vector<uint64_t> offs(n + 1, 0);
+        for (int32_t i = 0; i < n; ++i) offs[i + 1] = offs[i] + 1 + rng() % 200;
+        std::vector<uint8_t> res(offs[n]);
+        for (auto &r : res) r = (uint8_t)(1 + rng() % 24);
+        std::vector<int32_t> order(n), vs(n + 1);
+        int32_t nv = 0;
+        CHECK(hb_db_sort_volumes(offs.data(), n, 20000 + rng() % 100000, 0, order.data(), vs.data(),
+                                 n + 1, &nv) == HB_OK);
+        const std::string pre = dir + "/san_cos";
+        CHECK(hb_hdb_write(pre.c_str(), res.data(), offs.data(), n, order.data(), vs.data(), nv, 2,
+                           16) == HB_OK);
+        FakeDev d;
+        d.rng = &rng;
+        d.fail_cpu_at = rng() % 3 == 0 ? (int)(rng() % nv) : -1;
+        hb_cosched_steps st = {&d, fake_fits, fake_launch, fake_fetch, fake_abandon};
+        hb_runtime_options opts;
+        hb_runtime_options_default(&opts);
+        std::vector<hb_volume_report> rep(nv);
+        int32_t got = 0;
+        const int rc = hb_cosched_run((pre + ".hDbI").c_str(), &opts, 1 + (int32_t)(rng() % 16), 0,
+                                      300, 300, &st, fake_cpu, &d, rep.data(), nv, &got);
+        CHECK(got == nv);
+        CHECK(d.fetched + d.abandoned == d.launched);
+        // a ratio of 0 stops the loop as the reference stops the program
+        // (functions.cpp:65-69, HB_ERR_ARG here)
+        bool zero = false;
+        for (int32_t v = 0; v < nv; ++v) zero = zero || (rep[v].gpu_seqs == 0 && rep[v].ratio == 0.f);
+        if (rc == HB_ERR_ARG) {
+            CHECK(zero);
+        } else if (d.fail_cpu_at < 0) {
+            CHECK(rc == HB_OK);
+            for (int32_t v = 0; v + 1 < nv; ++v)
+                if (rep[v + 1].launch_seq) CHECK(rep[v + 1].launch_seq < rep[v].cpu_seq);
+        } else {
+            CHECK(rc == HB_ERR_STATE);
+        }
+    }
+}
+
 int main(int argc, char **argv)
 {
     const bool threads_only = argc > 1 && std::strcmp(argv[1], "threads") == 0;
+    const std::string dir = argc > 1 && !threads_only ? argv[1] : ".";
     std::mt19937_64 rng(20261202);
     if (threads_only) {
         oracle(rng, 30, 4);
@@ -329,6 +471,8 @@ int main(int argc, char **argv)
         hdb(rng, 300);
         balance(rng, 2000);
         seg(rng, 300);
+        volumes(rng, 300, dir);
+        cosched(rng, 40, dir);
     }
     std::fprintf(stderr, "%s: %d failed checks\n", threads_only ? "threads" : "all", fails);
     return fails ? 1 : 0;

@@ -252,3 +252,100 @@ def test_write_hdb_files_read_back(tmp_path):
         rest = int(offs[first + v["volume_seqs"]] - offs[first + v["gpu_seqs"]])
         assert v["cpu_rest_bytes"] == rest
         first += v["volume_seqs"]
+
+
+# --- sorted volumes (makeblastdb -sort_volumes) ------------------------------
+
+def offs_of(lens):
+    o = np.zeros(len(lens) + 1, np.uint64)
+    o[1:] = np.cumsum(np.asarray(lens, np.uint64))
+    return o
+
+
+def test_sort_volumes_by_hand():
+    # sequence file: offset 1, +len+1 per sequence; a later sequence fits
+    # while offset + len < max (writedb_files.hpp:435-494)
+    # max 40: 1 -> 12 (len 10) -> 33 (len 20, 12 + 20 < 40); len 30: 33 + 30
+    # >= 40 -> new volume: 1 -> 32; len 5: 32 + 5 < 40 -> 38
+    order, vs = hb.sort_volumes(offs_of([10, 20, 30, 5]), 40)
+    assert vs.tolist() == [2, 2]
+    assert order.tolist() == [0, 1, 3, 2]        # each volume shortest first
+    # offset + len == max does not fit (strict)
+    order, vs = hb.sort_volumes(offs_of([10, 20]), 32)
+    assert vs.tolist() == [1, 1]
+    order, vs = hb.sort_volumes(offs_of([10, 20]), 33)
+    assert vs.tolist() == [2]
+    # a volume's first sequence fits whatever its length
+    # 100 opens volume 0 (offset 102); 3: 102 + 3 >= 50 opens volume 1
+    # (offset 5); 200: 5 + 200 >= 50 opens volume 2
+    order, vs = hb.sort_volumes(offs_of([100, 3, 200]), 50)
+    assert vs.tolist() == [1, 1, 1]
+    assert order.tolist() == [0, 1, 2]
+    # the letter limit: letters + len > limit cuts
+    order, vs = hb.sort_volumes(offs_of([4, 4, 4]), 10 ** 6, max_letters=8)
+    assert vs.tolist() == [2, 1]
+    # descending input comes out ascending inside the volume
+    order, vs = hb.sort_volumes(offs_of([9, 7, 5, 3]), 10 ** 6)
+    assert vs.tolist() == [4] and order.tolist() == [3, 2, 1, 0]
+
+
+def cut_restated(lens, max_bytes, max_letters=0):
+    """The volume cut written as the file sizes after each sequence: a new
+    volume starts at the first sequence (after a volume's first) whose
+    sequence-file size would reach max_bytes or whose letters would pass
+    max_letters."""
+    out, start = [], 0
+    for i in range(1, len(lens) + 1):
+        if i == len(lens):
+            out.append(i - start)
+            break
+        size = 1 + sum(int(x) + 1 for x in lens[start:i])   # file size with start..i-1
+        letters = sum(int(x) for x in lens[start:i])
+        if size + int(lens[i]) >= max_bytes or (max_letters and letters + int(lens[i]) > max_letters):
+            out.append(i - start)
+            start = i
+    return out
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_sort_volumes_random(seed):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 400, 3000)
+    lens[rng.random(3000) < 0.3] = 77                 # many ties
+    offs = offs_of(lens)
+    for max_bytes, max_letters in [(20_000, 0), (100_000, 30_000), (10 ** 9, 0), (300, 0)]:
+        order, vs = hb.sort_volumes(offs, max_bytes, max_letters)
+        assert vs.tolist() == cut_restated(lens, max_bytes, max_letters)
+        first = 0
+        for c in vs:
+            got = order[first:first + c]
+            assert sorted(got.tolist()) == list(range(first, first + c))   # the volume's own
+            assert np.all(np.diff(lens[got]) >= 0)                          # shortest first
+            first += c
+
+
+def test_build_hdb_from_unsorted_sequences(tmp_path):
+    rng = np.random.default_rng(9)
+    lens = rng.integers(1, 300, 2500).astype(np.uint64)
+    offs = offs_of(lens)
+    res = rng.integers(1, 25, int(offs[-1]), dtype=np.uint8)
+    prefix = str(tmp_path / "built")
+    idx, order = hb.build_hdb(prefix, res, offs, max_file_bytes=150_000, blocksx=2, threadsx=32)
+    vols = idx["volumes"]
+    assert len(vols) > 1
+    assert [v["name"] for v in vols] == [f"built.{k:02d}.hDb" for k in range(len(vols))]
+    first = 0
+    for v in vols:
+        data = np.fromfile(os.path.join(tmp_path, v["name"]), np.uint8)
+        assert data.size == v["bytes"]
+        r2, o2 = O.hdb_unpack(data, v["gpu_seqs"], 2, 32, v["group_len"])
+        ids = order[first:first + v["gpu_seqs"]]
+        want = np.concatenate([res[int(offs[i]):int(offs[i + 1])] for i in ids]) \
+            if len(ids) else np.zeros(0, np.uint8)
+        assert np.array_equal(o2[1:] - o2[:-1], lens[ids])
+        assert np.array_equal(r2, want)
+        rest = order[first + v["gpu_seqs"]:first + v["volume_seqs"]]
+        assert v["cpu_rest_bytes"] == int(lens[rest].sum())
+        assert v["break_limit"] == first + v["volume_seqs"]
+        first += v["volume_seqs"]
+    assert first == 2500

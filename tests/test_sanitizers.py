@@ -1,7 +1,7 @@
 """Host code under sanitizers (CPU only, no GPU).
 
 Builds tests/sanitize/san_driver.cpp with the host-only C-ABI helpers
-(h-blast_amd/csrc/hb_seq.cpp, hb_hdb.cpp, hb_balance.cpp, hb_seg.cpp) and the
+(h-blast_amd/csrc/hb_seq.cpp, hb_hdb.cpp, hb_balance.cpp, hb_seg.cpp, hb_cosched.cpp) and the
 oracle sources twice -- with
 -fsanitize=address,undefined and with -fsanitize=thread -- in a temporary
 directory, runs them on random and malformed inputs, and fails on any report
@@ -20,7 +20,8 @@ SRC = [os.path.join(ROOT, "tests", "sanitize", "san_driver.cpp"),
        os.path.join(ROOT, "h-blast_amd", "csrc", "hb_seq.cpp"),
        os.path.join(ROOT, "h-blast_amd", "csrc", "hb_hdb.cpp"),
        os.path.join(ROOT, "h-blast_amd", "csrc", "hb_balance.cpp"),
-       os.path.join(ROOT, "h-blast_amd", "csrc", "hb_seg.cpp")]
+       os.path.join(ROOT, "h-blast_amd", "csrc", "hb_seg.cpp"),
+       os.path.join(ROOT, "h-blast_amd", "csrc", "hb_cosched.cpp")]
 CSRC = [os.path.join(ROOT, "oracle", "hbo.c"), os.path.join(ROOT, "oracle", "hbo_gap.c"),
         os.path.join(ROOT, "oracle", "hbo_hdb.c"), os.path.join(ROOT, "oracle", "hbo_balance.c"),
         os.path.join(ROOT, "oracle", "hbo_seg.c")]
@@ -47,7 +48,7 @@ def build_and_run(tmp, flags, args, env_extra):
 def test_address_and_undefined_behaviour_sanitizers():
     with tempfile.TemporaryDirectory() as tmp:
         r = build_and_run(tmp, ["-fsanitize=address,undefined", "-fno-sanitize-recover=all",
-                                "-fno-omit-frame-pointer"], [],
+                                "-fno-omit-frame-pointer"], [tmp],
                           {"ASAN_OPTIONS": "detect_leaks=1:abort_on_error=0:"
                                            "verify_asan_link_order=0",
                            "UBSAN_OPTIONS": "print_stacktrace=1"})
