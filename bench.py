@@ -408,6 +408,12 @@ def measure_blastx(ctx, db, res, offs, reps=5):
     n_res, n_subj = int(offs[-1]), len(offs) - 1
     kms = float(np.mean(ms))
     abytes = algorithmic_bytes(n_res, n_subj, len(h), n_entries)
+    traffic = src = None
+    tf = os.path.join(ROOT, "profiles", "hb_scan_kernel_c2_traffic.json")
+    if os.path.exists(tf):   # profile-derived: rocprofv3 PMC runs of these launches
+        with open(tf) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
+        src = "profiles/hb_scan_kernel_c2_traffic.json"
     return {"workload": "plain BLASTX: 1,500-nt query (six frames, two planted coding regions) "
                         "vs the same 2,000,000 subjects",
             "query_residues": int(len(spec.seq)), "contexts": int(spec.n_ctx),
@@ -417,7 +423,8 @@ def measure_blastx(ctx, db, res, offs, reps=5):
             "extensions": int(st["extensions"].sum()),
             "roofline": {"bound": "hbm", "achieved": abytes / (kms * 1e-3) / 1e9,
                          "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": abytes / (kms * 1e-3) / HBM_PEAK, "traffic": None}}
+                         "frac": abytes / (kms * 1e-3) / HBM_PEAK, "traffic": traffic,
+                         "traffic_source": src}}
 
 
 def measure_served_sweep(ctx, db, n_res, n_subj, start_qps=2.0, factor=1.6, points=6,
