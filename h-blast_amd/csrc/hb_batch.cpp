@@ -19,7 +19,7 @@ struct Class {
     hb_launch_cfg cfg;
     uint32_t tiles_per_item = 1;
     int place = HB_PLACE_LDS;          // where the diagonal state goes
-    unsigned long long *d_gstate = nullptr;   // diag_len_max words per wavefront of the grid
+    uint32_t *d_gstate = nullptr;   // diag_len_max words per wavefront of the grid
     size_t gstate_bytes = 0;
 };
 
@@ -211,7 +211,7 @@ static int build(hb_batch *b, int32_t target)
         if (c.place == HB_PLACE_GLOBAL) {
             c.gstate_bytes =
                 (size_t)c.cfg.blocks * c.cfg.waves * (size_t)c.diag_len_max * HB_GSTATE_WORD;
-            c.d_gstate = static_cast<unsigned long long *>(hb_pool_get(b->ctx, c.gstate_bytes));
+            c.d_gstate = static_cast<uint32_t *>(hb_pool_get(b->ctx, c.gstate_bytes));
             if (!c.d_gstate) return HB_ERR_NOMEM;
         }
     }

@@ -13,6 +13,7 @@
 #define HB_JOBCAP 128       // queued two-hit extensions per wave (a ring)
 #define HB_BLKCAP 128       // hits waiting on a pending diagonal per wave
 #define HB_SLOTS 8          // staged subjects per wave whose extensions may still be queued
+#define HB_GMARKS 1024      // ordering-marker slots per wave when the diagonal words are in global memory
 #ifndef HB_RING_MAX
 #define HB_RING_MAX 4096    // largest staging ring per wave (subjects up to the ring are staged)
 #endif
@@ -21,8 +22,8 @@
 // between resets (hb_scan_args::state_limit), so the longest subject and
 // its successor must fit: 2 * (longest + window) + 64 < 2^23.
 #define HB_LDS_POS_LIMIT (1 << 23)
-// ... in global memory: one 64-bit word per diagonal, 29-bit positions
-// (subjects up to 2^28 residues).
+// ... in global memory: one 32-bit word per diagonal (the ordering markers
+// are in LDS), 29-bit positions (subjects up to 2^28 residues).
 #define HB_GLOBAL_POS_LIMIT (1 << 28)
 
 // One query group: a concatenated query (one or more queries / frames as
@@ -76,15 +77,16 @@ struct hb_scan_args {
     uint32_t *out_count;
     uint32_t *stats;             // plain: 3 per oid (lookup hits, extensions, HSPs); else null
     unsigned long long *group_totals;  // batched: 3 per group; else null
-    // the diagonal state in global memory, diag_len_max 64-bit words per
+    // the diagonal state in global memory, diag_len_max 32-bit words per
     // wavefront of the grid (null: in LDS)
-    unsigned long long *gstate;
+    uint32_t *gstate;
     // the state is reset when a subject's base offset + length + window
     // reaches this (the position limit of the placement - the longest subject)
     int32_t state_limit;
 };
 
-// per wave: diagonal words, 64 segment tags, job and blocked queues (8 B
+// per wave: diagonal words (or, with the words in global memory, the
+// ordering-marker table), 64 segment tags, job and blocked queues (8 B
 // entries), the staged subjects' slots (32 B each) and the staging ring.
 // The regions of a workgroup's wavefronts are padded so that their distance
 // modulo 512 bytes is at least 128: distances of 0 or 64 bytes (mod 512)
@@ -94,8 +96,8 @@ struct hb_scan_args {
 // in global memory).
 static inline size_t hb_scan_wave_lds(int32_t diag_len, int32_t ring, int state_bytes = 4)
 {
-    size_t b = (size_t)state_bytes * (size_t)diag_len + 256 +
-               8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + 32 * HB_SLOTS + (size_t)ring;
+    size_t b = (state_bytes ? (size_t)state_bytes * (size_t)diag_len : 4 * (size_t)HB_GMARKS) +
+               256 + 8 * (size_t)(HB_JOBCAP + HB_BLKCAP) + 32 * HB_SLOTS + (size_t)ring;
     b = (b + 15) & ~(size_t)15;
     const size_t r = b % 512;
     if (r < 128) b += 128 - r;

@@ -323,9 +323,10 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 //              extension pending, bit 24 extended-right flag, bits 25..31
 //              the ordering marker of the decision pass under way (0
 //              between passes).  With the state in global memory (long
-//              queries) the word is a uint64 in the wavefront's own region:
-//              29-bit position, bit 29 pending, bit 30 flag, marker in the
-//              high half.
+//              queries) the word is a uint32 in the wavefront's own region
+//              (29-bit position, bit 29 pending, bit 30 flag) and the
+//              markers sit in a table of HB_GMARKS words here instead,
+//              hashed by diagonal.
 //   seg[64]    uint32 tagged segment starts of the current hit round
 //   jobs[kJobCap] uint2 queued two-hit extensions (a ring, oldest first):
 //              (q | (s - last) << 24, s | slot << 28)
@@ -339,6 +340,8 @@ __device__ __forceinline__ int32_t context_of(const int32_t *ctx_off, int32_t n_
 // Per-workgroup LDS: score matrix, context tables (stride ctx_pad_max) and
 // query of the group being scanned.
 constexpr uint32_t kJobCap = HB_JOBCAP;
+constexpr uint32_t kGMarks = HB_GMARKS;
+static_assert((kGMarks & (kGMarks - 1)) == 0, "marker table");
 constexpr uint32_t kBlkCap = HB_BLKCAP;
 static_assert((kJobCap & (kJobCap - 1)) == 0 && kJobCap >= 2 * HB_WAVE, "job ring");
 static_assert(kBlkCap >= 2 * HB_WAVE, "blocked list");
@@ -361,9 +364,8 @@ template <> struct DiagWord<false> {
     static constexpr int kMS = 25;
     static constexpr uint32_t kF = 1u << 24, kP = 1u << 23, kSM = (1u << 25) - 1;
 };
-template <> struct DiagWord<true> {
-    using T = unsigned long long;
-    static constexpr int kMS = 32;
+template <> struct DiagWord<true> {   // no marker bits: the markers are in an LDS table
+    using T = uint32_t;
     static constexpr uint32_t kF = 1u << 30, kP = 1u << 29, kSM = 0xffffffffu;
 };
 static_assert(HB_LDS_POS_LIMIT <= (1 << 23) && HB_GLOBAL_POS_LIMIT <= (1 << 28),
@@ -393,6 +395,7 @@ struct WaveScan {
     const int32_t *ctx;   // ctx_off[ctx_pad], xdrop[ctx_pad], cutoff[ctx_pad]
     int32_t ctx_pad;
     word_t *wd;        // diagonal words (LDS, or the wavefront's global region)
+    uint32_t *mkt;     // global-memory words: the wavefront's LDS marker table
     uint32_t *seg;
     uint2 *jobs;
     uint2 *blk;
@@ -499,20 +502,39 @@ struct WaveScan {
     {
         // d is a valid index in every lane, active or not
         const uint32_t mk = (uint32_t)(HB_WAVE - lane);
-        word_t old = 0;
-        if (active) old = w_max(d, (word_t)mk << W::kMS);
-        sync();
-        const word_t wv = w_ld(d);
-        const bool win = active && (uint32_t)(wv >> W::kMS) == mk;
-        // the first hit to reach the word found no marker and holds the state
-        // (the max cleared the state bits for the hits after it); a winner
-        // that came later gets it from that hit through the word
-        const bool first = active && (uint32_t)(old >> W::kMS) == 0;
-        uint32_t st = (uint32_t)old & W::kSM;
-        if (__ballot(win && !first)) {
-            if (first && !win) w_or(d, old);
+        bool win;
+        uint32_t st, m = 0;
+        if constexpr (G) {
+            // markers in the wavefront's LDS table, hashed by diagonal (hits of
+            // different diagonals that share a slot are ordered like one
+            // diagonal's, which only costs a carry); every hit loads its own
+            // state alongside
+            const uint32_t slot = d & (kGMarks - 1);
+            word_t w0 = 0;
+            if (active) {
+                w0 = w_ld(d);
+                atomicMax(&mkt[slot], mk);
+            }
+            wave_lds_sync();
+            if (active) m = mkt[slot];
+            win = active && m == mk;
+            st = (uint32_t)w0 & W::kSM;
+        } else {
+            word_t old = 0;
+            if (active) old = w_max(d, (word_t)mk << W::kMS);
             sync();
-            if (win && !first) st = (uint32_t)w_ld(d) & W::kSM;
+            const word_t wv = w_ld(d);
+            win = active && (uint32_t)(wv >> W::kMS) == mk;
+            // the first hit to reach the word found no marker and holds the
+            // state (the max cleared the state bits for the hits after it); a
+            // winner that came later gets it from that hit through the word
+            const bool first = active && (uint32_t)(old >> W::kMS) == 0;
+            st = (uint32_t)old & W::kSM;
+            if (__ballot(win && !first)) {
+                if (first && !win) w_or(d, old);
+                sync();
+                if (win && !first) st = (uint32_t)w_ld(d) & W::kSM;
+            }
         }
         const int32_t sb = so + sbase;
         // s - last on an idle diagonal; negative (below -kP + sb) when the
@@ -524,11 +546,26 @@ struct WaveScan {
         // last hit is a window or more behind, or on a flagged one it reached
         const bool set = diff >= window || (uint32_t)(diff + (int32_t)W::kF) < W::kP;
         const uint32_t nst = job ? W::kP : (set ? (uint32_t)sb : st);
-        if (win) w_st(d, (word_t)nst);
-        sync();
         // losers against the winner's new state: wait if it is pending, done
         // if they would skip, else carried
-        const uint32_t st2 = (uint32_t)w_ld(d) & W::kSM;
+        uint32_t st2;
+        if constexpr (G) {
+            if (win) {
+                w_st(d, (word_t)nst);
+                mkt[d & (kGMarks - 1)] = 0;
+            }
+            // the slot's winner holds the new state when it is on this
+            // hit's diagonal; a diagonal without a winner this pass kept its
+            // state
+            const int wl4 = (int)(((HB_WAVE - m) & 63u) << 2);
+            const uint32_t wdg = (uint32_t)__builtin_amdgcn_ds_bpermute(wl4, (int)d);
+            const uint32_t wns = (uint32_t)__builtin_amdgcn_ds_bpermute(wl4, (int)nst);
+            st2 = wdg == d ? wns : st;
+        } else {
+            if (win) w_st(d, (word_t)nst);
+            sync();
+            st2 = (uint32_t)w_ld(d) & W::kSM;
+        }
         const int32_t diff2 = (int32_t)((uint32_t)sb - st2);
         const bool lose = active && !win;
         const bool lwait = lose && st2 == W::kP;
@@ -1090,9 +1127,12 @@ __device__ __forceinline__ void scan_body(const hb_scan_args &a)
     w.ctx_pad = a.ctx_pad_max;
     if constexpr (GSTATE) {
         w.wd = a.gstate + ((size_t)blockIdx.x * (blockDim.x / HB_WAVE) + (size_t)wid) * (size_t)Dm;
-        w.seg = reinterpret_cast<uint32_t *>(wb);
+        w.mkt = reinterpret_cast<uint32_t *>(wb);
+        for (int i = lane; i < (int)kGMarks; i += HB_WAVE) w.mkt[i] = 0;
+        w.seg = w.mkt + kGMarks;
     } else {
         w.wd = reinterpret_cast<uint32_t *>(wb);
+        w.mkt = nullptr;
         w.seg = reinterpret_cast<uint32_t *>(wb + 4 * (size_t)Dm);   // Dm: multiple of 64
     }
     w.jobs = reinterpret_cast<uint2 *>(w.seg + HB_WAVE);

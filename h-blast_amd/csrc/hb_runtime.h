@@ -139,7 +139,7 @@ bool hb_query_lds_ok(const hb_ctx *c, int32_t qlen, int32_t n_ctx);
 // launch for a very long query then runs fewer workgroups.
 constexpr size_t HB_GSTATE_MAX_BYTES = size_t(1) << 30;
 // Bytes per diagonal of the global-memory state.
-constexpr size_t HB_GSTATE_WORD = 8;
+constexpr size_t HB_GSTATE_WORD = 4;
 // Bytes of that state for diagonal arrays of D entries (upper bound).
 size_t hb_gstate_bytes_max(const hb_ctx *c, int32_t D);
 // place: where the diagonal state goes (LDS bytes per diagonal follow).

@@ -352,11 +352,12 @@ int hb_scan_config(hb_ctx *c, bool multi, int32_t qlen_pad_max, int32_t ctx_pad_
 }
 
 // Automatic placement of the diagonal state.  LDS words beat the
-// global-memory state wherever they fit, down to one wavefront per CU: in
-// global memory every decision pass makes its marker atomic, reads and store
-// through L2 (measured on 200,000 subjects, ms LDS / global: 1,000 residues
-// 1.6 / 7.6, 12,000 65 / 153, 31,000 at one wavefront per CU 320 / 424).
-static constexpr int kGstateMinWaves = 1;
+// global-memory words (ordering markers in an LDS table, one load and one
+// store per hit) down to about kGstateMinWaves wavefronts per CU (measured
+// on 200,000 subjects, ms LDS / global: 1,000 residues 1.61 / 1.84, 8,000 at
+// 4 wavefronts per CU 23.7 / 27.6, 12,000 at 3 64.7 / 39.0, 31,000 at 1
+// 320 / 114).
+static constexpr int kGstateMinWaves = 4;
 static int lds_waves(const hb_ctx *c, int32_t D, size_t fixed, int state_bytes)
 {
     const size_t per_wave = hb_scan_wave_lds(D, 256, state_bytes);
@@ -409,7 +410,7 @@ struct hb_scan {
     uint32_t n_out = 0;
     float gap_ms = 0.f;        // device time of the last gapped stage
     uint32_t gap_wide = 0;     // subjects it ran in the wide pass
-    unsigned long long *d_gstate = nullptr;   // per-wavefront diagonal words (long queries)
+    uint32_t *d_gstate = nullptr;   // per-wavefront diagonal words (long queries)
     size_t gstate_bytes = 0;
     int place = HB_PLACE_LDS;      // where this scan keeps the diagonal state
 };
@@ -1051,7 +1052,7 @@ static int launch_scan(hb_scan *s)
     if (place == HB_PLACE_GLOBAL && !s->d_gstate) {
         s->gstate_bytes =
             (size_t)cfg.blocks * cfg.waves * (size_t)q->gs.diag_len_max * HB_GSTATE_WORD;
-        s->d_gstate = (unsigned long long *)hb_pool_get(c, s->gstate_bytes);
+        s->d_gstate = static_cast<uint32_t *>(hb_pool_get(c, s->gstate_bytes));
         if (!s->d_gstate) return HB_ERR_NOMEM;
     }
     hb_scan_args a;
@@ -1249,7 +1250,7 @@ int hb_scan_noalloc(hb_ctx *c, hb_db *db, hb_query *q, hb_hsp *d_hsps, uint32_t 
         const size_t per_block = HB_GSTATE_WORD * (size_t)q->gs.diag_len_max * (size_t)cfg.waves;
         if (per_block > HB_NOALLOC_STATE_BYTES) return HB_ERR_UNSUPPORTED;
         blocks = std::min(blocks, (int)(HB_NOALLOC_STATE_BYTES / per_block));
-        a.gstate = reinterpret_cast<unsigned long long *>(c->scratch_dev + HB_NOALLOC_HEAD);
+        a.gstate = reinterpret_cast<uint32_t *>(c->scratch_dev + HB_NOALLOC_HEAD);
     }
     uint32_t *ctr = reinterpret_cast<uint32_t *>(c->scratch_dev);
     a.out_count = ctr;
