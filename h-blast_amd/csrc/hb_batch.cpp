@@ -374,19 +374,19 @@ static int autotune(hb_batch *b, int32_t *best)
     uint32_t span = b->o.tune_tiles ? b->o.tune_tiles : std::max<uint32_t>(nt / 25, 64);
     span = std::min(span, nt);
     const uint32_t t0 = (nt - span) / 2;   // subjects of median length
-    int32_t longest = 0;
-    for (const hb_concat &q : b->queries) longest = std::max(longest, (int32_t)q.seq.size());
+    uint64_t total = 0;
+    for (const hb_concat &q : b->queries) total += q.seq.size() + (uint64_t)b->p.window;
     float best_ms = 0, ms = 0;
     *best = 0;
     b->tune_candidates = 0;
-    for (int32_t cand : {512, 768, 1024, 1536, 2048, 3072, 4096}) {
+    for (int32_t cand : {512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192}) {
         int rc = time_candidate(b, cand, t0, span, &ms);
         if (rc) return rc;
         if (ms >= 0 && (*best == 0 || ms < best_ms)) {
             best_ms = ms;
             *best = cand;
         }
-        if (cand >= longest + b->p.window && cand > 2048) break;   // larger groups change nothing
+        if ((uint64_t)cand >= total) break;   // one group already holds every query
     }
     if (!*best) return HB_ERR_UNSUPPORTED;
     // launch geometry at the chosen group size, one knob at a time
