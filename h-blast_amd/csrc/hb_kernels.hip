@@ -1310,9 +1310,22 @@ __global__ __launch_bounds__(kFinThreads) void fin_sum2_kernel(const uint32_t *s
         lh += __shfl_xor(lh, o, HB_WAVE);
         ex += __shfl_xor(ex, o, HB_WAVE);
     }
+    // the block's wavefront sums, then one atomic pair per block
+    __shared__ unsigned long long s64[2 * (kFinThreads / HB_WAVE)];
+    const int wid = threadIdx.x / HB_WAVE;
     if ((threadIdx.x & (HB_WAVE - 1)) == 0) {
-        atomicAdd(&tot[0], lh);
-        atomicAdd(&tot[1], ex);
+        s64[2 * wid] = lh;
+        s64[2 * wid + 1] = ex;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int w = 0; w < kFinThreads / HB_WAVE; ++w) {
+            a += s64[2 * w];
+            b += s64[2 * w + 1];
+        }
+        atomicAdd(&tot[0], a);
+        atomicAdd(&tot[1], b);
     }
 }
 
