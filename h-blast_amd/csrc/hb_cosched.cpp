@@ -110,7 +110,12 @@ extern "C" int hb_cosched_run(const char *index_path, const hb_runtime_options *
             const int32_t w = v + 1;
             const hb_hdb_volume &vol = vols[(size_t)w];
             ring_bytes[w % 3] = vol.bytes + vol.cpu_rest_bytes;
-            if (v > 0 && fixed != 1) {
+            // once a volume's ratio is 0 (the device got no share) the updates
+            // stop and the CPU takes every later volume: the reference's update
+            // divides by that ratio and stops the program there
+            // (functions.cpp:65-69)
+            const bool no_share = ring_ratio[0] == 0.f || ring_ratio[1] == 0.f;
+            if (v > 0 && fixed != 1 && !no_share) {
                 hb_balance_in in;
                 std::memset(&in, 0, sizeof(in));
                 in.volume = (uint32_t)v;

@@ -169,3 +169,20 @@ def test_a_failing_callback_stops_the_loop(volumes):
     with pytest.raises(KeyError):
         hbseed.cosched_run(volumes["index"], dev, cpu_search, 4)
     assert ("abandon", -1) in dev.events   # the share started for volume 2 was dropped
+
+
+def test_a_zero_ratio_sends_the_rest_to_the_cpu(volumes):
+    """Once a volume's ratio is 0 the reference's next update divides by it
+    and stops the program (functions.cpp:65-69); the loop keeps the ratio at
+    0 and the CPU searches every later volume whole, so the union still
+    equals the oracle.  An adaptive start ratio of 0 gives that from the
+    first volume; a device far slower than the CPU drives the ratio down
+    towards it."""
+    opts = hbseed.runtime_options()
+    opts.update(gpu_ratio=0.0, fixed_ratio=0)
+    out, events = run(volumes, [5.0] * len(SIZES), [0.0] * len(SIZES), options=opts)
+    assert all(r["ratio"] == 0.0 and r["gpu_seqs"] == 0 for r in out)
+    assert not [e for e in events if e[0] == "launch"]
+    out, _ = run(volumes, [500000.0] * len(SIZES), [0.0] * len(SIZES))
+    ratios = [r["ratio"] for r in out]
+    assert all(b < a for a, b in zip(ratios[1:], ratios[2:])) and ratios[-1] < 0.1, ratios
