@@ -111,7 +111,7 @@ struct hb_launch_cfg {
     int32_t ring = 1024;   // staging ring bytes per wave
 };
 // Where a launch keeps its per-diagonal two-hit state: 32-bit words in LDS,
-// or 64-bit words in a global-memory region per wavefront.
+// or 32-bit words in a global-memory region per wavefront (markers in LDS).
 enum hb_state_place { HB_PLACE_LDS = 0, HB_PLACE_GLOBAL = 2 };
 static inline int hb_state_bytes(int place) { return place == HB_PLACE_LDS ? 4 : 0; }
 // LDS words hold 23-bit positions between resets, so a subject and its

@@ -167,7 +167,8 @@ int hb_query_free(hb_query *q);
  *                     memory (the default);
  *   HB_STATE_LDS    - LDS (HB_ERR_UNSUPPORTED if the query's words do not
  *                     fit, or, at the scan, if the subjects are too long);
- *   HB_STATE_GLOBAL - a global-memory array per wavefront (64-bit words).
+ *   HB_STATE_GLOBAL - a global-memory array per wavefront (32-bit words; the
+ *                     ordering markers in an LDS table).
  * Results are identical; only speed differs.  Takes effect for later
  * hb_scan_start calls. */
 #define HB_STATE_AUTO 0
