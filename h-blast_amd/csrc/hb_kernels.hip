@@ -835,8 +835,9 @@ struct WaveScan {
         return c2;
     }
 
-    // The window at s0 from its cells (loaded one window ahead, so that the
-    // load's latency passes while the previous window's hits are decided).
+    // The window at s0 from its cells.  (Loading them a window ahead made
+    // the compiler wait on that load where it reused its registers, which
+    // also drained the round-ahead entry loads issued after it.)
     __device__ __forceinline__ void make_window(uint64_t c2, int32_t s0, int32_t s0m, Win &w) const
     {
         const uint32_t cst = (uint32_t)c2, cnt = (uint32_t)(c2 >> 32);
@@ -872,7 +873,6 @@ struct WaveScan {
         B = A;
         bool hasB = false;        // B holds the window after A
         int32_t nxt = 0, nxtm = 0;   // the next window to load, and it mod D
-        uint64_t pf = cell_at(S, slen, 0);   // its cells (in flight)
         uint32_t r = 0;           // hits of A already taken
         uint32_t c = 0;           // carried hits, in lanes 0..c-1
         uint32_t cq = 0, cd = 0;
@@ -891,8 +891,7 @@ struct WaveScan {
                     A = B;
                     hasB = false;
                 } else {
-                    make_window(pf, nxt, nxtm, A);
-                    pf = cell_at(S, slen, nxt + HB_WAVE);
+                    make_window(cell_at(S, slen, nxt), nxt, nxtm, A);
                     hits += A.H;
                     nxt += HB_WAVE;
                     nxtm += HB_WAVE;
@@ -904,8 +903,7 @@ struct WaveScan {
                 carry = 0;
             }
             if (!hasB && A.H - r < (uint32_t)HB_WAVE && nxt + 3 <= slen) {
-                make_window(pf, nxt, nxtm, B);
-                pf = cell_at(S, slen, nxt + HB_WAVE);
+                make_window(cell_at(S, slen, nxt), nxt, nxtm, B);
                 hits += B.H;
                 hasB = true;
                 nxt += HB_WAVE;
