@@ -413,7 +413,9 @@ struct WaveScan {
     uint32_t s_tail;   // slot of the oldest staged subject
     uint32_t ring_head;  // ring offset after the newest staged subject
     bool rwrap;        // the staged residues wrap: newer subjects start at 0
-    uint64_t tot_hits, tot_ext, tot_hsp;  // wave-uniform running totals
+    // running totals, per lane (a lane adds the counters of the subjects it
+    // retires), so that they take no scalar registers in the scan loops
+    uint64_t tot_hits, tot_ext, tot_hsp;
 
     __device__ __forceinline__ uint32_t mod_d(uint32_t x) const
     {
@@ -711,9 +713,9 @@ struct WaveScan {
                 a->stats[3 * (size_t)A.x + 2] = hs;
             }
         }
-        tot_hits += wave_sum(h);
-        tot_ext += wave_sum(e);
-        tot_hsp += wave_sum(hs);
+        tot_hits += h;
+        tot_ext += e;
+        tot_hsp += hs;
         n_live = 0;
         s_tail = 0;
         ring_head = 0;
@@ -741,14 +743,16 @@ struct WaveScan {
         const uint4 A = sla[sl], B = slb[sl];
         // wave-uniform, so that the ring state stays uniform
         const uint32_t t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)A.w);
-        if (lane == 0 && a->stats) {
-            a->stats[3 * (size_t)A.x + 0] = B.x;
-            a->stats[3 * (size_t)A.x + 1] = B.y;
-            a->stats[3 * (size_t)A.x + 2] = B.z;
+        if (lane == 0) {
+            if (a->stats) {
+                a->stats[3 * (size_t)A.x + 0] = B.x;
+                a->stats[3 * (size_t)A.x + 1] = B.y;
+                a->stats[3 * (size_t)A.x + 2] = B.z;
+            }
+            tot_hits += B.x;
+            tot_ext += B.y;
+            tot_hsp += B.z;
         }
-        tot_hits += (uint32_t)__builtin_amdgcn_readfirstlane((int)B.x);
-        tot_ext += (uint32_t)__builtin_amdgcn_readfirstlane((int)B.y);
-        tot_hsp += (uint32_t)__builtin_amdgcn_readfirstlane((int)B.z);
         s_tail = (sl + 1) & (HB_SLOTS - 1);
         n_live -= 1;
         if (n_live == 0) {   // an empty ring starts over at slot 0, offset 0
@@ -1040,7 +1044,7 @@ struct WaveScan {
 
     __device__ __forceinline__ void flush_totals(uint32_t g)
     {
-        if (a->group_totals && lane == 0 && (tot_hits | tot_ext | tot_hsp)) {
+        if (a->group_totals && (tot_hits | tot_ext | tot_hsp)) {
             atomicAdd(&a->group_totals[3 * (size_t)g + 0], (unsigned long long)tot_hits);
             atomicAdd(&a->group_totals[3 * (size_t)g + 1], (unsigned long long)tot_ext);
             atomicAdd(&a->group_totals[3 * (size_t)g + 2], (unsigned long long)tot_hsp);
