@@ -1087,15 +1087,15 @@ __device__ __forceinline__ void bind_group(WaveScan<GS> &w, const hb_group_desc 
 // per wavefront of the grid) instead of LDS, for queries (groups) whose
 // state does not fit LDS or would leave it too few wavefronts, and for
 // databases whose subjects are too long for the LDS word's positions.
-// The LDS-state kernels (hb_scan_kernel) keep a register budget for five
-// wavefronts per SIMD (no spills at this limit; short queries or small
-// groups leave LDS for that many); the global-state ones (hb_scan_kernel_g)
-// four, which they need to run without spills.
+// Every scan kernel keeps a register budget for four wavefronts per SIMD
+// (128 VGPRs), the most a 1000-residue query's LDS leaves room for: at five
+// (96 VGPRs) the many-group kernel spills to scratch, and 20 wavefronts per
+// CU measured slower than 16 where the LDS would allow them.
 template <bool MULTI, bool GSTATE>
 __device__ __forceinline__ void scan_body(const hb_scan_args &a);
 
 template <bool MULTI, bool GSTATE>
-__global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void hb_scan_kernel(hb_scan_args a)
+__global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void hb_scan_kernel(hb_scan_args a)
 {
     scan_body<MULTI, GSTATE>(a);
 }
