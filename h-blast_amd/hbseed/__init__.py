@@ -915,6 +915,22 @@ class Query:
             self.h = None
 
 
+class ResultBuffers:
+    """Host arrays reused by Scan.results / results_compact: each grows
+    (to 1.25x the request) when a result does not fit, and is otherwise
+    handed back as a view of its first n elements."""
+
+    def __init__(self):
+        self._a = {}
+
+    def get(self, name, dtype, n):
+        a = self._a.get(name)
+        if a is None or a.dtype != dtype or len(a) < n:
+            a = np.zeros(max(int(n * 1.25), 1), dtype)
+            self._a[name] = a
+        return a[:n]
+
+
 class Scan:
     def __init__(self, ctx: Context, db: Database, q: Query):
         self.db = db
@@ -935,28 +951,40 @@ class Scan:
         _check(_lib().hb_scan_elapsed_ms(self.h, C.byref(ms)), "hb_scan_elapsed_ms")
         return ms.value
 
-    def results(self):
-        """-> (stats[n_subjects], hsps[n]) numpy structured arrays."""
+    def results(self, out=None):
+        """-> (stats[n_subjects], hsps[n]) numpy structured arrays.  out: an
+        optional ResultBuffers whose arrays are reused (grown when too small),
+        the way the reference's caller keeps its result buffers across
+        volumes; the returned arrays are then views of them."""
         n = C.c_uint64()
         _check(_lib().hb_scan_results(self.h, None, None, 0, C.byref(n)), "hb_scan_results")
-        stats = np.zeros(self.db.n, STATS_DTYPE)
-        hsps = np.zeros(n.value, HSP_DTYPE)
+        if out is None:
+            stats = np.zeros(self.db.n, STATS_DTYPE)
+            hsps = np.zeros(n.value, HSP_DTYPE)
+        else:
+            stats = out.get("stats", STATS_DTYPE, self.db.n)
+            hsps = out.get("hsps", HSP_DTYPE, n.value)
         _check(_lib().hb_scan_results(self.h, _ptr(stats), _ptr(hsps), n.value, C.byref(n)),
                "hb_scan_results")
         return stats, hsps
 
-    def results_compact(self):
+    def results_compact(self, out=None):
         """-> (subjects[n_subjects with HSPs], hsps[n], totals dict): the
-        compact results (hb_scan_results_compact)."""
+        compact results (hb_scan_results_compact).  out: as for results."""
         n = C.c_uint64()
         _check(_lib().hb_scan_results(self.h, None, None, 0, C.byref(n)), "hb_scan_results")
-        subj = np.zeros(max(n.value, 1), SUBJ_HITS_DTYPE)
-        hsps = np.zeros(n.value, HSP_DTYPE)
+        cap = max(min(n.value, self.db.n), 1)   # subjects with HSPs <= both
+        if out is None:
+            subj = np.zeros(cap, SUBJ_HITS_DTYPE)
+            hsps = np.zeros(n.value, HSP_DTYPE)
+        else:
+            subj = out.get("subj", SUBJ_HITS_DTYPE, cap)
+            hsps = out.get("hsps", HSP_DTYPE, n.value)
         tot = (C.c_uint64 * 4)()
         _check(_lib().hb_scan_results_compact(self.h, _ptr(subj), len(subj), _ptr(hsps),
                                               n.value, tot), "hb_scan_results_compact")
         totals = dict(zip(("lookup_hits", "extensions", "n_hsp", "n_subjects"), list(tot)))
-        return subj[:totals["n_subjects"]].copy(), hsps, totals
+        return subj[:totals["n_subjects"]], hsps, totals
 
     def gapped(self, cutoffs, gap_open=DEFAULT_GAP_OPEN, gap_extend=DEFAULT_GAP_EXTEND,
                x_dropoff=DEFAULT_GAP_XDROP):

@@ -84,3 +84,20 @@ def test_synthetic_generator_properties():
     assert freq[[0, 2, 21, 23, 24, 25, 26, 27]].sum() == 0
     r2, o2 = hbseed.make_db(5, 2000)
     assert np.array_equal(res, r2)
+
+
+def test_result_buffers_grow_and_hand_back_views():
+    """ResultBuffers: an array is allocated on the first request, handed back
+    as a view of its first n elements while it is large enough, and replaced
+    by a larger one (1.25x the request) when it is not."""
+    b = hbseed.ResultBuffers()
+    a = b.get("hsps", hbseed.HSP_DTYPE, 100)
+    assert len(a) == 100 and a.base is not None and len(a.base) == 125
+    a[:] = 7
+    a2 = b.get("hsps", hbseed.HSP_DTYPE, 40)
+    assert len(a2) == 40 and a2.base is a.base and (a2.view(np.int32) == 7).all()
+    a3 = b.get("hsps", hbseed.HSP_DTYPE, 200)
+    assert len(a3) == 200 and len(a3.base) == 250 and a3.base is not a.base
+    assert len(b.get("hsps", hbseed.HSP_DTYPE, 0)) == 0
+    s = b.get("stats", hbseed.STATS_DTYPE, 10)
+    assert s.dtype == hbseed.STATS_DTYPE and len(s) == 10
