@@ -75,9 +75,10 @@ __global__ void __launch_bounds__(kThreads) hdb_lengths_kernel(hb_hdb_args a)
 // aligned address and its last (up to 3) bytes after the last whole dword
 // go out as single bytes, so no store touches a neighbouring sequence.
 // Every copied residue is checked against the alphabet on the way.
-// (Loading a tile's rows into registers all at once and issuing the next
-// tile's loads before the stores was measured slower: 0.614 vs 0.578 ms on
-// the 2M-sequence volume, at 100 instead of 34 VGPRs.)
+// A wavefront issues all of its rows of a tile before writing them to LDS.
+// (Also issuing the next tile's loads before the stores was measured
+// slower: 0.614 vs 0.578 ms on the 2M-sequence volume, at 100 instead of 34
+// VGPRs.)
 __global__ void __launch_bounds__(kThreads) hdb_copy_kernel(hb_hdb_args a, uint32_t per_block,
                                                             uint32_t n_items)
 {
@@ -106,9 +107,24 @@ __global__ void __launch_bounds__(kThreads) hdb_copy_kernel(hb_hdb_args a, uint3
         __syncthreads();
         const uint32_t rows = s_rows;   // <= L / 4: every load below is inside the block
         for (uint32_t r0 = 0; r0 < rows; r0 += kRows) {
-            if (lane < nt)
-                for (uint32_t r = wave; r <= kRows && r0 + r < rows; r += kThreads / 64)
-                    tile[r * kPitch + lane] = w[(uint64_t)(r0 + r) * a.threadsx + lane];
+            if (lane < nt) {
+                // all of the wave's rows of the tile in flight at once, then
+                // into LDS (one load at a time left a single 256-byte load
+                // per wavefront waiting on HBM latency)
+                constexpr uint32_t kPer = (kRows + 1 + kThreads / 64 - 1) / (kThreads / 64);
+                uint32_t v[kPer];
+#pragma unroll
+                for (uint32_t i = 0; i < kPer; ++i) {
+                    const uint32_t r = wave + i * (kThreads / 64);
+                    v[i] = r <= kRows && r0 + r < rows ? w[(uint64_t)(r0 + r) * a.threadsx + lane]
+                                                       : 0u;
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < kPer; ++i) {
+                    const uint32_t r = wave + i * (kThreads / 64);
+                    if (r <= kRows && r0 + r < rows) tile[r * kPitch + lane] = v[i];
+                }
+            }
             __syncthreads();
             for (uint32_t j = wave; j < nt; j += kThreads / 64) {
                 const uint32_t len = s_len[j];
