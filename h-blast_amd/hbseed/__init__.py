@@ -987,20 +987,27 @@ class Scan:
         return subj[:totals["n_subjects"]], hsps, totals
 
     def gapped(self, cutoffs, gap_open=DEFAULT_GAP_OPEN, gap_extend=DEFAULT_GAP_EXTEND,
-               x_dropoff=DEFAULT_GAP_XDROP):
+               x_dropoff=DEFAULT_GAP_XDROP, out=None):
         """Preliminary gapped extension of this scan's HSPs (hb_scan_gapped)
-        -> (stats[n_subjects], gapped hsps[n]) numpy structured arrays."""
+        -> (stats[n_subjects], gapped hsps[n]) numpy structured arrays.  out:
+        an optional ResultBuffers (as for results); the returned arrays are
+        then views of its arrays."""
         gp = (C.c_int32 * 4)(gap_open, gap_extend, x_dropoff, 0)
         cut = np.ascontiguousarray(cutoffs, np.int32)
         n_ungapped = C.c_uint64()
         _check(_lib().hb_scan_results(self.h, None, None, 0, C.byref(n_ungapped)),
                "hb_scan_results")
-        stats = np.zeros(self.db.n, GSTATS_DTYPE)
-        out = np.zeros(max(n_ungapped.value, 1), GHSP_DTYPE)
+        cap = max(n_ungapped.value, 1)
+        if out is None:
+            stats = np.zeros(self.db.n, GSTATS_DTYPE)
+            ghsp = np.zeros(cap, GHSP_DTYPE)
+        else:
+            stats = out.get("gstats", GSTATS_DTYPE, self.db.n)
+            ghsp = out.get("ghsps", GHSP_DTYPE, cap)
         n = C.c_uint64()
-        _check(_lib().hb_scan_gapped(self.h, C.byref(gp), _ptr(cut), _ptr(stats), _ptr(out),
-                                     len(out), C.byref(n)), "hb_scan_gapped")
-        return stats, out[:n.value].copy()
+        _check(_lib().hb_scan_gapped(self.h, C.byref(gp), _ptr(cut), _ptr(stats), _ptr(ghsp),
+                                     len(ghsp), C.byref(n)), "hb_scan_gapped")
+        return stats, (ghsp[:n.value].copy() if out is None else ghsp[:n.value])
 
     def gapped_info(self):
         """-> (device ms of the last gapped stage, subjects run in the wide pass)"""

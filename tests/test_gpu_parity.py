@@ -329,14 +329,14 @@ def test_compact_results_equal_dense(ctx):
 
 
 def test_results_into_reused_buffers(ctx):
-    """Scan.results / results_compact into one ResultBuffers over scans of
-    different result sizes (a larger one first, then a smaller one, then a
-    larger one again) return the same stats, entries and records as fresh
-    arrays."""
+    """Scan.results / results_compact / gapped into one ResultBuffers each
+    over scans of different result sizes (a larger one first, then a smaller
+    one, then a larger one again) return the same stats, entries and records
+    as fresh arrays."""
     res, offs = hbseed.make_db(7600, 20000, plant_query=hbseed.random_protein(7601, 600),
                                plant_frac=0.05)
     db = hbseed.Database(ctx, res, offs)
-    bufs, cbufs = hbseed.ResultBuffers(), hbseed.ResultBuffers()
+    bufs, cbufs, gbufs = hbseed.ResultBuffers(), hbseed.ResultBuffers(), hbseed.ResultBuffers()
     try:
         for seed, qlen in ((7601, 600), (7602, 150), (7603, 700)):
             q = hbseed.Query(ctx, hbseed.blastp_query(hbseed.random_protein(seed, qlen)))
@@ -346,11 +346,14 @@ def test_results_into_reused_buffers(ctx):
                 stats_r, hsps_r = s.results(bufs)
                 subj, hsps_c, tot = s.results_compact()
                 subj_r, hsps_cr, tot_r = s.results_compact(cbufs)
+                gst, gh = s.gapped([60])
+                gst_r, gh_r = s.gapped([60], out=gbufs)
                 s.close()
             finally:
                 q.close()
             assert np.array_equal(stats_r, stats) and np.array_equal(hsps_r, hsps)
             assert np.array_equal(subj_r, subj) and np.array_equal(hsps_cr, hsps_c)
             assert tot_r == tot and len(hsps_r) == tot["n_hsp"]
+            assert np.array_equal(gst_r, gst) and np.array_equal(gh_r, gh)
     finally:
         db.close()
