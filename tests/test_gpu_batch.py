@@ -32,9 +32,10 @@ def setup():
     ctx.close()
 
 
-def oracle(spec, res, offs):
-    _, h = oracle_lib.OracleQuery.from_spec(spec).scan(res, offs, threads=8)
-    return h.view(np.int32).reshape(-1, 8)
+def oracle(spec, res, offs, counters=False):
+    c, h = oracle_lib.OracleQuery.from_spec(spec).scan(res, offs, threads=8)
+    h = h.view(np.int32).reshape(-1, 8)
+    return (h, c) if counters else h
 
 
 @pytest.mark.parametrize("group_residues", [1024, 2048])
@@ -45,15 +46,19 @@ def test_batch_equals_alone(setup, group_residues):
     b.run()
     info = b.info()
     assert info["n_groups"] < len(specs) and info["n_classes"] >= 2
-    total = 0
+    total = hits = ext = 0
     first = {}
     for qid, spec in zip(ids, specs):
         got = b.results(qid).view(np.int32).reshape(-1, 8)
-        exp = oracle(spec, res, offs)
+        exp, cnt = oracle(spec, res, offs, counters=True)
         assert np.array_equal(got, exp), (qid, len(got), len(exp))
         first[qid] = got.copy()
         total += len(got)
+        hits += int(cnt[:, 0].sum(dtype=np.uint64))
+        ext += int(cnt[:, 1].sum(dtype=np.uint64))
     assert total == info["hsps"] and total > 0
+    # the launch's totals (summed over the wavefronts' per-lane counters)
+    assert info["lookup_hits"] == hits and info["extensions"] == ext
     b.run()   # graph replay
     for qid in ids:
         assert np.array_equal(b.results(qid).view(np.int32).reshape(-1, 8), first[qid])
