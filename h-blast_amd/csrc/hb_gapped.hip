@@ -288,16 +288,22 @@ struct GapWave {
             const int8_t *mrow = mat + 32 * lane_read(rows, (a - 1) & 63);
             int32_t last, adv, rafter;
             if (end - lo <= 64) {
-                // the whole band in one step (most rows)
+                // the whole band in one step (most rows).  Every lane loads
+                // and stores (no exec-mask branches): the column is clamped
+                // into the subject, and the lanes past the band write
+                // (kNeg, kNeg) to ring slots of columns that are rewritten
+                // before the band reaches them (the band is under 64
+                // columns, so those slots are not the band's)
                 const int32_t b = lo + lane;
                 const bool valid = b < end;
                 const int32_t col = (valid && b >= 1) ? b : 1;
                 const int32_t bl = REV ? subj[so + 1 - col] : subj[so + col];
+                const int32_t sc = mrow[bl];
                 const int2 cell = ring[b & rmask];
                 const int32_t ob = valid ? cell.x : kNeg;
                 const int32_t ov = valid ? cell.y : kNeg;
                 const int32_t pb = from_prev_lane(ob, kNeg);
-                const int32_t T = valid ? max(pb + mrow[bl], ov) : kNeg;
+                const int32_t T = valid ? max(pb + sc, ov) : kNeg;
                 uint32_t uA = (uint32_t)(T + lane_ge) + kBias;
                 uint32_t uB = (uint32_t)T + kBias;
                 scan_max2(uA, uB);
@@ -307,9 +313,7 @@ struct GapWave {
                 const int32_t s = max(T, R);
                 const int32_t tb = lane ? max(top, (int32_t)(eB - kBias)) : top;
                 const bool live = valid && !(tb - s > X);
-                if (valid)
-                    ring[b & rmask] =
-                        live ? make_int2(s, max(s - goe, ov - ge)) : make_int2(kNeg, ov);
+                ring[b & rmask] = live ? make_int2(s, max(s - goe, ov - ge)) : make_int2(kNeg, ov);
                 const uint64_t lm = __ballot(live);
                 if (!lm) break;   // every column dropped
                 adv = (int32_t)__builtin_ctzll(lm);
@@ -332,11 +336,12 @@ struct GapWave {
                     const bool valid = b < end;
                     const int32_t col = (valid && b >= 1) ? b : 1;
                     const int32_t bl = REV ? subj[so + 1 - col] : subj[so + col];
+                    const int32_t sc = mrow[bl];   // loaded by every lane (clamped column)
                     const int2 cell = ring[b & rmask];
                     const int32_t ob = valid ? cell.x : kNeg;
                     const int32_t ov = valid ? cell.y : kNeg;
                     const int32_t pb = from_prev_lane(ob, carry);
-                    const int32_t T = valid ? max(pb + mrow[bl], ov) : kNeg;
+                    const int32_t T = valid ? max(pb + sc, ov) : kNeg;
                     uint32_t uA = (uint32_t)(T + lane_ge) + kBias;
                     uint32_t uB = (uint32_t)T + kBias;
                     scan_max2(uA, uB);
