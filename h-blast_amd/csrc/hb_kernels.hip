@@ -75,7 +75,10 @@ __device__ __forceinline__ uint32_t lane63(uint32_t v)
     return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return lane63(wave_incl_add(v)); }
+// Lane mask of a compare, and a lane's bit of a mask (no instructions when
+// the mask is used as the lane predicate).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
 // ---------------------------------------------------------------- residues
 __global__ void check_residues_kernel(const uint8_t *res, uint64_t n, uint32_t *flag)
@@ -499,12 +502,17 @@ struct WaveScan {
     //   diff >= window     -> last = s
     //   diff < 3           -> skip
     //   otherwise          -> extension (diagonal pending until it ran)
-    __device__ __forceinline__ bool decide_pass(bool active, uint32_t q, int32_t so, uint32_t d,
-                                                int32_t sbase, uint32_t sl)
+    // The set and the outcomes travel as lane masks: a vote on one compare is
+    // the compare's own result, masks combine in scalar instructions, and a
+    // lane's predicate is its bit (inverse_ballot), whereas a vote on a
+    // combined predicate costs two VALU instructions (0/1 select and compare).
+    __device__ __forceinline__ uint64_t decide_pass(uint64_t am, uint32_t q, int32_t so, uint32_t d,
+                                                    int32_t sbase, uint32_t sl)
     {
         // d is a valid index in every lane, active or not
+        const bool active = lane_bit(am);
         const uint32_t mk = (uint32_t)(HB_WAVE - lane);
-        bool win;
+        uint64_t mwin;
         uint32_t st, m = 0;
         if constexpr (G) {
             // markers in the wavefront's LDS table, hashed by diagonal (hits of
@@ -519,31 +527,34 @@ struct WaveScan {
             }
             wave_lds_sync();
             if (active) m = mkt[slot];
-            win = active && m == mk;
+            mwin = ballot(m == mk) & am;
             st = (uint32_t)w0 & W::kSM;
         } else {
             word_t old = 0;
             if (active) old = w_max(d, (word_t)mk << W::kMS);
             sync();
             const word_t wv = w_ld(d);
-            win = active && (uint32_t)(wv >> W::kMS) == mk;
+            mwin = ballot((uint32_t)(wv >> W::kMS) == mk) & am;
             // the first hit to reach the word found no marker and holds the
             // state (the max cleared the state bits for the hits after it); a
             // winner that came later gets it from that hit through the word
-            const bool first = active && (uint32_t)(old >> W::kMS) == 0;
+            const uint64_t mfirst = ballot((uint32_t)(old >> W::kMS) == 0) & am;
             st = (uint32_t)old & W::kSM;
-            if (__ballot(win && !first)) {
-                if (first && !win) w_or(d, old);
+            if (mwin & ~mfirst) {
+                if (lane_bit(mfirst & ~mwin)) w_or(d, old);
                 sync();
-                if (win && !first) st = (uint32_t)w_ld(d) & W::kSM;
+                if (lane_bit(mwin & ~mfirst)) st = (uint32_t)w_ld(d) & W::kSM;
             }
         }
+        const bool win = lane_bit(mwin);
         const int32_t sb = so + sbase;
         // s - last on an idle diagonal; negative (below -kP + sb) when the
         // diagonal is pending or flagged
         const int32_t diff = (int32_t)((uint32_t)sb - st);
-        const bool job = win && diff >= 3 && diff < window;
-        const bool wait = win && st == W::kP;
+        // 3 <= diff < window (window > 3)
+        const uint64_t mj = ballot((uint32_t)(diff - 3) < (uint32_t)(window - 3)) & mwin;
+        const uint64_t mwait = ballot(st == W::kP) & mwin;
+        const bool job = lane_bit(mj);
         // the hit becomes last (and clears the flag) on an idle diagonal whose
         // last hit is a window or more behind, or on a flagged one it reached
         const bool set = diff >= window || (uint32_t)(diff + (int32_t)W::kF) < W::kP;
@@ -569,26 +580,26 @@ struct WaveScan {
             st2 = (uint32_t)w_ld(d) & W::kSM;
         }
         const int32_t diff2 = (int32_t)((uint32_t)sb - st2);
-        const bool lose = active && !win;
-        const bool lwait = lose && st2 == W::kP;
-        const bool carry = lose && (diff2 >= 3 || (uint32_t)(diff2 + (int32_t)W::kF) < W::kP);
-        const uint64_t mj = __ballot(job), mb = __ballot(wait || lwait);
+        const uint64_t mlose = am & ~mwin;
+        const uint64_t mb = mwait | (ballot(st2 == W::kP) & mlose);
+        const uint64_t mcarry =
+            (ballot(diff2 >= 3) | ballot((uint32_t)(diff2 + (int32_t)W::kF) < W::kP)) & mlose;
         const uint32_t tso = (uint32_t)so | (sl << 28);
         if (job)
             jobs[(j_head + n_jobs + lane_rank(mj)) & (kJobCap - 1)] =
                 make_uint2(q | ((uint32_t)diff << 24), tso);
-        if (wait || lwait) blk[n_blk + lane_rank(mb)] = make_uint2(q, tso);
+        if (lane_bit(mb)) blk[n_blk + lane_rank(mb)] = make_uint2(q, tso);
         n_jobs += (uint32_t)__popcll(mj);
         n_blk += (uint32_t)__popcll(mb);
         sync();
-        return carry;
+        return mcarry;
     }
 
     // Decides a set of hits completely (passes until no hit is left).
-    __device__ __forceinline__ void decide(bool active, uint32_t q, int32_t so, uint32_t d,
+    __device__ __forceinline__ void decide(uint64_t am, uint32_t q, int32_t so, uint32_t d,
                                            int32_t sbase, uint32_t sl)
     {
-        while (__ballot(active)) active = decide_pass(active, q, so, d, sbase, sl);
+        while (am) am = decide_pass(am, q, so, d, sbase, sl);
     }
 
     // Runs the (up to 64) oldest queued extensions, one per lane, writes HSPs
@@ -669,7 +680,7 @@ struct WaveScan {
             const uint32_t sl = e.y >> 28;
             const int32_t sbase = v ? (int32_t)sla[sl].z : 0;
             sync();
-            decide(v, e.x, so, diag_of(e.x, so), sbase, sl);
+            decide(ballot(v), e.x, so, diag_of(e.x, so), sbase, sl);
         }
     }
 
@@ -793,15 +804,14 @@ struct WaveScan {
         }
     }
 
-    // Moves the hits that lost a pass (lost = true) to lanes 0..n-1 in lane
+    // Moves the hits that lost a pass (mask ml) to lanes 0..n-1 in lane
     // order; returns n.  Their values travel in cq/cs/cd.
-    __device__ __forceinline__ uint32_t carry_losers(bool lost, uint32_t q, int32_t so, uint32_t d,
+    __device__ __forceinline__ uint32_t carry_losers(uint64_t ml, uint32_t q, int32_t so, uint32_t d,
                                                      uint32_t &cq, int32_t &cs, uint32_t &cd)
     {
-        const uint64_t ml = __ballot(lost);
         const uint32_t n = (uint32_t)__popcll(ml);
         if (n) {
-            if (lost) seg[lane_rank(ml)] = (uint32_t)lane;   // source lane of each rank
+            if (lane_bit(ml)) seg[lane_rank(ml)] = (uint32_t)lane;   // source lane of each rank
             sync();
             const int src = (uint32_t)lane < n ? (int)(seg[lane] & 63u) : lane;
             cq = (uint32_t)__shfl((int)q, src, HB_WAVE);
@@ -983,12 +993,12 @@ struct WaveScan {
             int32_t dn = (int32_t)qn - smp;
             if (dn < 0) dn += D;
             const bool isc = (uint32_t)lane < c;
-            const bool isnew = (uint32_t)lane >= c && (uint32_t)lane < c + m;
             const uint32_t q = isc ? cq : qn;
             const int32_t so = isc ? cs_ : son;
             const uint32_t d = isc ? cd : (uint32_t)dn;
-            const bool lost = decide_pass(isc || isnew, q, so, d, sbase, sl);
-            c = carry_losers(lost, q, so, d, cq, cs_, cd);
+            // the carried hits and the new ones: lanes 0 .. c + m - 1
+            const uint64_t am = c + m >= (uint32_t)HB_WAVE ? ~0ull : (1ull << (c + m)) - 1;
+            c = carry_losers(decide_pass(am, q, so, d, sbase, sl), q, so, d, cq, cs_, cd);
         }
         return hits;
     }
