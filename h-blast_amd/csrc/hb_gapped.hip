@@ -115,6 +115,13 @@ __device__ __forceinline__ int32_t lane_read(int32_t v, int32_t lane)
 
 __device__ __forceinline__ int32_t uniform(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Lane mask of one compare, and a lane's bit of a mask.  Votes on single
+// compares cost nothing beyond the compare, and masks combine in scalar
+// instructions; a vote on a combined predicate costs a 0/1 select and a
+// compare.
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
 // Stores of one lane become visible to the other lanes of its wavefront
 // (LDS: wavefront scope; the wide variant's global scratch: workgroup scope,
 // which waits for the stores to complete).
@@ -312,9 +319,10 @@ struct GapWave {
                 const int32_t R = (int32_t)(eA - kBias) - goe - lane_ge + ge;
                 const int32_t s = max(T, R);
                 const int32_t tb = lane ? max(top, (int32_t)(eB - kBias)) : top;
-                const bool live = valid && !(tb - s > X);
+                const uint64_t vm = ballot(valid);
+                const uint64_t lm = ballot(tb - s <= X) & vm;
+                const bool live = lane_bit(lm);
                 ring[b & rmask] = live ? make_int2(s, max(s - goe, ov - ge)) : make_int2(kNeg, ov);
-                const uint64_t lm = __ballot(live);
                 if (!lm) break;   // every column dropped
                 adv = (int32_t)__builtin_ctzll(lm);
                 last = lo + 63 - (int32_t)__builtin_clzll(lm);
@@ -322,7 +330,7 @@ struct GapWave {
                 if (cmax > top) {
                     top = cmax;
                     abest = a;
-                    bbest = lo + (int32_t)__builtin_ctzll(__ballot(valid && s == cmax));
+                    bbest = lo + (int32_t)__builtin_ctzll(ballot(s == cmax) & vm);
                 }
                 rafter = lane_read(max(T - goe, R - ge), end - lo - 1);
             } else {
@@ -352,11 +360,12 @@ struct GapWave {
                     const int32_t s = max(T, R);
                     const int32_t before = lane ? max(rin, (int32_t)(eB - kBias)) : kNeg;
                     const int32_t tb = max(topc, before);
-                    const bool live = valid && !(tb - s > X);
+                    const uint64_t vm = ballot(valid);
+                    const uint64_t lm = ballot(tb - s <= X) & vm;
+                    const bool live = lane_bit(lm);
                     if (valid)
                         ring[b & rmask] =
                             live ? make_int2(s, max(s - goe, ov - ge)) : make_int2(kNeg, ov);
-                    const uint64_t lm = __ballot(live);
                     const int32_t nvalid = end - c0 < 64 ? end - c0 : 64;
                     if (lead) {
                         if (lm) {
@@ -371,7 +380,7 @@ struct GapWave {
                         max(rin, (int32_t)((uint32_t)lane_read((int32_t)uB, 63) - kBias));
                     if (cmax > topc) {
                         topc = cmax;
-                        rowarg = c0 + (int32_t)__builtin_ctzll(__ballot(valid && s == cmax));
+                        rowarg = c0 + (int32_t)__builtin_ctzll(ballot(s == cmax) & vm);
                     }
                     const int32_t rnext = max(T - goe, R - ge);   // R after column b
                     rin = lane_read(rnext, 63);
