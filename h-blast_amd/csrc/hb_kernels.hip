@@ -479,23 +479,23 @@ struct WaveScan {
         base = window;
     }
 
-    // One pass of two-hit decisions over a set of hits (at most one per
-    // lane; active lanes are in subject order by lane).  Among the hits of
-    // the set that share a diagonal only the lowest lane decides: each hit
-    // raises the marker bits of its diagonal's word to 64 - lane with an
-    // atomic max (which returns the state to the first hit to arrive and
-    // clears the state bits), the hit that finds its own marker after the
-    // pass's sync decides, and its store of the new state clears the
-    // marker.  The winner is usually the first to arrive; when it is not,
-    // the first hit ORs the state back into the word for it.  A winner whose diagonal waits on a pending extension joins
-    // the blocked list.  The losers then read the winner's new state: a
-    // loser whose diagonal is now pending joins the blocked list behind the
-    // winner, and a loser that would skip against that state is done (a skip
-    // changes nothing, and skipping is monotone along a diagonal: a hit
-    // skips a flagged state iff s < last, an unflagged one iff s - last < 3,
-    // so every earlier loser of the diagonal skips too).  Returns true for
-    // the other losers: they decide in a later pass, before any later hit of
-    // their diagonal.
+    // One pass of two-hit decisions over a set of hits (lane mask am, at
+    // most one hit per lane; the set's lanes are in subject order).  Among
+    // the hits of the set that share a diagonal only the lowest lane
+    // decides: each hit raises the marker bits of its diagonal's word to
+    // 64 - lane with an atomic max (which returns the state to the first hit
+    // to arrive and clears the state bits), the hit that finds its own marker
+    // after the pass's sync decides, and its store of the new state clears
+    // the marker.  The winner is usually the first to arrive; when it is
+    // not, the first hit ORs the state back into the word for it.  A winner
+    // whose diagonal waits on a pending extension joins the blocked list.
+    // The losers then read the winner's new state: a loser whose diagonal is
+    // now pending joins the blocked list behind the winner, and a loser that
+    // would skip against that state is done (a skip changes nothing, and
+    // skipping is monotone along a diagonal: a hit skips a flagged state iff
+    // s < last, an unflagged one iff s - last < 3, so every earlier loser of
+    // the diagonal skips too).  Returns the mask of the other losers: they
+    // decide in a later pass, before any later hit of their diagonal.
     // Per hit (state per diagonal, flagged = extended to the right):
     //   pending            -> wait
     //   flagged            -> s < last: skip; else last = s, unflag
