@@ -6,8 +6,9 @@ log-normal with a random median and spread up to 12,000 residues, planted
 homologs, runs of one residue and short tandem repeats), neighbourhood threshold, two-hit window,
 X-drop and cutoff, and compares the plain path with the oracle bit-exactly
 (HSP records and every subject's counters).  The batched form is checked
-the same way on a mix of such queries.  Sizes keep the oracle at a fraction
-of a second per case.
+the same way on a mix of such queries, and the gapped stage on further
+random cases with random gap costs, X-drop and cutoffs.  Sizes keep the
+oracle at a fraction of a second per case.
 """
 import numpy as np
 import pytest
@@ -126,3 +127,17 @@ def test_random_batch_matches_oracle(ctx):
     finally:
         b.close()
         db.close()
+
+
+@pytest.mark.parametrize("i", range(40))
+def test_random_gapped_stage_matches_oracle(ctx, i):
+    """The gapped stage on the HSPs of random scans, with random gap open
+    (4-15), gap extension (1-3), X-drop (0-80) and per-context cutoffs."""
+    from test_gpu_gapped import run_both
+    spec, res, offs, _, _ = random_case(1000 + i)
+    rng = np.random.default_rng(7900 + i)
+    go, ge, x = int(rng.integers(4, 16)), int(rng.integers(1, 4)), int(rng.integers(0, 81))
+    cut = rng.integers(25, 90, size=len(spec.ctx_off)).tolist()
+    print(f"gapped case {i}: query {len(spec.seq)} residues, {len(offs) - 1} subjects, "
+          f"open {go} extend {ge} X {x}", flush=True)
+    run_both(ctx, spec, res, offs, cut, go, ge, x)
