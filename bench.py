@@ -329,7 +329,8 @@ def measure_served(ctx, db, specs, arrivals, max_batch_residues, max_inflight, c
     for i, h in kept.items():
         _, ref = hbseed.search(ctx, db, specs[i][1])
         ok = ok and np.array_equal(h.view(np.int32), ref.view(np.int32))
-    log(f"served results equal plain path on {len(kept)} sampled queries: {ok}")
+    if kept:
+        log(f"served results equal plain path on {len(kept)} sampled queries: {ok}")
     n_done = st["completed"] - st0["completed"]
     return {"arrivals": n, "blastx": sum(1 for k, _ in specs if k == "blastx"),
             "completed": n_done, "queries_per_s": n_done / wall, "wall_s": wall,
