@@ -423,7 +423,9 @@ struct GapWave {
                     ring[(end + i) & rmask] = make_int2(v, v - goe);
                 }
             }
-            if (guard && lane == 0) ring[new_end & rmask] = make_int2(kNeg, kNeg);
+            // every lane stores the same guard cell (a uniform branch, no
+            // single-lane exec mask)
+            if (guard) ring[new_end & rmask] = make_int2(kNeg, kNeg);
             end = new_end + guard;
             sync_lanes<WIDE>();
         }
