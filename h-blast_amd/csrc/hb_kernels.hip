@@ -805,19 +805,19 @@ struct WaveScan {
     }
 
     // Moves the hits that lost a pass (mask ml) to lanes 0..n-1 in lane
-    // order; returns n.  Their values travel in cq/cs/cd.
+    // order; returns n.  Their values travel in cq/cs/cd: each loser pushes
+    // them to the lane of its rank (ds_permute, every lane active); the
+    // other lanes push to lane 63, which no loser needs (a pass has a
+    // winner, so n <= 63).
     __device__ __forceinline__ uint32_t carry_losers(uint64_t ml, uint32_t q, int32_t so, uint32_t d,
                                                      uint32_t &cq, int32_t &cs, uint32_t &cd)
     {
         const uint32_t n = (uint32_t)__popcll(ml);
         if (n) {
-            if (lane_bit(ml)) seg[lane_rank(ml)] = (uint32_t)lane;   // source lane of each rank
-            sync();
-            const int src = (uint32_t)lane < n ? (int)(seg[lane] & 63u) : lane;
-            cq = (uint32_t)__shfl((int)q, src, HB_WAVE);
-            cs = __shfl(so, src, HB_WAVE);
-            cd = (uint32_t)__shfl((int)d, src, HB_WAVE);
-            sync();
+            const int a4 = lane_bit(ml) ? (int)(lane_rank(ml) << 2) : 63 * 4;
+            cq = (uint32_t)__builtin_amdgcn_ds_permute(a4, (int)q);
+            cs = __builtin_amdgcn_ds_permute(a4, so);
+            cd = (uint32_t)__builtin_amdgcn_ds_permute(a4, (int)d);
         }
         return n;
     }
