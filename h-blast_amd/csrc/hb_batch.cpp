@@ -167,9 +167,26 @@ static void size_items(hb_batch *b, uint32_t n_tiles)
     }
 }
 
-// Splits the (diagonal-sorted) groups into classes: a new class starts when
-// the diagonal array would grow past twice the class's first one, so that
-// one launch's LDS layout does not waste occupancy on smaller groups.
+// Wavefronts per CU that the scan reaches with one class's LDS layout
+// (0 when no configuration fits).
+static int class_waves(hb_batch *b, int32_t qlen_pad, int32_t ctx_pad, int32_t diag_len)
+{
+    const int place = hb_state_choose(b->ctx, diag_len, hb_scan_fixed_lds(qlen_pad, ctx_pad),
+                                      b->db->max_len, b->p.window);
+    hb_launch_cfg cfg;
+    if (hb_scan_config(b->ctx, true, qlen_pad, ctx_pad, diag_len, 1u << 30, &cfg, place,
+                       b->g_ring, b->g_waves) != HB_OK)
+        return 0;
+    return cfg.blocks / std::max(b->ctx->n_cu, 1) * cfg.waves;
+}
+
+// Splits the (diagonal-sorted) groups into classes, so that one launch's
+// LDS layout, sized for its largest group, does not cost its other groups
+// occupancy: a new class starts when the diagonal array would grow past
+// twice the class's first one, or when a group would lower the wavefronts
+// per CU the class reaches with its first group (packed groups sit just
+// below the group size, and a class that also took the longest queries
+// would run all of them at the occupancy of the longest).
 static int build(hb_batch *b, int32_t target)
 {
     free_groups(b);
@@ -189,10 +206,19 @@ static int build(hb_batch *b, int32_t target)
         Class c;
         c.g_begin = g;
         const int32_t lim = std::max(diag[g] * 2, 1024);
+        int w0 = -1;
         while (g < ng && diag[g] <= lim) {
-            c.qlen_pad_max = std::max(c.qlen_pad_max, b->gs.desc[g].qlen_pad);
-            c.ctx_pad_max = std::max(c.ctx_pad_max, hb_ctx_pad(b->gs.desc[g].n_ctx));
-            c.diag_len_max = std::max(c.diag_len_max, diag[g]);
+            const int32_t qp = std::max(c.qlen_pad_max, b->gs.desc[g].qlen_pad);
+            const int32_t cp = std::max(c.ctx_pad_max, hb_ctx_pad(b->gs.desc[g].n_ctx));
+            const int32_t dl = std::max(c.diag_len_max, diag[g]);
+            if (w0 < 0) {
+                w0 = class_waves(b, qp, cp, dl);
+            } else if (qp != c.qlen_pad_max || cp != c.ctx_pad_max || dl != c.diag_len_max) {
+                if (class_waves(b, qp, cp, dl) < w0) break;
+            }
+            c.qlen_pad_max = qp;
+            c.ctx_pad_max = cp;
+            c.diag_len_max = dl;
             ++g;
         }
         c.g_end = g;
